@@ -1,0 +1,176 @@
+"""Benchmark of the LD hot path on one MI355X.
+
+Default workload: one region of 30,000 SNPs x 16,622 haploid samples
+(seeded haplotype-block Markov data), all-pairs r^2 over the lower triangle
+(449,985,000 pairs), packed table resident in HBM.  A step = one full pass
+(contraction + fused r^2 epilogue, FP32 r^2 tiles written to HBM).
+
+Prints ONE JSON line (rank 0):
+  value         pairs / second of the whole step, Gpairs/s
+  roofline      dominant kernel's achieved rate vs the MI355X peak of the
+                unit it is bound by, measured with HIP events on the engine's
+                stream
+  cpu_baseline  the oracle's C/OpenMP restatement of the reference's BLIS CPU
+                path (blocked AND+POPCNT + get_pairwise_ld_score epilogue),
+                timed on a bounded sample of the same workload on this host
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = ("SNP-pair r²/sec (Gpairs/s) at 16,622 samples; batched vs plain GPU, "
+          "p99 latency")
+
+# MI355X peaks used as roofline denominators.
+PEAK = {
+    "valu_int32": 39.3e12,   # 256 CU x 64 lanes x 2.4 GHz (int32 VALU ops/s)
+    "mfma_fp4": 10.1e15,     # dense fp4 MFMA
+    "mfma_i8": 5.0e15,       # dense int8 MFMA
+    "hbm": 8.0e12,
+}
+DTYPE = {"popcount": "u64", "mfma_i8": "int8", "mfma_fp4": "fp4"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def roofline(engine, pairs, words, samples, kernel_s):
+    if engine == "popcount":
+        ops = pairs * 4.0 * words          # 2x v_and_b32 + 2x v_bcnt per word pair
+        ach = ops / kernel_s
+        return {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK["valu_int32"] / 1e12,
+                "unit": "Tops/s (int32 VALU)", "frac": ach / PEAK["valu_int32"],
+                "traffic": None, "kernel": "ld_popcount_kernel",
+                "ops_per_pair": 4 * words, "kernel_ms": kernel_s * 1e3}
+    flops = pairs * 2.0 * samples
+    ach = flops / kernel_s
+    pk = PEAK[engine]
+    return {"bound": "mfma", "achieved": ach / 1e12, "peak": pk / 1e12,
+            "unit": "TFLOP/s", "frac": ach / pk, "traffic": None,
+            "flops_per_pair": 2 * samples, "kernel_ms": kernel_s * 1e3}
+
+
+def cpu_baseline(A, bc, N, budget_s=10.0):
+    """Time the oracle's CPU restatement on the first S SNPs (triangle)."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+    def run(S):
+        sub = np.ascontiguousarray(A[:S])
+        t0 = time.perf_counter()
+        c = oracle.counts_blocked(sub, tri=True, nthreads=threads)
+        oracle.r2_reference(bc[:S], bc[:S], c, N, tri=True)
+        return time.perf_counter() - t0
+
+    S = min(3000, A.shape[0])
+    t = run(S)
+    S2 = int(min(A.shape[0], S * (budget_s / max(t, 1e-3)) ** 0.5))
+    if S2 > S:
+        S = S2
+        t = run(S)
+    pairs = S * (S - 1) / 2
+    return {"value": pairs / t / 1e9, "unit": "Gpairs/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {S} SNPs x {N} samples, lower triangle "
+                      f"({int(pairs)} pairs), counts + r^2, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--snps", type=int, default=30000)
+    ap.add_argument("--samples", type=int, default=16622)
+    ap.add_argument("--engine", default="auto")
+    ap.add_argument("--r2-mode", default="reference")
+    ap.add_argument("--seed", type=int, default=20261203 + 2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(each box has one MI355X; the path would shard by output tiles)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import qld_amd
+
+    N, m = args.samples, args.snps
+    log(f"[bench] generating {m} SNPs x {N} samples (seed {args.seed})")
+    A, bc = qld_amd.synth_blocks(args.seed, m, N)
+    W = A.shape[1]
+    pairs = m * (m - 1) // 2
+
+    eng = qld_amd.Engine(engine=args.engine, r2_mode=args.r2_mode, timing=True)
+    dA = eng.alloc(A.nbytes).upload(A)
+    dbc = eng.alloc(bc.nbytes).upload(bc)
+    dres = eng.alloc(m * m * 4)
+
+    def step():
+        eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    eng.sync()
+    engine = eng.last_engine
+
+    log(f"[bench] timing {args.steps} steps, engine {engine}")
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.timer_start()
+    for _ in range(args.steps):
+        step()
+    dev_ms = eng.timer_stop()
+    eng.sync()
+    wall_s = time.perf_counter() - t0
+    ms_per_step = wall_s * 1e3 / args.steps
+
+    # Per-launch duration of the contraction kernel (events around it alone).
+    kms = []
+    for _ in range(max(3, min(args.steps, 10))):
+        step()
+        kms.append(eng.last_kernel_ms())
+    kernel_s = float(np.mean(kms)) / 1e3
+    log(f"[bench] {ms_per_step:.3f} ms/step wall, {dev_ms / args.steps:.3f} ms/step "
+        f"device, kernel {kernel_s * 1e3:.3f} ms")
+
+    value = pairs / (ms_per_step / 1e3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gpairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE.get(engine, engine),
+        "data": f"synthetic haplotype-block Markov table, seed {args.seed}",
+        "config": {"workload": f"single region all-pairs r2, {m} SNPs x {N} haploid "
+                               "samples, lower triangle, FP32 r2 tiles in HBM",
+                   "snps": m, "samples": N, "words": W, "pairs": pairs,
+                   "engine": engine, "r2_mode": args.r2_mode,
+                   "parallelism": "1 GPU"},
+        "device_ms_per_step": round(dev_ms / args.steps, 4),
+        "roofline": roofline(engine, pairs, W, N, kernel_s),
+    }
+    for b in (dA, dbc, dres):
+        b.free()
+    eng.close()
+
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle restatement of the BLIS path)")
+        out["cpu_baseline"] = cpu_baseline(A, bc, N)
+        out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

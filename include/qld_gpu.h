@@ -1,0 +1,133 @@
+/*
+ * qld_gpu.h -- C ABI of the MI355X LD engine (libqld_hip.so).
+ *
+ * The engine replaces the OpenCL GPU path of qLD-compute: the pairwise
+ * AND+popcount "GEMM" over packed haplotype rows and the host-side r^2
+ * epilogue.  All entry points are plain C: pointers, sizes and int status
+ * codes; no HIP or C++ types cross this boundary.  No entry point calls
+ * exit(); every failure is reported as a negative status and a message
+ * retrievable with qld_gpu_last_error().
+ *
+ * Data layout shared with the reference (include/header.h:60-62,
+ * include/correlate_MEM.h:66-75 of the reference):
+ *   table    row-major uint64 words, one SNP per row, `words` = ceil(N/64)
+ *            words per row, sample s at bit 63-(s%64) of word s/64, the
+ *            low bits of the last word zero (snip_processing.c:783-815)
+ *   bc       uint32 popcount of each row (the MDF "BC" column)
+ *   results  float, results[i*ldr + j] = r^2(A_j, B_i): row i indexes table
+ *            B, column j indexes table A (gemm_cpu.c:353-363)
+ *   counts   uint32, same indexing as results: popcount(A_j & B_i)
+ * Triangular mode (A == B, posWset2 == 0 in the reference) defines the cells
+ * j > i only -- exactly the cells writeResults prints
+ * (correlate_IO.c:403-406); other cells are left untouched.
+ */
+#ifndef QLD_GPU_H
+#define QLD_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define QLD_OK            0
+#define QLD_ERR_INVALID  (-1)   /* bad argument or shape                  */
+#define QLD_ERR_NOMEM    (-2)   /* device or host allocation failed       */
+#define QLD_ERR_DEVICE   (-3)   /* HIP runtime / launch error             */
+#define QLD_ERR_BUDGET   (-4)   /* request exceeds the HBM budget         */
+#define QLD_ERR_NOTREADY (-5)   /* asynchronous result not ready yet      */
+#define QLD_ERR_NOTFOUND (-6)   /* unknown handle / request id            */
+
+/* ---- contraction engines --------------------------------------------- */
+#define QLD_ENGINE_AUTO      0  /* pick per shape (autotuned table)       */
+#define QLD_ENGINE_POPCOUNT  1  /* VALU AND + v_bcnt over packed words    */
+#define QLD_ENGINE_MFMA_I8   2  /* int8 MFMA over unpacked 0/1 bytes      */
+#define QLD_ENGINE_MFMA_FP4  3  /* fp4 (e2m1) MFMA over unpacked 0/1      */
+
+/* ---- r^2 epilogue modes ------------------------------------------------ */
+/* REFERENCE: bit-identical to get_pairwise_ld_score (gemm_cpu.c:343-404)
+ *            as the reference's own build compiles it (float p's, float
+ *            numerator, double denominator and division).
+ * EXACT:     integer-exact D*N^2 = c*N - a*b, double ratio, rounded once
+ *            to float (<= ~1 float ulp from the FP64 closed form).
+ * Both keep the reference's special cases: a or b == 0 -> 0; a or b == N ->
+ * NaN (never printed); r^2 > 1.0001 -> 123.456. */
+#define QLD_R2_REFERENCE 0
+#define QLD_R2_EXACT     1
+
+typedef struct qld_ctx qld_ctx;
+
+typedef struct qld_cfg {
+    int    device;      /* HIP device ordinal                              */
+    int    engine;      /* QLD_ENGINE_*                                    */
+    int    r2_mode;     /* QLD_R2_*                                        */
+    int    timing;      /* nonzero: record HIP events around each kernel   */
+    size_t hbm_budget;  /* bytes the context may allocate; 0 = 90% of free */
+} qld_cfg;
+
+/* Fill *cfg with defaults (device 0, AUTO engine, REFERENCE r^2). */
+void qld_cfg_default(qld_cfg *cfg);
+
+/* Replaces gpu_init() (gemm_gpu.h:52, gemm_gpu.c:298-447): binds the
+ * device, creates the context's stream and workspace.  Thread-safe; one
+ * context per calling thread is the intended use. */
+int qld_gpu_init(qld_ctx **ctx, const qld_cfg *cfg);
+
+/* Replaces gpu_release() (gemm_gpu.h:54, gemm_gpu.c:886-906 frees). */
+void qld_gpu_release(qld_ctx *ctx);
+
+/* Message of the last failure on this context ("" if none); ctx may be
+ * NULL for failures of qld_gpu_init itself. */
+const char *qld_gpu_last_error(const qld_ctx *ctx);
+
+/* Replaces the compute part of correlate_gpu() (gemm_gpu.h:56-70,
+ * gemm_gpu.c:753-845: gpu_gemm + get_pairwise_ld_score_gpu).  Host
+ * buffers, synchronous.  A: m x words, B: n x words (B may equal A);
+ * triangular requires A == B content and m == n.  results (n*m floats) and
+ * counts (n*m uint32) are each optional (NULL = not wanted). */
+int qld_gpu_correlate(qld_ctx *ctx,
+                      const uint64_t *A, const uint32_t *bcA, int m,
+                      const uint64_t *B, const uint32_t *bcB, int n,
+                      int words, int snp_size, int triangular,
+                      float *results, uint32_t *counts);
+
+/* ---- device-resident interface ----------------------------------------- */
+/* Device memory owned by the context (counted against hbm_budget). */
+int qld_dev_alloc(qld_ctx *ctx, size_t bytes, void **dptr);
+int qld_dev_free(qld_ctx *ctx, void *dptr);
+int qld_h2d(qld_ctx *ctx, void *dst, const void *src, size_t bytes);
+int qld_d2h(qld_ctx *ctx, void *dst, const void *src, size_t bytes);
+int qld_dev_fill(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32);
+/* Wait for all work queued on the context's stream. */
+int qld_sync(qld_ctx *ctx);
+
+/* Asynchronous on the context's stream: same computation as
+ * qld_gpu_correlate on device buffers.  ldr / ldc are the row pitches (in
+ * elements) of results / counts; either output may be NULL. */
+int qld_correlate_dev(qld_ctx *ctx,
+                      const uint64_t *dA, const uint32_t *dbcA, int m,
+                      const uint64_t *dB, const uint32_t *dbcB, int n,
+                      int words, int snp_size, int triangular,
+                      float *dresults, int64_t ldr,
+                      uint32_t *dcounts, int64_t ldc);
+
+/* Milliseconds of device time of the kernels queued by the last
+ * qld_correlate_dev / qld_gpu_correlate call (needs cfg.timing); waits for
+ * them.  Mirrors the reference's per-phase GEMM timer (gemm_gpu.c:823-829). */
+int qld_last_kernel_ms(qld_ctx *ctx, float *ms);
+
+/* Device-time stopwatch over a region of the context's stream: start
+ * records a HIP event on the stream, stop records a second one, waits for
+ * it and returns the elapsed milliseconds. */
+int qld_timer_start(qld_ctx *ctx);
+int qld_timer_stop(qld_ctx *ctx, float *ms);
+
+/* Engine the last call used (QLD_ENGINE_*), after AUTO resolution. */
+int qld_last_engine(const qld_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,300 @@
+"""MI355X-native LD engine for qLD-compute (Python glue over the C ABI).
+
+The product is the pair of native libraries built in ``qld_amd/lib``:
+``libqld_hip.so`` (HIP kernels behind ``include/qld_gpu.h``) and
+``libqld_host.so`` (C host side, ``include/qld_host.h``).  This module only
+loads them with ctypes for tests, benchmarks and scripting; it contains no
+compute path of its own, and every call fails loudly when the HIP library
+is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(PKG, "lib")
+HIP_LIB = os.path.join(LIBDIR, "libqld_hip.so")
+HOST_LIB = os.path.join(LIBDIR, "libqld_host.so")
+
+QLD_OK = 0
+QLD_ERR_INVALID = -1
+QLD_ERR_NOMEM = -2
+QLD_ERR_DEVICE = -3
+QLD_ERR_BUDGET = -4
+QLD_ERR_NOTREADY = -5
+QLD_ERR_NOTFOUND = -6
+
+ENGINES = {"auto": 0, "popcount": 1, "mfma_i8": 2, "mfma_fp4": 3}
+ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
+R2_MODES = {"reference": 0, "exact": 1}
+
+
+class QldError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"qld error {code}: {msg}")
+        self.code = code
+
+
+class Cfg(C.Structure):
+    _fields_ = [("device", C.c_int), ("engine", C.c_int), ("r2_mode", C.c_int),
+                ("timing", C.c_int), ("hbm_budget", C.c_size_t)]
+
+
+_u64p = C.POINTER(C.c_uint64)
+_u32p = C.POINTER(C.c_uint32)
+_f32p = C.POINTER(C.c_float)
+_vp = C.c_void_p
+
+# (name, restype, argtypes) of every symbol include/qld_gpu.h declares.
+GPU_SYMBOLS = [
+    ("qld_cfg_default", None, [C.POINTER(Cfg)]),
+    ("qld_gpu_init", C.c_int, [C.POINTER(_vp), C.POINTER(Cfg)]),
+    ("qld_gpu_release", None, [_vp]),
+    ("qld_gpu_last_error", C.c_char_p, [_vp]),
+    ("qld_gpu_correlate", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int,
+                                    C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    ("qld_dev_alloc", C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    ("qld_dev_free", C.c_int, [_vp, _vp]),
+    ("qld_h2d", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    ("qld_d2h", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    ("qld_dev_fill", C.c_int, [_vp, _vp, C.c_uint32, C.c_size_t]),
+    ("qld_sync", C.c_int, [_vp]),
+    ("qld_correlate_dev", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int,
+                                    C.c_int, C.c_int, C.c_int, _vp, C.c_int64,
+                                    _vp, C.c_int64]),
+    ("qld_last_kernel_ms", C.c_int, [_vp, _f32p]),
+    ("qld_timer_start", C.c_int, [_vp]),
+    ("qld_timer_stop", C.c_int, [_vp, _f32p]),
+    ("qld_last_engine", C.c_int, [_vp]),
+]
+
+HOST_SYMBOLS = [
+    ("qld_format_float", C.c_int, [C.c_char_p, C.c_float, C.c_int]),
+    ("qld_write_results_header", None, [_vp]),
+    ("qld_write_results", C.c_long, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
+                                     C.c_int, C.c_int, _vp, C.c_float, C.c_int]),
+    ("qld_correlate_report", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
+                                       _vp, _vp, _vp, _vp, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_float]),
+    ("qld_words", C.c_int, [C.c_int]),
+    ("qld_synth_blocks", None, [C.c_uint64, C.c_int, C.c_int, C.c_double,
+                                C.c_double, C.c_double, C.c_double, _vp, _vp,
+                                C.c_int]),
+    ("qld_synth_regions", None, [C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, _vp, _vp, _vp, _vp,
+                                 C.c_int]),
+    ("qld_bitcounts", None, [_vp, C.c_int, C.c_int, _vp, C.c_int]),
+]
+
+_hip = None
+_host = None
+
+
+def _bind(lib, table):
+    for name, res, args in table:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def hip_lib() -> C.CDLL:
+    """The HIP engine library; raises if it has not been built."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise QldError(QLD_ERR_DEVICE,
+                           f"{HIP_LIB} is missing: run `python -m qld_amd.build` "
+                           "(there is no CPU fallback)")
+        lib = C.CDLL(HIP_LIB, mode=C.RTLD_GLOBAL)
+        _bind(lib, GPU_SYMBOLS)
+        _hip = lib
+    return _hip
+
+
+def host_lib() -> C.CDLL:
+    global _host
+    if _host is None:
+        hip_lib()
+        if not os.path.exists(HOST_LIB):
+            raise QldError(QLD_ERR_DEVICE, f"{HOST_LIB} is missing: run "
+                           "`python -m qld_amd.build`")
+        lib = C.CDLL(HOST_LIB)
+        _bind(lib, HOST_SYMBOLS)
+        _host = lib
+    return _host
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def words_for(n_samples: int) -> int:
+    return (n_samples + 63) // 64
+
+
+# ---- synthetic data (host C generator) ------------------------------------
+
+def synth_blocks(seed: int, n_snps: int, n_samples: int, mean_block=50.0,
+                 maf_lo=0.01, maf_hi=0.5, copy_prob=0.9, nthreads=0):
+    """Haplotype-block Markov table: (table[n_snps, W] uint64, bc uint32)."""
+    W = words_for(n_samples)
+    table = np.empty((n_snps, W), dtype=np.uint64)
+    bc = np.empty(n_snps, dtype=np.uint32)
+    host_lib().qld_synth_blocks(seed, n_snps, n_samples, mean_block, maf_lo,
+                                maf_hi, copy_prob, _ptr(table), _ptr(bc), nthreads)
+    return table, bc
+
+
+def synth_regions(seed: int, m_a: int, n_b: int, n_samples: int,
+                  mean_block=50.0, maf_lo=0.005, maf_hi=0.5, copy_prob=0.9,
+                  copy_frac=0.05, flip_prob=0.02, nthreads=0):
+    W = words_for(n_samples)
+    ta = np.empty((m_a, W), dtype=np.uint64)
+    tb = np.empty((n_b, W), dtype=np.uint64)
+    ba = np.empty(m_a, dtype=np.uint32)
+    bb = np.empty(n_b, dtype=np.uint32)
+    host_lib().qld_synth_regions(seed, m_a, n_b, n_samples, mean_block, maf_lo,
+                                 maf_hi, copy_prob, copy_frac, flip_prob,
+                                 _ptr(ta), _ptr(ba), _ptr(tb), _ptr(bb), nthreads)
+    return ta, ba, tb, bb
+
+
+def format_float(val: float, precision: int = 9) -> str:
+    buf = C.create_string_buffer(64)
+    n = host_lib().qld_format_float(buf, C.c_float(val), precision)
+    return buf.raw[:n].decode()
+
+
+class DevBuffer:
+    """Device memory owned by an Engine context."""
+
+    def __init__(self, eng: "Engine", nbytes: int):
+        self.eng = eng
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        eng._check(hip_lib().qld_dev_alloc(eng._ctx, self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+
+    def upload(self, arr: np.ndarray) -> "DevBuffer":
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.eng._check(hip_lib().qld_h2d(self.eng._ctx, self.ptr, _ptr(arr), arr.nbytes))
+        return self
+
+    def download(self, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.eng._check(hip_lib().qld_d2h(self.eng._ctx, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def fill(self, value_u32: int = 0):
+        self.eng._check(hip_lib().qld_dev_fill(self.eng._ctx, self.ptr, value_u32,
+                                               self.nbytes // 4))
+        return self
+
+    def free(self):
+        if self.ptr:
+            self.eng._check(hip_lib().qld_dev_free(self.eng._ctx, self.ptr))
+            self.ptr = 0
+
+
+class Engine:
+    """One qld_ctx: a device, a stream, an engine choice and an r^2 mode."""
+
+    def __init__(self, device=0, engine="auto", r2_mode="reference",
+                 timing=False, hbm_budget=0):
+        lib = hip_lib()
+        cfg = Cfg()
+        lib.qld_cfg_default(C.byref(cfg))
+        cfg.device = device
+        cfg.engine = ENGINES[engine] if isinstance(engine, str) else int(engine)
+        cfg.r2_mode = R2_MODES[r2_mode] if isinstance(r2_mode, str) else int(r2_mode)
+        cfg.timing = 1 if timing else 0
+        cfg.hbm_budget = hbm_budget
+        ctx = C.c_void_p()
+        rc = lib.qld_gpu_init(C.byref(ctx), C.byref(cfg))
+        if rc != QLD_OK:
+            raise QldError(rc, lib.qld_gpu_last_error(None).decode())
+        self._ctx = ctx
+        self.cfg = cfg
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            hip_lib().qld_gpu_release(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != QLD_OK:
+            raise QldError(rc, hip_lib().qld_gpu_last_error(self._ctx).decode())
+        return rc
+
+    @property
+    def last_engine(self) -> str:
+        return ENGINE_NAMES[hip_lib().qld_last_engine(self._ctx)]
+
+    def correlate(self, A, bcA, B=None, bcB=None, snp_size=None,
+                  triangular=None, results=True, counts=False):
+        """Host-buffer all-pairs r^2 (qld_gpu_correlate)."""
+        A = np.ascontiguousarray(A, dtype=np.uint64)
+        bcA = np.ascontiguousarray(bcA, dtype=np.uint32)
+        same = B is None
+        if same:
+            B, bcB = A, bcA
+        B = np.ascontiguousarray(B, dtype=np.uint64)
+        bcB = np.ascontiguousarray(bcB, dtype=np.uint32)
+        m, W = A.shape
+        n = B.shape[0]
+        assert B.shape[1] == W
+        if snp_size is None:
+            snp_size = W * 64
+        if triangular is None:
+            triangular = same
+        res = np.empty((n, m), dtype=np.float32) if results else None
+        cnt = np.empty((n, m), dtype=np.uint32) if counts else None
+        self._check(hip_lib().qld_gpu_correlate(
+            self._ctx, _ptr(A), _ptr(bcA), m, _ptr(B), _ptr(bcB), n, W,
+            int(snp_size), 1 if triangular else 0, _ptr(res), _ptr(cnt)))
+        return res, cnt
+
+    def alloc(self, nbytes: int) -> DevBuffer:
+        return DevBuffer(self, nbytes)
+
+    def correlate_dev(self, dA, dbcA, m, dB, dbcB, n, words, snp_size,
+                      triangular, dres=None, ldr=0, dcnt=None, ldc=0):
+        def p(x):
+            return None if x is None else (x.ptr if isinstance(x, DevBuffer) else int(x))
+        self._check(hip_lib().qld_correlate_dev(
+            self._ctx, p(dA), p(dbcA), m, p(dB), p(dbcB), n, words, snp_size,
+            1 if triangular else 0, p(dres), ldr or m, p(dcnt), ldc or m))
+
+    def sync(self):
+        self._check(hip_lib().qld_sync(self._ctx))
+
+    def timer_start(self):
+        self._check(hip_lib().qld_timer_start(self._ctx))
+
+    def timer_stop(self) -> float:
+        ms = C.c_float()
+        self._check(hip_lib().qld_timer_stop(self._ctx, C.byref(ms)))
+        return ms.value
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        self._check(hip_lib().qld_last_kernel_ms(self._ctx, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,42 @@
+// The context object behind the opaque qld_ctx* of the C ABI.
+#pragma once
+
+#include "qld_internal.h"
+
+#include <new>
+#include <unordered_map>
+
+struct qld_ctx;
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t size = 0;
+    qld_ctx *owner = nullptr;
+    int reserve(qld_ctx *ctx, size_t bytes);
+    void release();
+};
+
+struct qld_ctx {
+    qld_cfg cfg;
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;     // last call's kernels
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;   // qld_timer_start/stop
+    bool timed = false;
+    int last_engine = QLD_ENGINE_AUTO;
+    size_t budget = 0, used = 0;
+    std::unordered_map<void *, size_t> allocs;   // qld_dev_alloc'd blocks
+    DevBuf host_stage;   // staging for the host-buffer entry point
+    DevBuf ws;           // engine workspace (unpacked operands)
+    char err[512] = {0};
+};
+
+void qld_set_error(qld_ctx *ctx, const char *fmt, ...);
+int qld_hip_fail(qld_ctx *ctx, hipError_t e, const char *what);
+int qld_ctx_malloc(qld_ctx *ctx, size_t bytes, void **p);
+void qld_ctx_mfree(qld_ctx *ctx, void *p, size_t bytes);
+int qld_fill_async(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32);
+int qld_run_job(qld_ctx *ctx, const LdJob &job);
+int qld_pick_engine(const qld_ctx *ctx, const LdJob &job);

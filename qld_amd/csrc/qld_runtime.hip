@@ -1,0 +1,418 @@
+// C-ABI runtime of the LD engine: contexts, device memory, dispatch.
+//
+// Replaces the process-global OpenCL state of the reference
+// (gemm_gpu.h:36-50, gemm_gpu.c:298-447) with a per-caller context that owns
+// one HIP stream, its allocations and an error string.  Nothing here calls
+// exit(); every failure becomes a negative status.
+#include "qld_internal.h"
+#include "qld_ctx.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+static char g_init_error[256];
+
+void qld_set_error(qld_ctx *ctx, const char *fmt, ...)
+{
+    char *dst = ctx ? ctx->err : g_init_error;
+    size_t cap = ctx ? sizeof(ctx->err) : sizeof(g_init_error);
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(dst, cap, fmt, ap);
+    va_end(ap);
+}
+
+int qld_hip_fail(qld_ctx *ctx, hipError_t e, const char *what)
+{
+    qld_set_error(ctx, "%s: %s", what, hipGetErrorString(e));
+    return QLD_ERR_DEVICE;
+}
+
+extern "C" void qld_cfg_default(qld_cfg *cfg)
+{
+    if (!cfg) return;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->device = 0;
+    cfg->engine = QLD_ENGINE_AUTO;
+    cfg->r2_mode = QLD_R2_REFERENCE;
+    cfg->timing = 0;
+    cfg->hbm_budget = 0;
+}
+
+extern "C" int qld_gpu_init(qld_ctx **out, const qld_cfg *cfg)
+{
+    if (!out) {
+        qld_set_error(nullptr, "qld_gpu_init: out is NULL");
+        return QLD_ERR_INVALID;
+    }
+    *out = nullptr;
+    qld_cfg c;
+    if (cfg) c = *cfg; else qld_cfg_default(&c);
+    if (c.engine < QLD_ENGINE_AUTO || c.engine > QLD_ENGINE_MFMA_FP4 ||
+        (c.r2_mode != QLD_R2_REFERENCE && c.r2_mode != QLD_R2_EXACT)) {
+        qld_set_error(nullptr, "qld_gpu_init: bad engine %d or r2_mode %d",
+                      c.engine, c.r2_mode);
+        return QLD_ERR_INVALID;
+    }
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        qld_set_error(nullptr, "qld_gpu_init: no HIP device (%s)",
+                      hipGetErrorString(e));
+        return QLD_ERR_DEVICE;
+    }
+    if (c.device < 0 || c.device >= ndev) {
+        qld_set_error(nullptr, "qld_gpu_init: device %d of %d", c.device, ndev);
+        return QLD_ERR_INVALID;
+    }
+    if ((e = hipSetDevice(c.device)) != hipSuccess) {
+        qld_set_error(nullptr, "hipSetDevice: %s", hipGetErrorString(e));
+        return QLD_ERR_DEVICE;
+    }
+    qld_ctx *ctx = new (std::nothrow) qld_ctx();
+    if (!ctx) {
+        qld_set_error(nullptr, "qld_gpu_init: out of host memory");
+        return QLD_ERR_NOMEM;
+    }
+    ctx->cfg = c;
+    ctx->device = c.device;
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
+        qld_set_error(nullptr, "hipStreamCreate: %s", hipGetErrorString(e));
+        delete ctx;
+        return QLD_ERR_DEVICE;
+    }
+    if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->tev0)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->tev1)) != hipSuccess) {
+        qld_set_error(nullptr, "hipEventCreate: %s", hipGetErrorString(e));
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return QLD_ERR_DEVICE;
+    }
+    size_t freeb = 0, totalb = 0;
+    hipMemGetInfo(&freeb, &totalb);
+    ctx->budget = c.hbm_budget ? c.hbm_budget : (size_t)(0.9 * (double)freeb);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c.device) == hipSuccess)
+        ctx->num_cu = prop.multiProcessorCount;
+    *out = ctx;
+    return QLD_OK;
+}
+
+extern "C" void qld_gpu_release(qld_ctx *ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->allocs) hipFree(kv.first);
+    ctx->allocs.clear();
+    ctx->ws.release();
+    ctx->host_stage.release();
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipEventDestroy(ctx->tev0);
+    hipEventDestroy(ctx->tev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *qld_gpu_last_error(const qld_ctx *ctx)
+{
+    return ctx ? ctx->err : g_init_error;
+}
+
+// ---- memory ---------------------------------------------------------------
+
+int qld_ctx_malloc(qld_ctx *ctx, size_t bytes, void **p)
+{
+    if (ctx->used + bytes > ctx->budget) {
+        qld_set_error(ctx, "allocation of %zu bytes exceeds HBM budget "
+                      "(%zu of %zu used)", bytes, ctx->used, ctx->budget);
+        return QLD_ERR_BUDGET;
+    }
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        qld_set_error(ctx, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        return QLD_ERR_NOMEM;
+    }
+    ctx->used += bytes;
+    return QLD_OK;
+}
+
+void qld_ctx_mfree(qld_ctx *ctx, void *p, size_t bytes)
+{
+    if (!p) return;
+    hipFree(p);
+    ctx->used -= bytes;
+}
+
+int DevBuf::reserve(qld_ctx *ctx, size_t bytes)
+{
+    if (bytes <= size) return QLD_OK;
+    if (ptr) {
+        hipStreamSynchronize(ctx->stream);
+        qld_ctx_mfree(ctx, ptr, size);
+        ptr = nullptr;
+        size = 0;
+    }
+    owner = ctx;
+    int rc = qld_ctx_malloc(ctx, bytes, &ptr);
+    if (rc == QLD_OK) size = bytes;
+    return rc;
+}
+
+void DevBuf::release()
+{
+    if (ptr && owner) qld_ctx_mfree(owner, ptr, size);
+    ptr = nullptr;
+    size = 0;
+}
+
+extern "C" int qld_dev_alloc(qld_ctx *ctx, size_t bytes, void **dptr)
+{
+    if (!ctx || !dptr) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    int rc = qld_ctx_malloc(ctx, bytes, dptr);
+    if (rc == QLD_OK) ctx->allocs[*dptr] = bytes;
+    return rc;
+}
+
+extern "C" int qld_dev_free(qld_ctx *ctx, void *dptr)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    if (!dptr) return QLD_OK;
+    auto it = ctx->allocs.find(dptr);
+    if (it == ctx->allocs.end()) {
+        qld_set_error(ctx, "qld_dev_free: %p was not allocated by this context", dptr);
+        return QLD_ERR_NOTFOUND;
+    }
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    qld_ctx_mfree(ctx, dptr, it->second);
+    ctx->allocs.erase(it);
+    return QLD_OK;
+}
+
+extern "C" int qld_h2d(qld_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_h2d");
+}
+
+extern "C" int qld_d2h(qld_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_d2h");
+}
+
+__global__ void qld_fill_u32(uint32_t *p, uint32_t v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+int qld_fill_async(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32)
+{
+    if (n_u32 == 0) return QLD_OK;
+    size_t blocks = (n_u32 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(qld_fill_u32, dim3((unsigned)blocks), dim3(256), 0,
+                       ctx->stream, (uint32_t *)dptr, value, n_u32);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "fill kernel");
+}
+
+extern "C" int qld_dev_fill(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32)
+{
+    if (!ctx || (!dptr && n_u32)) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    return qld_fill_async(ctx, dptr, value, n_u32);
+}
+
+extern "C" int qld_sync(qld_ctx *ctx)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_sync");
+}
+
+// ---- all-pairs dispatch ------------------------------------------------------
+
+static int check_job(qld_ctx *ctx, const LdJob &j)
+{
+    if (j.m < 0 || j.n < 0 || j.words <= 0) {
+        qld_set_error(ctx, "bad shape m=%d n=%d words=%d", j.m, j.n, j.words);
+        return QLD_ERR_INVALID;
+    }
+    if (j.snp_size <= 0 || j.snp_size >= QLD_MAX_SNP_SIZE ||
+        (int64_t)j.words * 64 < (int64_t)j.snp_size - 63) {
+        qld_set_error(ctx, "bad snp_size %d for %d words", j.snp_size, j.words);
+        return QLD_ERR_INVALID;
+    }
+    if (j.tri && j.m != j.n) {
+        qld_set_error(ctx, "triangular mode needs m == n (m=%d n=%d)", j.m, j.n);
+        return QLD_ERR_INVALID;
+    }
+    if ((j.m && (!j.A || !j.bcA)) || (j.n && (!j.B || !j.bcB))) {
+        qld_set_error(ctx, "NULL table or bit-count pointer");
+        return QLD_ERR_INVALID;
+    }
+    if ((j.res && j.ldr < j.m) || (j.cnt && j.ldc < j.m)) {
+        qld_set_error(ctx, "output pitch smaller than m");
+        return QLD_ERR_INVALID;
+    }
+    return QLD_OK;
+}
+
+int qld_pick_engine(const qld_ctx *ctx, const LdJob &j)
+{
+    (void)j;
+    if (ctx->cfg.engine != QLD_ENGINE_AUTO) return ctx->cfg.engine;
+    return QLD_ENGINE_POPCOUNT;
+}
+
+int qld_run_job(qld_ctx *ctx, const LdJob &job)
+{
+    int rc = check_job(ctx, job);
+    if (rc) return rc;
+    if (job.m == 0 || job.n == 0 || (!job.res && !job.cnt)) return QLD_OK;
+    int engine = qld_pick_engine(ctx, job);
+    if (ctx->cfg.timing) hipEventRecord(ctx->ev0, ctx->stream);
+    hipError_t e = hipSuccess;
+    switch (engine) {
+    case QLD_ENGINE_POPCOUNT:
+        e = launch_popcount(job, ctx->cfg.r2_mode, ctx->stream);
+        break;
+    default:
+        qld_set_error(ctx, "engine %d not available", engine);
+        return QLD_ERR_INVALID;
+    }
+    if (e != hipSuccess) return qld_hip_fail(ctx, e, "kernel launch");
+    if (ctx->cfg.timing) hipEventRecord(ctx->ev1, ctx->stream);
+    ctx->last_engine = engine;
+    ctx->timed = ctx->cfg.timing != 0;
+    return QLD_OK;
+}
+
+extern "C" int qld_correlate_dev(qld_ctx *ctx,
+                                 const uint64_t *dA, const uint32_t *dbcA, int m,
+                                 const uint64_t *dB, const uint32_t *dbcB, int n,
+                                 int words, int snp_size, int triangular,
+                                 float *dresults, int64_t ldr,
+                                 uint32_t *dcounts, int64_t ldc)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    LdJob j{dA, dbcA, m, dB, dbcB, n, words, snp_size, triangular ? 1 : 0,
+            dresults, ldr, dcounts, ldc};
+    return qld_run_job(ctx, j);
+}
+
+extern "C" int qld_gpu_correlate(qld_ctx *ctx,
+                                 const uint64_t *A, const uint32_t *bcA, int m,
+                                 const uint64_t *B, const uint32_t *bcB, int n,
+                                 int words, int snp_size, int triangular,
+                                 float *results, uint32_t *counts)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    LdJob probe{A, bcA, m, B, bcB, n, words, snp_size, triangular ? 1 : 0,
+                results, m, counts, m};
+    int rc = check_job(ctx, probe);
+    if (rc) return rc;
+    if (m == 0 || n == 0 || (!results && !counts)) return QLD_OK;
+
+    const bool same = (A == B) && (bcA == bcB) && (m == n);
+    const size_t ta = (size_t)m * words * 8, tb = same ? 0 : (size_t)n * words * 8;
+    const size_t cells = (size_t)n * m;
+    const size_t align = 256;
+    auto up = [&](size_t x) { return (x + align - 1) / align * align; };
+    size_t off_a = 0, off_b = off_a + up(ta), off_bca = off_b + up(tb),
+           off_bcb = off_bca + up((size_t)m * 4),
+           off_res = off_bcb + up(same ? 0 : (size_t)n * 4),
+           off_cnt = off_res + up(results ? cells * 4 : 0),
+           total = off_cnt + up(counts ? cells * 4 : 0);
+    if ((rc = ctx->host_stage.reserve(ctx, total))) return rc;
+    char *base = (char *)ctx->host_stage.ptr;
+    uint64_t *dA = (uint64_t *)(base + off_a);
+    uint64_t *dB = same ? dA : (uint64_t *)(base + off_b);
+    uint32_t *dbcA = (uint32_t *)(base + off_bca);
+    uint32_t *dbcB = same ? dbcA : (uint32_t *)(base + off_bcb);
+    float *dres = results ? (float *)(base + off_res) : nullptr;
+    uint32_t *dcnt = counts ? (uint32_t *)(base + off_cnt) : nullptr;
+
+    hipError_t e;
+    hipStream_t s = ctx->stream;
+    if ((e = hipMemcpyAsync(dA, A, ta, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(dbcA, bcA, (size_t)m * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return qld_hip_fail(ctx, e, "H2D");
+    if (!same &&
+        ((e = hipMemcpyAsync(dB, B, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+         (e = hipMemcpyAsync(dbcB, bcB, (size_t)n * 4, hipMemcpyHostToDevice, s)) != hipSuccess))
+        return qld_hip_fail(ctx, e, "H2D");
+    // Cells the triangular launch does not define come back as 0, as the
+    // reference zeroes its results before filling them (gemm_gpu.c:808-812).
+    if (triangular) {
+        if (dres && (rc = qld_fill_async(ctx, dres, 0u, cells))) return rc;
+        if (dcnt && (rc = qld_fill_async(ctx, dcnt, 0u, cells))) return rc;
+    }
+    LdJob j{dA, dbcA, m, dB, dbcB, n, words, snp_size, triangular ? 1 : 0,
+            dres, m, dcnt, m};
+    if ((rc = qld_run_job(ctx, j))) return rc;
+    if (results &&
+        (e = hipMemcpyAsync(results, dres, cells * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return qld_hip_fail(ctx, e, "D2H");
+    if (counts &&
+        (e = hipMemcpyAsync(counts, dcnt, cells * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return qld_hip_fail(ctx, e, "D2H");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+        return qld_hip_fail(ctx, e, "qld_gpu_correlate");
+    return QLD_OK;
+}
+
+extern "C" int qld_last_kernel_ms(qld_ctx *ctx, float *ms)
+{
+    if (!ctx || !ms) return QLD_ERR_INVALID;
+    if (!ctx->timed) {
+        qld_set_error(ctx, "no timed call (set cfg.timing)");
+        return QLD_ERR_NOTREADY;
+    }
+    hipSetDevice(ctx->device);
+    hipError_t e = hipEventSynchronize(ctx->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_last_kernel_ms");
+}
+
+extern "C" int qld_timer_start(qld_ctx *ctx)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipError_t e = hipEventRecord(ctx->tev0, ctx->stream);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_timer_start");
+}
+
+extern "C" int qld_timer_stop(qld_ctx *ctx, float *ms)
+{
+    if (!ctx || !ms) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipError_t e = hipEventRecord(ctx->tev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->tev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->tev0, ctx->tev1);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_timer_stop");
+}
+
+extern "C" int qld_last_engine(const qld_ctx *ctx)
+{
+    return ctx ? ctx->last_engine : QLD_ENGINE_AUTO;
+}

@@ -1,0 +1,198 @@
+"""Parity of the HIP engine with the oracle and the reference's golden
+fixtures, through the C ABI.  Needs an MI355X.
+
+Bar: counts bit-exact; REFERENCE-mode r^2 bit-identical to the reference's
+get_pairwise_ld_score; EXACT-mode r^2 bit-identical to the oracle's exact
+restatement; both within 1e-6 absolute of the FP64 closed form."""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import qld_amd
+from conftest import golden_cases, load_case
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = ["popcount"]
+TOL_FP64 = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _sel(n, m, tri):
+    if tri:
+        return np.triu(np.ones((n, m), dtype=bool), 1)
+    return np.ones((n, m), dtype=bool)
+
+
+def _check(res, cnt, A, bcA, B, bcB, N, tri, mode, ref_counts=None):
+    n, m = res.shape
+    sel = _sel(n, m, tri)
+    want_c = ref_counts if ref_counts is not None else oracle.counts_naive(A, B)
+    assert (cnt[sel] == want_c[sel]).all(), "counts differ"
+    f = oracle.r2_reference if mode == "reference" else oracle.r2_exact
+    want = f(bcA, bcB, want_c, N)
+    got_bits, want_bits = res.view(np.uint32)[sel], want.view(np.uint32)[sel]
+    bad = np.nonzero(got_bits != want_bits)[0]
+    assert bad.size == 0, f"{bad.size} r2 cells differ, first {res[sel][bad[:3]]} vs {want[sel][bad[:3]]}"
+    r64 = oracle.r2_fp64(bcA, bcB, want_c, N)[sel]
+    ok = np.isfinite(r64)
+    if ok.any():
+        assert np.abs(res[sel][ok].astype(np.float64) - r64[ok]).max() <= TOL_FP64
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("path", golden_cases(), ids=lambda p: os.path.basename(p))
+def test_golden_cases(engine, path, engine_factory):
+    d = load_case(path)
+    say("golden", d["name"], engine)
+    N = int(d["snp_size"])
+    tri = int(d["posWset2"]) == 0
+    e = engine_factory(engine=engine, r2_mode="reference")
+    if tri:
+        res, cnt = e.correlate(d["A"], d["bcA"], snp_size=N, triangular=True, counts=True)
+    else:
+        res, cnt = e.correlate(d["A"], d["bcA"], d["B"], d["bcB"], snp_size=N,
+                               triangular=False, counts=True)
+    sel = _sel(*res.shape, tri)
+    assert (cnt[sel] == d["counts"][sel]).all()
+    assert (res.view(np.uint32)[sel] == d["r2"].view(np.uint32)[sel]).all()
+    if tri:   # cells outside the triangle come back zeroed
+        assert (res[~sel] == 0).all() and (cnt[~sel] == 0).all()
+
+
+@pytest.mark.parametrize("path", golden_cases(), ids=lambda p: os.path.basename(p))
+def test_dropin_report_text(path, tmp_path, engine_factory):
+    """qld_correlate_report (the correlate_gpu drop-in) writes the reference's
+    report byte for byte."""
+    d = load_case(path)
+    say("report", d["name"])
+    e = engine_factory(engine="auto", r2_mode="reference")
+    lib = qld_amd.host_lib()
+    libc = oracle.libc()
+    m, n = len(d["posA"]), len(d["posB"])
+    tri = int(d["posWset2"]) == 0
+    ida = (C.c_char_p * m)(*[s.encode() for s in d["idA"]])
+    idb = (C.c_char_p * n)(*[s.encode() for s in d["idB"]])
+    A = np.ascontiguousarray(d["A"])
+    B = A if tri else np.ascontiguousarray(d["B"])
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    posB = d["posA"] if tri else d["posB"]
+    bcB = d["bcA"] if tri else d["bcB"]
+    out = tmp_path / "r.txt"
+    fp = libc.fopen(str(out).encode(), b"w")
+    rc = lib.qld_correlate_report(e._ctx, fp, p(A), p(d["posA"]), C.cast(ida, C.c_void_p),
+                                  p(d["bcA"]), m, p(B), p(np.ascontiguousarray(posB)),
+                                  C.cast(idb if not tri else ida, C.c_void_p),
+                                  p(np.ascontiguousarray(bcB)), n, A.shape[1],
+                                  int(d["snp_size"]), int(d["posWset2"]),
+                                  C.c_float(float(d["r2limit"])))
+    libc.fclose(fp)
+    assert rc == 0, qld_amd.hip_lib().qld_gpu_last_error(e._ctx)
+    assert out.read_text() == d["report"]
+
+
+SHAPES_TRI = [1, 2, 127, 128, 129, 300]
+SHAPES_RECT = [(1, 1), (5, 300), (129, 127), (300, 257)]
+SAMPLES = [1, 63, 64, 65, 96, 130, 16622, 100000]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+@pytest.mark.parametrize("N", SAMPLES)
+def test_shapes_triangular(engine, mode, N, engine_factory):
+    e = engine_factory(engine=engine, r2_mode=mode)
+    for m in SHAPES_TRI:
+        say("tri", engine, mode, N, m)
+        A, bc = qld_amd.synth_blocks(1000 + m + N, m, N)
+        res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+        _check(res, cnt, A, bc, A, bc, N, True, mode)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+@pytest.mark.parametrize("N", SAMPLES)
+def test_shapes_rectangular(engine, mode, N, engine_factory):
+    e = engine_factory(engine=engine, r2_mode=mode)
+    for m, n in SHAPES_RECT:
+        say("rect", engine, mode, N, m, n)
+        A, bA, B, bB = qld_amd.synth_regions(7 + m * n + N, m, n, N)
+        res, cnt = e.correlate(A, bA, B, bB, snp_size=N, triangular=False, counts=True)
+        _check(res, cnt, A, bA, B, bB, N, False, mode)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_monomorphic_and_duplicate_rows(engine, engine_factory):
+    N = 16622
+    A, bc = qld_amd.synth_blocks(77, 200, N)
+    A[10] = 0
+    A[11] = A[12]
+    W = A.shape[1]
+    A[13] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    A[13, W - 1] = np.uint64(((1 << (N % 64)) - 1) << (64 - N % 64))
+    A[14] = A[15] ^ A[13]                                  # complement
+    bc = np.array([int(sum(bin(int(w)).count("1") for w in r)) for r in A], np.uint32)
+    for mode in ("reference", "exact"):
+        e = engine_factory(engine=engine, r2_mode=mode)
+        res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+        _check(res, cnt, A, bc, A, bc, N, True, mode)
+        assert res[10, 11] == 0.0 and np.isnan(res[13, 20])
+        assert res[11, 12] == pytest.approx(1.0, abs=1e-6)
+        assert res[14, 15] == pytest.approx(1.0, abs=1e-6)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_full_size_sampled_rows(engine, engine_factory):
+    """30,000 SNPs x 16,622 samples (the single-region benchmark shape):
+    device-resident run, then 64 random rows checked against the oracle."""
+    N, m = 16622, 30000
+    A, bc = qld_amd.synth_blocks(20261203 + 2, m, N)
+    W = A.shape[1]
+    e = engine_factory(engine=engine, r2_mode="reference")
+    dA = e.alloc(A.nbytes).upload(A)
+    dbc = e.alloc(bc.nbytes).upload(bc)
+    dres = e.alloc(m * m * 4)
+    dcnt = e.alloc(m * m * 4)
+    say("full size", engine)
+    e.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m, dcnt, m)
+    e.sync()
+    rows = np.sort(np.random.default_rng(3).choice(m, 64, replace=False))
+    lib = qld_amd.hip_lib()
+    for i in rows:
+        r = np.empty(m, np.float32)
+        c = np.empty(m, np.uint32)
+        assert lib.qld_d2h(e._ctx, r.ctypes.data_as(C.c_void_p), dres.ptr + i * m * 4, m * 4) == 0
+        assert lib.qld_d2h(e._ctx, c.ctypes.data_as(C.c_void_p), dcnt.ptr + i * m * 4, m * 4) == 0
+        want_c = oracle.counts_naive(A, A[i:i + 1])[0]
+        j = np.arange(i + 1, m)
+        assert (c[j] == want_c[j]).all()
+        want = oracle.r2_reference(bc, bc[i:i + 1], want_c[None, :], N)[0]
+        assert (r[j].view(np.uint32) == want[j].view(np.uint32)).all()
+    for b in (dA, dbc, dres, dcnt):
+        b.free()
+
+
+def test_errors_are_status_codes(engine_factory):
+    e = engine_factory(engine="popcount")
+    A, bc = qld_amd.synth_blocks(1, 10, 100)
+    B, bB = qld_amd.synth_blocks(2, 11, 100)
+    with pytest.raises(qld_amd.QldError) as x:
+        e.correlate(A, bc, B, bB, snp_size=100, triangular=True)
+    assert x.value.code == qld_amd.QLD_ERR_INVALID
+    with pytest.raises(qld_amd.QldError) as x:
+        e.correlate(A, bc, snp_size=64 * A.shape[1] + 64)
+    assert x.value.code == qld_amd.QLD_ERR_INVALID
+    small = engine_factory(engine="popcount", hbm_budget=1 << 10)
+    with pytest.raises(qld_amd.QldError) as x:
+        small.alloc(1 << 20)
+    assert x.value.code == qld_amd.QLD_ERR_BUDGET
+    # the context stays usable after errors
+    res, _ = e.correlate(A, bc, snp_size=100)
+    assert res.shape == (10, 10)
