@@ -168,8 +168,9 @@ def test_full_size_sampled_rows(engine, engine_factory):
     for i in rows:
         r = np.empty(m, np.float32)
         c = np.empty(m, np.uint32)
-        assert lib.qld_d2h(e._ctx, r.ctypes.data_as(C.c_void_p), dres.ptr + i * m * 4, m * 4) == 0
-        assert lib.qld_d2h(e._ctx, c.ctypes.data_as(C.c_void_p), dcnt.ptr + i * m * 4, m * 4) == 0
+        off = int(i) * m * 4
+        assert lib.qld_d2h(e._ctx, r.ctypes.data_as(C.c_void_p), dres.ptr + off, m * 4) == 0
+        assert lib.qld_d2h(e._ctx, c.ctypes.data_as(C.c_void_p), dcnt.ptr + off, m * 4) == 0
         want_c = oracle.counts_naive(A, A[i:i + 1])[0]
         j = np.arange(i + 1, m)
         assert (c[j] == want_c[j]).all()
