@@ -141,7 +141,7 @@ def main():
     kms = []
     for _ in range(max(3, min(args.steps, 10))):
         step()
-        kms.append(eng.last_kernel_ms())
+        kms.append(eng.last_contract_ms())
     kernel_s = float(np.mean(kms)) / 1e3
     log(f"[bench] {ms_per_step:.3f} ms/step wall, {dev_ms / args.steps:.3f} ms/step "
         f"device, kernel {kernel_s * 1e3:.3f} ms")

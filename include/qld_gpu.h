@@ -118,6 +118,10 @@ int qld_correlate_dev(qld_ctx *ctx,
  * them.  Mirrors the reference's per-phase GEMM timer (gemm_gpu.c:823-829). */
 int qld_last_kernel_ms(qld_ctx *ctx, float *ms);
 
+/* Milliseconds of the contraction kernel alone (popcount or MFMA, with
+ * its fused epilogue) of the last call; needs cfg.timing; waits for it. */
+int qld_last_contract_ms(qld_ctx *ctx, float *ms);
+
 /* Device-time stopwatch over a region of the context's stream: start
  * records a HIP event on the stream, stop records a second one, waits for
  * it and returns the elapsed milliseconds. */

@@ -66,6 +66,7 @@ GPU_SYMBOLS = [
                                     C.c_int, C.c_int, C.c_int, _vp, C.c_int64,
                                     _vp, C.c_int64]),
     ("qld_last_kernel_ms", C.c_int, [_vp, _f32p]),
+    ("qld_last_contract_ms", C.c_int, [_vp, _f32p]),
     ("qld_timer_start", C.c_int, [_vp]),
     ("qld_timer_stop", C.c_int, [_vp, _f32p]),
     ("qld_last_engine", C.c_int, [_vp]),
@@ -292,6 +293,11 @@ class Engine:
     def timer_stop(self) -> float:
         ms = C.c_float()
         self._check(hip_lib().qld_timer_stop(self._ctx, C.byref(ms)))
+        return ms.value
+
+    def last_contract_ms(self) -> float:
+        ms = C.c_float()
+        self._check(hip_lib().qld_last_contract_ms(self._ctx, C.byref(ms)))
         return ms.value
 
     def last_kernel_ms(self) -> float:

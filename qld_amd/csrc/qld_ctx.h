@@ -5,6 +5,7 @@
 
 #include <new>
 #include <unordered_map>
+#include <vector>
 
 struct qld_ctx;
 
@@ -23,13 +24,17 @@ struct qld_ctx {
     int num_cu = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;     // last call's kernels
+    hipEvent_t kev0 = nullptr, kev1 = nullptr;   // last call's contraction kernel
     hipEvent_t tev0 = nullptr, tev1 = nullptr;   // qld_timer_start/stop
     bool timed = false;
     int last_engine = QLD_ENGINE_AUTO;
     size_t budget = 0, used = 0;
     std::unordered_map<void *, size_t> allocs;   // qld_dev_alloc'd blocks
     DevBuf host_stage;   // staging for the host-buffer entry point
-    DevBuf ws;           // engine workspace (unpacked operands)
+    DevBuf ws;           // engine workspace (expanded operands)
+    DevBuf tiles;        // device tile list of the MFMA engines
+    std::vector<uint32_t> tiles_host;
+    int tiles_key[3] = {-1, -1, -1};             // (ti_count, tj_count, tri)
     char err[512] = {0};
 };
 

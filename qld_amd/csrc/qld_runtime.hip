@@ -6,6 +6,7 @@
 // exit(); every failure becomes a negative status.
 #include "qld_internal.h"
 #include "qld_ctx.h"
+#include "ld_mfma.h"
 
 #include <stdarg.h>
 #include <stdio.h>
@@ -84,6 +85,8 @@ extern "C" int qld_gpu_init(qld_ctx **out, const qld_cfg *cfg)
     }
     if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess ||
         (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->kev0)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->kev1)) != hipSuccess ||
         (e = hipEventCreate(&ctx->tev0)) != hipSuccess ||
         (e = hipEventCreate(&ctx->tev1)) != hipSuccess) {
         qld_set_error(nullptr, "hipEventCreate: %s", hipGetErrorString(e));
@@ -109,9 +112,12 @@ extern "C" void qld_gpu_release(qld_ctx *ctx)
     for (auto &kv : ctx->allocs) hipFree(kv.first);
     ctx->allocs.clear();
     ctx->ws.release();
+    ctx->tiles.release();
     ctx->host_stage.release();
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
+    hipEventDestroy(ctx->kev0);
+    hipEventDestroy(ctx->kev1);
     hipEventDestroy(ctx->tev0);
     hipEventDestroy(ctx->tev1);
     hipStreamDestroy(ctx->stream);
@@ -282,6 +288,73 @@ int qld_pick_engine(const qld_ctx *ctx, const LdJob &j)
     return QLD_ENGINE_POPCOUNT;
 }
 
+// Tile order of the MFMA launches: 8 x 8 super-tiles, column-major inside,
+// so that the ~32 tiles one XCD runs at a time share 4 column panels and
+// 8 row panels in its L2.  Triangles keep only tiles with tj >= ti.
+static void build_tile_list(std::vector<uint32_t> &out, int ti_count, int tj_count, int tri)
+{
+    const int S = 8;
+    out.clear();
+    const int si = (ti_count + S - 1) / S, sj = (tj_count + S - 1) / S;
+    for (int J = 0; J < sj; ++J)
+        for (int I = 0; I < si; ++I) {
+            if (tri && I > J) continue;
+            for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj)
+                for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti)
+                    if (!tri || ti <= tj)
+                        out.push_back((uint32_t)ti | ((uint32_t)tj << 16));
+        }
+}
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
+{
+    const int T = mfma_tile_edge();
+    const int ti_count = (job.n + T - 1) / T, tj_count = (job.m + T - 1) / T;
+    if (ti_count > 65535 || tj_count > 65535) {
+        qld_set_error(ctx, "too many rows for the MFMA tile list (m=%d n=%d)", job.m, job.n);
+        return QLD_ERR_INVALID;
+    }
+    const int64_t rb = mfma_row_bytes(engine, job.words);
+    const bool same = job.A == job.B && job.m == job.n;
+    const size_t bytesA = align256((size_t)job.m * rb);
+    const size_t bytesB = same ? 0 : align256((size_t)job.n * rb);
+    int rc = ctx->ws.reserve(ctx, bytesA + bytesB);
+    if (rc) return rc;
+    uint8_t *XA = (uint8_t *)ctx->ws.ptr;
+    uint8_t *XB = same ? XA : XA + bytesA;
+
+    if (ctx->tiles_key[0] != ti_count || ctx->tiles_key[1] != tj_count ||
+        ctx->tiles_key[2] != job.tri) {
+        build_tile_list(ctx->tiles_host, ti_count, tj_count, job.tri);
+        size_t tb = ctx->tiles_host.size() * sizeof(uint32_t);
+        if ((rc = ctx->tiles.reserve(ctx, tb ? tb : 4))) return rc;
+        hipError_t e = hipMemcpyAsync(ctx->tiles.ptr, ctx->tiles_host.data(), tb,
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "tile list upload");
+        ctx->tiles_key[0] = ti_count;
+        ctx->tiles_key[1] = tj_count;
+        ctx->tiles_key[2] = job.tri;
+    }
+
+    hipError_t e = launch_expand(engine, job.A, job.m, job.words, XA, ctx->stream);
+    if (e == hipSuccess && !same)
+        e = launch_expand(engine, job.B, job.n, job.words, XB, ctx->stream);
+    if (e != hipSuccess) return qld_hip_fail(ctx, e, "expand kernel");
+    MfmaJob mj;
+    mj.job = job;
+    mj.XA = XA;
+    mj.XB = XB;
+    mj.row_bytes = rb;
+    mj.tiles = (const uint32_t *)ctx->tiles.ptr;
+    mj.ntiles = (int64_t)ctx->tiles_host.size();
+    if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
+    e = launch_mfma(engine, ctx->cfg.r2_mode, mj, ctx->stream);
+    if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "MFMA kernel launch");
+}
+
 int qld_run_job(qld_ctx *ctx, const LdJob &job)
 {
     int rc = check_job(ctx, job);
@@ -289,16 +362,22 @@ int qld_run_job(qld_ctx *ctx, const LdJob &job)
     if (job.m == 0 || job.n == 0 || (!job.res && !job.cnt)) return QLD_OK;
     int engine = qld_pick_engine(ctx, job);
     if (ctx->cfg.timing) hipEventRecord(ctx->ev0, ctx->stream);
-    hipError_t e = hipSuccess;
     switch (engine) {
-    case QLD_ENGINE_POPCOUNT:
-        e = launch_popcount(job, ctx->cfg.r2_mode, ctx->stream);
+    case QLD_ENGINE_POPCOUNT: {
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
+        hipError_t e = launch_popcount(job, ctx->cfg.r2_mode, ctx->stream);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "popcount kernel launch");
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
+        break;
+    }
+    case QLD_ENGINE_MFMA_I8:
+    case QLD_ENGINE_MFMA_FP4:
+        if ((rc = run_mfma(ctx, engine, job))) return rc;
         break;
     default:
         qld_set_error(ctx, "engine %d not available", engine);
         return QLD_ERR_INVALID;
     }
-    if (e != hipSuccess) return qld_hip_fail(ctx, e, "kernel launch");
     if (ctx->cfg.timing) hipEventRecord(ctx->ev1, ctx->stream);
     ctx->last_engine = engine;
     ctx->timed = ctx->cfg.timing != 0;
@@ -392,6 +471,19 @@ extern "C" int qld_last_kernel_ms(qld_ctx *ctx, float *ms)
     hipError_t e = hipEventSynchronize(ctx->ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_last_kernel_ms");
+}
+
+extern "C" int qld_last_contract_ms(qld_ctx *ctx, float *ms)
+{
+    if (!ctx || !ms) return QLD_ERR_INVALID;
+    if (!ctx->timed) {
+        qld_set_error(ctx, "no timed call (set cfg.timing)");
+        return QLD_ERR_NOTREADY;
+    }
+    hipSetDevice(ctx->device);
+    hipError_t e = hipEventSynchronize(ctx->kev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->kev0, ctx->kev1);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_last_contract_ms");
 }
 
 extern "C" int qld_timer_start(qld_ctx *ctx)

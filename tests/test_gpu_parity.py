@@ -17,7 +17,7 @@ from conftest import golden_cases, load_case
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = ["popcount"]
+ENGINES = ["popcount", "mfma_fp4", "mfma_i8"]
 TOL_FP64 = 1e-6
 
 
