@@ -128,8 +128,77 @@ int qld_last_contract_ms(qld_ctx *ctx, float *ms);
 int qld_timer_start(qld_ctx *ctx);
 int qld_timer_stop(qld_ctx *ctx, float *ms);
 
+/* Tile edge of the MFMA engines for plain calls: 256, 128, or 0 to choose
+ * by shape (256 once a launch has a 256-tile for every CU). */
+int qld_set_plain_tile(qld_ctx *ctx, int tile);
+
 /* Engine the last call used (QLD_ENGINE_*), after AUTO resolution. */
 int qld_last_engine(const qld_ctx *ctx);
+
+/* ---- resident tables ------------------------------------------------------ */
+/* A packed table kept in HBM together with its expanded matrix-core operand
+ * form (fp4 or int8, per the context's engine; fp4 for AUTO/POPCOUNT too, so
+ * windows and requests can always run on the matrix cores).  Replaces the
+ * reference's per-task table reload (correlate_function.c:155-228) for
+ * callers that correlate many windows of one chromosome. */
+typedef struct qld_table qld_table;
+
+/* Copy `rows` packed rows and their bit counts from host memory. */
+int qld_table_create(qld_ctx *ctx, const uint64_t *rows_host,
+                     const uint32_t *bc_host, int rows, int words,
+                     int snp_size, qld_table **out);
+/* Same from device buffers (copied). */
+int qld_table_create_dev(qld_ctx *ctx, const uint64_t *rows_dev,
+                         const uint32_t *bc_dev, int rows, int words,
+                         int snp_size, qld_table **out);
+int qld_table_free(qld_ctx *ctx, qld_table *t);
+/* Rows, words and device bytes held by a table. */
+int qld_table_rows(const qld_table *t);
+size_t qld_table_bytes(const qld_table *t);
+
+/* ---- batched windows ------------------------------------------------------ */
+/* One window = rows [start, start + size) of a resident table, correlated
+ * with itself; its r^2 triangle (cells j > i of a size x size row-major
+ * block, pitch `size`) goes to out + out_offset (floats).  This is the
+ * reference's many-task mode (input_list.c:205-323, split_region:17-39),
+ * one correlate() per window. */
+typedef struct qld_window {
+    int32_t start;
+    int32_t size;
+    int64_t out_offset;
+} qld_window;
+
+#define QLD_BATCH_PLAIN      0  /* one kernel launch per window             */
+#define QLD_BATCH_PERSISTENT 1  /* every window's tiles in one persistent
+                                   launch with per-XCD work queues         */
+
+typedef struct qld_plan qld_plan;
+
+/* Prepare a window set: validates windows, picks tile shapes (autotuned
+ * table), uploads job descriptors and the tile work list, and captures the
+ * launch sequence in a HIP graph.  `out` is a device buffer of at least
+ * max(out_offset + size*size) floats. */
+int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
+                     int nwin, float *out, int mode, qld_plan **plan);
+/* Queue one execution of the plan on the context's stream (asynchronous). */
+int qld_plan_run(qld_ctx *ctx, qld_plan *plan);
+/* Replay through the captured HIP graph (nonzero) or by direct launches. */
+int qld_plan_use_graph(qld_plan *plan, int use_graph);
+/* Tiles and pairs the plan computes; kernel launches per run. */
+int qld_plan_stats(const qld_plan *plan, int64_t *tiles, int64_t *pairs,
+                   int *launches);
+int qld_plan_free(qld_ctx *ctx, qld_plan *plan);
+
+/* Tile-shape policy of window plans: windows of at least `cutoff` SNPs use
+ * 256 x 256 tiles, smaller ones 128 x 128 (0 = all large). */
+int qld_set_tile_cutoff(qld_ctx *ctx, int cutoff);
+int qld_get_tile_cutoff(const qld_ctx *ctx);
+/* Autotuner: time the given window set as persistent plans under each
+ * candidate cutoff (`reps` runs each after a warm-up), keep the fastest in
+ * the context and report it.  Writes r^2 into `out` as a plan run does. */
+int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
+                         int nwin, float *out, int reps, int *best_cutoff,
+                         float *best_ms);
 
 #ifdef __cplusplus
 }

@@ -70,7 +70,28 @@ GPU_SYMBOLS = [
     ("qld_timer_start", C.c_int, [_vp]),
     ("qld_timer_stop", C.c_int, [_vp, _f32p]),
     ("qld_last_engine", C.c_int, [_vp]),
+    ("qld_set_plain_tile", C.c_int, [_vp, C.c_int]),
+    ("qld_table_create", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("qld_table_create_dev", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(_vp)]),
+    ("qld_table_free", C.c_int, [_vp, _vp]),
+    ("qld_table_rows", C.c_int, [_vp]),
+    ("qld_table_bytes", C.c_size_t, [_vp]),
+    ("qld_plan_windows", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, C.POINTER(_vp)]),
+    ("qld_plan_run", C.c_int, [_vp, _vp]),
+    ("qld_plan_use_graph", C.c_int, [_vp, C.c_int]),
+    ("qld_plan_stats", C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int)]),
+    ("qld_plan_free", C.c_int, [_vp, _vp]),
+    ("qld_set_tile_cutoff", C.c_int, [_vp, C.c_int]),
+    ("qld_get_tile_cutoff", C.c_int, [_vp]),
+    ("qld_autotune_windows", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int,
+                                       C.POINTER(C.c_int), _f32p]),
 ]
+
+WINDOW_DTYPE = np.dtype([("start", "<i4"), ("size", "<i4"), ("out_offset", "<i8")])
+BATCH_PLAIN = 0
+BATCH_PERSISTENT = 1
 
 HOST_SYMBOLS = [
     ("qld_format_float", C.c_int, [C.c_char_p, C.c_float, C.c_int]),
@@ -300,7 +321,98 @@ class Engine:
         self._check(hip_lib().qld_last_contract_ms(self._ctx, C.byref(ms)))
         return ms.value
 
+    # ---- resident tables and window plans ---------------------------------
+    def table(self, rows, bc, snp_size=None) -> "Table":
+        return Table(self, rows, bc, snp_size)
+
+    def plan_windows(self, table: "Table", windows: np.ndarray, out,
+                     mode=BATCH_PERSISTENT) -> "Plan":
+        return Plan(self, table, windows, out, mode)
+
+    def set_plain_tile(self, tile: int):
+        self._check(hip_lib().qld_set_plain_tile(self._ctx, int(tile)))
+
+    def set_tile_cutoff(self, cutoff: int):
+        self._check(hip_lib().qld_set_tile_cutoff(self._ctx, int(cutoff)))
+
+    @property
+    def tile_cutoff(self) -> int:
+        return hip_lib().qld_get_tile_cutoff(self._ctx)
+
+    def autotune_windows(self, table: "Table", windows: np.ndarray, out, reps=3):
+        w = np.ascontiguousarray(windows, dtype=WINDOW_DTYPE)
+        best = C.c_int()
+        ms = C.c_float()
+        optr = out.ptr if isinstance(out, DevBuffer) else int(out)
+        self._check(hip_lib().qld_autotune_windows(self._ctx, table._t, _ptr(w), len(w),
+                                                   optr, reps, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
     def last_kernel_ms(self) -> float:
         ms = C.c_float()
         self._check(hip_lib().qld_last_kernel_ms(self._ctx, C.byref(ms)))
         return ms.value
+
+
+class Table:
+    """A packed table resident in HBM with its expanded MFMA operand form."""
+
+    def __init__(self, eng: Engine, rows: np.ndarray, bc: np.ndarray, snp_size=None):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        bc = np.ascontiguousarray(bc, dtype=np.uint32)
+        self.eng = eng
+        self.rows, self.words = rows.shape
+        self.snp_size = int(snp_size if snp_size is not None else self.words * 64)
+        t = C.c_void_p()
+        eng._check(hip_lib().qld_table_create(eng._ctx, _ptr(rows), _ptr(bc), self.rows,
+                                              self.words, self.snp_size, C.byref(t)))
+        self._t = t
+
+    @property
+    def nbytes(self) -> int:
+        return hip_lib().qld_table_bytes(self._t)
+
+    def free(self):
+        if self._t:
+            self.eng._check(hip_lib().qld_table_free(self.eng._ctx, self._t))
+            self._t = None
+
+
+class Plan:
+    """A prepared window set (qld_plan_windows)."""
+
+    def __init__(self, eng: Engine, table: Table, windows: np.ndarray, out, mode):
+        w = np.ascontiguousarray(windows, dtype=WINDOW_DTYPE)
+        self.eng = eng
+        self.windows = w
+        p = C.c_void_p()
+        optr = out.ptr if isinstance(out, DevBuffer) else int(out)
+        eng._check(hip_lib().qld_plan_windows(eng._ctx, table._t, _ptr(w), len(w), optr,
+                                              int(mode), C.byref(p)))
+        self._p = p
+
+    def run(self):
+        self.eng._check(hip_lib().qld_plan_run(self.eng._ctx, self._p))
+
+    def use_graph(self, flag: bool):
+        self.eng._check(hip_lib().qld_plan_use_graph(self._p, 1 if flag else 0))
+
+    def stats(self):
+        t, p, n = C.c_int64(), C.c_int64(), C.c_int()
+        self.eng._check(hip_lib().qld_plan_stats(self._p, C.byref(t), C.byref(p), C.byref(n)))
+        return {"tiles": t.value, "pairs": p.value, "launches": n.value}
+
+    def free(self):
+        if self._p:
+            self.eng._check(hip_lib().qld_plan_free(self.eng._ctx, self._p))
+            self._p = None
+
+
+def window_layout(starts, sizes) -> np.ndarray:
+    """Window array with consecutive output blocks (size*size floats each)."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    w = np.zeros(len(sizes), dtype=WINDOW_DTYPE)
+    w["start"] = starts
+    w["size"] = sizes
+    w["out_offset"] = np.concatenate([[0], np.cumsum(sizes * sizes)[:-1]]) if len(sizes) else []
+    return w

@@ -1,26 +1,76 @@
 // Interface of the matrix-core engines (ld_mfma.hip).
+//
+// Every launch of the MFMA kernel consumes a list of tile work items.  An
+// item names one job descriptor and one (ti, tj) tile of that job, so a
+// single launch can mix jobs: the tiles of one region (plain call), of a
+// region pair (rectangle), of thousands of windows (batched call) or of
+// concurrent serving requests on different resident tables.
 #pragma once
 
 #include "qld_internal.h"
 
-// One all-pairs job on expanded operands.
-struct MfmaJob {
-    LdJob job;              // shapes, bit counts, outputs
-    const uint8_t *XA;      // expanded table A (m rows x row_bytes)
-    const uint8_t *XB;      // expanded table B (n rows x row_bytes)
-    int64_t row_bytes;
-    const uint32_t *tiles;  // device tile list, (ti | tj << 16)
-    int64_t ntiles;
+// One all-pairs job on expanded operands.  Row i of the job reads expanded
+// row XI + i*row_bytes and bit count bcI[i]; column j reads XJ / bcJ.  Cell
+// (i, j) goes to res[i*ldr + j] (and cnt[i*ldc + j]); tri != 0 keeps j > i.
+struct WorkDesc {
+    const uint8_t *XI;
+    const uint8_t *XJ;
+    const uint32_t *bcI;
+    const uint32_t *bcJ;
+    float *res;
+    uint32_t *cnt;
+    int64_t ldr;
+    int64_t ldc;
+    int64_t row_bytes;   // multiple of the 128-byte k stage
+    int ni, nj;
+    int tri;
+    int snp_size;
+    float fN;            // epilogue constants of snp_size
+    float pad_;
+    double dN;
+    double invN;
 };
 
-// Packed words per row after padding to whole k stages.
+// One tile: rows [ti*T, ti*T + T) x columns [tj*T, tj*T + T) of job `desc`.
+struct WorkItem {
+    uint32_t desc;
+    uint16_t ti, tj;
+};
+
+void fill_work_desc(WorkDesc &d, const uint8_t *XI, const uint8_t *XJ,
+                    const uint32_t *bcI, const uint32_t *bcJ, int ni, int nj,
+                    int64_t row_bytes, int tri, int snp_size, float *res,
+                    int64_t ldr, uint32_t *cnt, int64_t ldc);
+
+// Packed words per row after padding to whole 128-byte k stages.
 int mfma_padded_words(int engine, int words);
 // Bytes per expanded row.
 int64_t mfma_row_bytes(int engine, int words);
-// Tile edge (pairs) of the MFMA kernels.
-int mfma_tile_edge();
+
+// Tile edges the kernel is built for.
+#define QLD_TILE_LARGE 256
+#define QLD_TILE_SMALL 128
+
+// Append the tiles of one job (tile edge T) to `out`, ordered in 8 x 8
+// super-tiles so that concurrently running tiles share operand panels.
+void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int nj,
+                      int tri, int T);
+size_t job_tile_count(int ni, int nj, int tri, int T);
 
 // Expand packed rows to fp4 nibbles / int8 bytes (rows x row_bytes).
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
                          uint8_t *X, hipStream_t s);
-hipError_t launch_mfma(int engine, int r2_mode, const MfmaJob &j, hipStream_t s);
+
+// Allow every MFMA kernel variant its dynamic LDS (call once before use).
+hipError_t mfma_init_attributes();
+
+// Launch the MFMA kernel over `nitems` items.
+//   descs       device array of job descriptors, or NULL to use `inline_desc`
+//               for every item (single-job launches need no upload)
+//   persistent  0: one workgroup per item;  >0: that many workgroups pull
+//               items from eight per-XCD queues (queue = 16 device uint32,
+//               zeroed by this call on the stream before the kernel)
+hipError_t launch_mfma_items(int engine, int r2_mode, int tile,
+                             const WorkDesc *descs, const WorkDesc &inline_desc,
+                             const WorkItem *items, int64_t nitems,
+                             int persistent, uint32_t *queue, hipStream_t s);

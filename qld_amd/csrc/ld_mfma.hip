@@ -9,10 +9,10 @@
 // The pairing of samples to K positions only has to be the same for every
 // row; the expansion keeps each 64-bit word's samples together.
 //
-// Work decomposition (both engines)
-//   workgroup : 4 wave64 (2 x 2), one 256 x 256 tile of pairs
-//   wave      : 128 x 128 = 8 x 8 MFMA blocks of 16 x 16 (256 fp32/int32
-//               accumulators per lane)
+// Work decomposition (both engines), tile edge T = 256 or 128
+//   workgroup : 4 wave64 (2 x 2), one T x T tile of pairs of one job
+//   wave      : T/2 x T/2 = (T/32)^2 MFMA blocks of 16 x 16 (256 or 64
+//               fp32/int32 accumulators per lane)
 //   k stage   : 128 bytes of every tile row (256 fp4 / 128 int8 samples),
 //               two MFMA k-steps; stages are double-buffered in LDS and
 //               filled with buffer_load_dwordx4 ... lds (no register
@@ -32,13 +32,19 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int MT = 256;          // tile edge
-constexpr int WT = 128;          // wave tile edge
-constexpr int NB = WT / 16;      // MFMA blocks per wave edge
 constexpr int SB = 128;          // bytes per row per k stage
-constexpr int TILE_BYTES = MT * SB;            // one operand, one stage
-constexpr int THREADS = 256;
-constexpr int LOADS_PER_WAVE = MT * SB / 1024 / 4;   // 1 KiB wave loads per operand
+constexpr int THREADS = 256;     // 4 wave64 in a 2 x 2 arrangement
+
+// Geometry of a TILE x TILE workgroup tile.
+template <int TILE>
+struct Geo {
+    static constexpr int WT = TILE / 2;              // wave tile edge
+    static constexpr int NB = WT / 16;               // 16 x 16 MFMA blocks per wave edge
+    static constexpr int ROWS_PER_WAVE = TILE / 4;   // rows each wave stages per operand
+    static constexpr int LOADS = ROWS_PER_WAVE / 8;  // 1 KiB wave loads per operand
+    static constexpr int TILE_BYTES = TILE * SB;     // one operand, one stage
+    static constexpr int LDS_BYTES = 4 * TILE_BYTES; // 2 operands x 2 stages
+};
 
 struct FP4 {
     typedef v4f acc_t;
@@ -61,50 +67,44 @@ struct I8 {
     static __device__ __forceinline__ float count(int v) { return (float)v; }
 };
 
-struct MfmaArgs {
-    const uint8_t *XA;      // expanded table A (columns j)
-    const uint8_t *XB;      // expanded table B (rows i)
-    int64_t row_bytes;      // expanded row pitch (multiple of SB)
-    int stages;             // row_bytes / SB
-    const uint32_t *tiles;  // (ti | tj << 16) per logical tile
-    int64_t ntiles;
-    const uint32_t *bcA;
-    const uint32_t *bcB;
-    int m, n, tri;
-    float *res;
-    int64_t ldr;
-    uint32_t *cnt;
-    int64_t ldc;
-    EpiConst ec;
+struct KArgs {
+    WorkDesc inline_desc;     // used when descs == nullptr
+    const WorkDesc *descs;
+    const WorkItem *items;
+    int64_t nitems;
+    uint32_t *queue;          // persistent launches: 8 per-XCD queue heads
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t panel_rsrc(const uint8_t *base,
-                                                            int row0, int rows,
-                                                            int64_t row_bytes)
+template <int TILE>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t panel_rsrc(const uint8_t *base, int row0,
+                                                            int rows, int64_t row_bytes)
 {
     int64_t have = (int64_t)(rows - row0) * row_bytes;
-    int64_t lim = (int64_t)MT * row_bytes;
+    int64_t lim = (int64_t)TILE * row_bytes;
     int64_t nrec = have < lim ? have : lim;
     if (nrec < 0) nrec = 0;
     return __builtin_amdgcn_make_buffer_rsrc((void *)(base + (int64_t)row0 * row_bytes),
                                              0, (int)nrec, 0x00020000);
 }
 
-// Queue one k stage of one operand: wave w fills tile rows [64w, 64w + 64).
-// Lane L of load q lands in LDS slot (64w + 8q)*8 + L, i.e. row 64w+8q+L/8,
-// physical chunk L%8, which holds logical chunk (L%8) ^ (L/8).
+// Queue one k stage of one operand: wave w fills tile rows
+// [w*R, w*R + R), R = TILE/4.  Lane L of load q lands in LDS slot
+// (w*R + 8q)*8 + L, i.e. row w*R + 8q + L/8, physical chunk L%8, which holds
+// logical chunk (L%8) ^ (L/8).
+template <int TILE>
 __device__ __forceinline__ void stage_load(__amdgpu_buffer_rsrc_t rs, uint8_t *lds_tile,
                                            int64_t row_bytes, int k0, int wave, int lane)
 {
+    typedef Geo<TILE> G;
     const int r_in = lane >> 3, pc = lane & 7;
     const int c = pc ^ r_in;
 #pragma unroll
-    for (int q = 0; q < LOADS_PER_WAVE; ++q) {
-        int row = wave * 64 + q * 8 + r_in;
+    for (int q = 0; q < G::LOADS; ++q) {
+        int row = wave * G::ROWS_PER_WAVE + q * 8 + r_in;
         uint32_t voff = (uint32_t)((int64_t)row * row_bytes + k0 + c * 16);
-        uint8_t *dst = lds_tile + (wave * 64 + q * 8) * SB;
+        uint8_t *dst = lds_tile + (wave * G::ROWS_PER_WAVE + q * 8) * SB;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)dst, 16, voff, 0, 0, 0);
     }
 }
@@ -123,24 +123,25 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <class OP, int MODE>
-__global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(MfmaArgs p)
+// One tile of one job: k loop over double-buffered LDS stages, then the
+// fused epilogue straight to the job's result rows.
+template <class OP, int MODE, int TILE>
+__device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uint8_t *smem)
 {
-    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
-    // Stage buffer b: operand A (rows i) at 2b*TILE_BYTES, B (cols j) after it.
-    auto sA = [&](int b) { return smem + (2 * b) * TILE_BYTES; };
-    auto sB = [&](int b) { return smem + (2 * b + 1) * TILE_BYTES; };
+    typedef Geo<TILE> G;
+    constexpr int NB = G::NB, WT = G::WT;
+    auto sA = [&](int b) { return smem + (2 * b) * G::TILE_BYTES; };
+    auto sB = [&](int b) { return smem + (2 * b + 1) * G::TILE_BYTES; };
 
-    const int64_t L = xcd_logical(blockIdx.x, gridDim.x);
-    const uint32_t t = p.tiles[L];
-    const int ti = (int)(t & 0xFFFF), tj = (int)(t >> 16);
-    const int i0 = ti * MT, j0 = tj * MT;
+    const int i0 = ti * TILE, j0 = tj * TILE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
+    const int64_t rb = d.row_bytes;
+    const int stages = (int)(rb / SB);
 
-    // Operand "A" of the MFMA = rows i (table B); operand "B" = columns j.
-    const __amdgpu_buffer_rsrc_t rsI = panel_rsrc(p.XB, i0, p.n, p.row_bytes);
-    const __amdgpu_buffer_rsrc_t rsJ = panel_rsrc(p.XA, j0, p.m, p.row_bytes);
+    // MFMA operand "A" = job rows i, operand "B" = job columns j.
+    const __amdgpu_buffer_rsrc_t rsI = panel_rsrc<TILE>(d.XI, i0, d.ni, rb);
+    const __amdgpu_buffer_rsrc_t rsJ = panel_rsrc<TILE>(d.XJ, j0, d.nj, rb);
 
     typename OP::acc_t acc[NB][NB];
 #pragma unroll
@@ -148,16 +149,16 @@ __global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(MfmaArgs p)
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = typename OP::acc_t{0, 0, 0, 0};
 
-    stage_load(rsI, sA(0), p.row_bytes, 0, wave, lane);
-    stage_load(rsJ, sB(0), p.row_bytes, 0, wave, lane);
+    stage_load<TILE>(rsI, sA(0), rb, 0, wave, lane);
+    stage_load<TILE>(rsJ, sB(0), rb, 0, wave, lane);
 
     const int r16 = lane & 15, g = lane >> 4;
-    for (int s = 0; s < p.stages; ++s) {
+    for (int s = 0; s < stages; ++s) {
         const int cur = s & 1;
-        if (s + 1 < p.stages) {
-            stage_load(rsI, sA(cur ^ 1), p.row_bytes, (s + 1) * SB, wave, lane);
-            stage_load(rsJ, sB(cur ^ 1), p.row_bytes, (s + 1) * SB, wave, lane);
-            wait_vm<2 * LOADS_PER_WAVE>();
+        if (s + 1 < stages) {
+            stage_load<TILE>(rsI, sA(cur ^ 1), rb, (s + 1) * SB, wave, lane);
+            stage_load<TILE>(rsJ, sB(cur ^ 1), rb, (s + 1) * SB, wave, lane);
+            wait_vm<2 * G::LOADS>();
         } else {
             wait_vm<0>();
         }
@@ -183,29 +184,29 @@ __global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(MfmaArgs p)
     }
 
     // Fused epilogue.  Block (a, b) register v holds the pair
-    // i = i0 + wm*128 + a*16 + 4*(lane/16) + v,  j = j0 + wn*128 + b*16 + lane%16.
+    // i = i0 + wm*WT + a*16 + 4*(lane/16) + v,  j = j0 + wn*WT + b*16 + lane%16.
     const int jb = j0 + wn * WT + r16;
     if (MODE == QLD_R2_REFERENCE) {
         RefTerm tj_[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             int j = jb + b * 16;
-            tj_[b] = ref_term(j < p.m ? p.bcA[j] : 0u, p.ec.fN);
+            tj_[b] = ref_term(j < d.nj ? d.bcJ[j] : 0u, d.fN);
         }
 #pragma unroll
         for (int a = 0; a < NB; ++a)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 int i = i0 + wm * WT + a * 16 + 4 * g + v;
-                if (i >= p.n) continue;
-                RefTerm ti_ = ref_term(p.bcB[i], p.ec.fN);
+                if (i >= d.ni) continue;
+                RefTerm ti_ = ref_term(d.bcI[i], d.fN);
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     int j = jb + b * 16;
-                    if (j >= p.m || (p.tri && j <= i)) continue;
+                    if (j >= d.nj || (d.tri && j <= i)) continue;
                     float c = OP::count(acc[a][b][v]);
-                    if (p.res) p.res[(int64_t)i * p.ldr + j] = r2_reference(c, tj_[b], ti_, p.ec.invN);
-                    if (p.cnt) p.cnt[(int64_t)i * p.ldc + j] = (uint32_t)c;
+                    if (d.res) d.res[(int64_t)i * d.ldr + j] = r2_reference(c, tj_[b], ti_, d.invN);
+                    if (d.cnt) d.cnt[(int64_t)i * d.ldc + j] = (uint32_t)c;
                 }
             }
     } else {
@@ -213,24 +214,69 @@ __global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(MfmaArgs p)
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             int j = jb + b * 16;
-            tj_[b] = exact_term(j < p.m ? p.bcA[j] : 0u, p.ec.dN);
+            tj_[b] = exact_term(j < d.nj ? d.bcJ[j] : 0u, d.dN);
         }
 #pragma unroll
         for (int a = 0; a < NB; ++a)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 int i = i0 + wm * WT + a * 16 + 4 * g + v;
-                if (i >= p.n) continue;
-                ExactTerm ti_ = exact_term(p.bcB[i], p.ec.dN);
+                if (i >= d.ni) continue;
+                ExactTerm ti_ = exact_term(d.bcI[i], d.dN);
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     int j = jb + b * 16;
-                    if (j >= p.m || (p.tri && j <= i)) continue;
+                    if (j >= d.nj || (d.tri && j <= i)) continue;
                     float c = OP::count(acc[a][b][v]);
-                    if (p.res) p.res[(int64_t)i * p.ldr + j] = r2_exact(c, tj_[b], ti_, p.ec.dN);
-                    if (p.cnt) p.cnt[(int64_t)i * p.ldc + j] = (uint32_t)c;
+                    if (d.res) d.res[(int64_t)i * d.ldr + j] = r2_exact(c, tj_[b], ti_, d.dN);
+                    if (d.cnt) d.cnt[(int64_t)i * d.ldc + j] = (uint32_t)c;
                 }
             }
+    }
+}
+
+template <class OP, int MODE, int TILE>
+__device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, uint8_t *smem)
+{
+    const WorkItem it = p.items[idx];
+    if (p.descs) {
+        const WorkDesc d = p.descs[it.desc];
+        mfma_tile<OP, MODE, TILE>(d, it.ti, it.tj, smem);
+    } else {
+        mfma_tile<OP, MODE, TILE>(p.inline_desc, it.ti, it.tj, smem);
+    }
+}
+
+// One workgroup per item (PERSIST == 0) or a fixed grid draining eight
+// per-XCD queues: workgroup b starts on queue b % 8 -- the XCD it runs on --
+// whose items are a contiguous range of the list, then steals from the
+// other queues.  Every claim advances a head, so the loop ends once all
+// eight ranges are exhausted.
+template <class OP, int MODE, int TILE, int PERSIST>
+__global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(KArgs p)
+{
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    if (!PERSIST) {
+        run_item<OP, MODE, TILE>(p, xcd_logical(blockIdx.x, gridDim.x), smem);
+        return;
+    }
+    __shared__ int64_t claimed;
+    const int x0 = (int)(blockIdx.x % QLD_NUM_XCD);
+    for (int q = 0; q < QLD_NUM_XCD; ++q) {
+        const int x = (x0 + q) % QLD_NUM_XCD;
+        const int64_t begin = p.nitems * x / QLD_NUM_XCD;
+        const int64_t end = p.nitems * (x + 1) / QLD_NUM_XCD;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                int64_t k = (int64_t)atomicAdd(&p.queue[x], 1u);
+                claimed = begin + k < end ? begin + k : -1;
+            }
+            __syncthreads();
+            const int64_t idx = claimed;
+            __syncthreads();
+            if (idx < 0) break;
+            run_item<OP, MODE, TILE>(p, idx, smem);
+        }
     }
 }
 
@@ -302,7 +348,53 @@ int64_t mfma_row_bytes(int engine, int words)
     return per_word * mfma_padded_words(engine, words);
 }
 
-int mfma_tile_edge() { return MT; }
+void fill_work_desc(WorkDesc &d, const uint8_t *XI, const uint8_t *XJ,
+                    const uint32_t *bcI, const uint32_t *bcJ, int ni, int nj,
+                    int64_t row_bytes, int tri, int snp_size, float *res,
+                    int64_t ldr, uint32_t *cnt, int64_t ldc)
+{
+    EpiConst e = make_epi_const(snp_size);
+    d.XI = XI;
+    d.XJ = XJ;
+    d.bcI = bcI;
+    d.bcJ = bcJ;
+    d.res = res;
+    d.cnt = cnt;
+    d.ldr = ldr;
+    d.ldc = ldc;
+    d.row_bytes = row_bytes;
+    d.ni = ni;
+    d.nj = nj;
+    d.tri = tri;
+    d.snp_size = snp_size;
+    d.fN = e.fN;
+    d.pad_ = 0.0f;
+    d.dN = e.dN;
+    d.invN = e.invN;
+}
+
+size_t job_tile_count(int ni, int nj, int tri, int T)
+{
+    size_t a = (size_t)((ni + T - 1) / T), b = (size_t)((nj + T - 1) / T);
+    return tri ? a * (a + 1) / 2 : a * b;
+}
+
+void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int nj,
+                      int tri, int T)
+{
+    const int S = 8;
+    const int ti_count = (ni + T - 1) / T, tj_count = (nj + T - 1) / T;
+    const int si = (ti_count + S - 1) / S, sj = (tj_count + S - 1) / S;
+    size_t k = *count;
+    for (int J = 0; J < sj; ++J)
+        for (int I = 0; I < si; ++I) {
+            if (tri && I > J) continue;
+            for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj)
+                for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti)
+                    if (!tri || ti <= tj) out[k++] = WorkItem{desc, (uint16_t)ti, (uint16_t)tj};
+        }
+    *count = k;
+}
 
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
                          uint8_t *X, hipStream_t s)
@@ -321,46 +413,84 @@ hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
     return hipGetLastError();
 }
 
-template <class OP, int MODE>
-static hipError_t launch_one(const MfmaArgs &a, hipStream_t s)
+__global__ void zero_queue_kernel(uint32_t *q)
 {
-    const size_t lds = 4 * (size_t)TILE_BYTES;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)ld_mfma_kernel<OP, MODE>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE>), dim3((unsigned)a.ntiles), dim3(THREADS),
-                       lds, s, a);
+    if (threadIdx.x < 16) q[threadIdx.x] = 0u;
+}
+
+template <class OP, int MODE, int TILE, int PERSIST>
+static hipError_t set_lds_limit()
+{
+    return hipFuncSetAttribute((const void *)ld_mfma_kernel<OP, MODE, TILE, PERSIST>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)Geo<TILE>::LDS_BYTES);
+}
+
+template <class OP, int MODE>
+static hipError_t set_lds_limits_mode()
+{
+    hipError_t e;
+    if ((e = set_lds_limit<OP, MODE, QLD_TILE_LARGE, 0>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, QLD_TILE_LARGE, 1>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, QLD_TILE_SMALL, 0>()) != hipSuccess) return e;
+    return set_lds_limit<OP, MODE, QLD_TILE_SMALL, 1>();
+}
+
+// Raise the dynamic LDS limit of every kernel variant (done once per
+// process, before any launch -- and so before any graph capture).
+hipError_t mfma_init_attributes()
+{
+    hipError_t e;
+    if ((e = set_lds_limits_mode<FP4, QLD_R2_REFERENCE>()) != hipSuccess) return e;
+    if ((e = set_lds_limits_mode<FP4, QLD_R2_EXACT>()) != hipSuccess) return e;
+    if ((e = set_lds_limits_mode<I8, QLD_R2_REFERENCE>()) != hipSuccess) return e;
+    return set_lds_limits_mode<I8, QLD_R2_EXACT>();
+}
+
+template <class OP, int MODE, int TILE, int PERSIST>
+static hipError_t launch_one(const KArgs &a, unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE, TILE, PERSIST>), dim3(grid), dim3(THREADS),
+                       (size_t)Geo<TILE>::LDS_BYTES, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_mfma(int engine, int r2_mode, const MfmaJob &j, hipStream_t s)
+template <class OP, int MODE, int TILE>
+static hipError_t launch_mode(const KArgs &a, int persistent, hipStream_t s)
 {
-    MfmaArgs a;
-    a.XA = j.XA;
-    a.XB = j.XB;
-    a.row_bytes = j.row_bytes;
-    a.stages = (int)(j.row_bytes / SB);
-    a.tiles = j.tiles;
-    a.ntiles = j.ntiles;
-    a.bcA = j.job.bcA;
-    a.bcB = j.job.bcB;
-    a.m = j.job.m;
-    a.n = j.job.n;
-    a.tri = j.job.tri;
-    a.res = j.job.res;
-    a.ldr = j.job.ldr;
-    a.cnt = j.job.cnt;
-    a.ldc = j.job.ldc;
-    a.ec = make_epi_const(j.job.snp_size);
-    if (a.ntiles == 0) return hipSuccess;
+    if (persistent > 0)
+        return launch_one<OP, MODE, TILE, 1>(a, (unsigned)persistent, s);
+    return launch_one<OP, MODE, TILE, 0>(a, (unsigned)a.nitems, s);
+}
+
+template <class OP, int MODE>
+static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStream_t s)
+{
+    if (tile == QLD_TILE_SMALL) return launch_mode<OP, MODE, QLD_TILE_SMALL>(a, persistent, s);
+    return launch_mode<OP, MODE, QLD_TILE_LARGE>(a, persistent, s);
+}
+
+hipError_t launch_mfma_items(int engine, int r2_mode, int tile,
+                             const WorkDesc *descs, const WorkDesc &inline_desc,
+                             const WorkItem *items, int64_t nitems,
+                             int persistent, uint32_t *queue, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    KArgs a;
+    a.inline_desc = inline_desc;
+    a.descs = descs;
+    a.items = items;
+    a.nitems = nitems;
+    a.queue = queue;
+    if (persistent > 0) {
+        if (!queue) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, queue);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     if (engine == QLD_ENGINE_MFMA_FP4)
-        return r2_mode == QLD_R2_EXACT ? launch_one<FP4, QLD_R2_EXACT>(a, s)
-                                       : launch_one<FP4, QLD_R2_REFERENCE>(a, s);
-    return r2_mode == QLD_R2_EXACT ? launch_one<I8, QLD_R2_EXACT>(a, s)
-                                   : launch_one<I8, QLD_R2_REFERENCE>(a, s);
+        return r2_mode == QLD_R2_EXACT ? launch_tile<FP4, QLD_R2_EXACT>(a, tile, persistent, s)
+                                       : launch_tile<FP4, QLD_R2_REFERENCE>(a, tile, persistent, s);
+    return r2_mode == QLD_R2_EXACT ? launch_tile<I8, QLD_R2_EXACT>(a, tile, persistent, s)
+                                   : launch_tile<I8, QLD_R2_REFERENCE>(a, tile, persistent, s);
 }

@@ -1,0 +1,440 @@
+// Resident tables and window plans: the batched path.
+//
+// The reference correlates many windows as independent tasks, each a full
+// correlate() call that reloads its table and runs one GEMM
+// (input_list.c:205-323, correlate_function.c:112-290).  Here a chromosome
+// is uploaded and expanded once (qld_table), and a window set becomes a
+// plan: one job descriptor per window, the tile work list of every window,
+// and either one persistent MFMA launch per tile shape (draining per-XCD
+// work queues) or -- as the baseline -- one launch per window.  Either
+// launch sequence is captured in a HIP graph and replayed.
+#include "qld_internal.h"
+#include "qld_ctx.h"
+#include "ld_mfma.h"
+
+#include <algorithm>
+#include <numeric>
+#include <string.h>
+#include <vector>
+
+struct qld_table {
+    int rows = 0, words = 0, snp_size = 0, engine = QLD_ENGINE_MFMA_FP4;
+    int64_t row_bytes = 0;
+    uint64_t *packed = nullptr;
+    uint32_t *bc = nullptr;
+    uint8_t *X = nullptr;
+    size_t bytes_packed = 0, bytes_bc = 0, bytes_x = 0;
+};
+
+struct PlanClass {
+    int tile = QLD_TILE_LARGE;
+    int grid = 0;                   // persistent workgroups
+    size_t ndesc = 0, nitems = 0;
+    WorkDesc *d_descs = nullptr;
+    WorkItem *d_items = nullptr;
+    uint32_t *d_queue = nullptr;
+};
+
+struct PlainLaunch {
+    WorkDesc desc;
+    int64_t item_off, item_cnt;
+    int tile;
+};
+
+struct qld_plan {
+    int mode = QLD_BATCH_PERSISTENT;
+    int engine = QLD_ENGINE_MFMA_FP4;
+    int r2_mode = QLD_R2_REFERENCE;
+    std::vector<PlanClass> classes;       // persistent mode
+    std::vector<PlainLaunch> plain;       // plain mode
+    WorkItem *d_items_all = nullptr;      // plain mode item array
+    std::vector<std::pair<void *, size_t>> owned;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = true;
+    int64_t tiles = 0, pairs = 0;
+    int launches = 0;
+};
+
+static int table_alloc(qld_ctx *ctx, void **p, size_t bytes)
+{
+    return qld_ctx_malloc(ctx, bytes ? bytes : 1, p);
+}
+
+static void table_release(qld_ctx *ctx, qld_table *t)
+{
+    qld_ctx_mfree(ctx, t->packed, t->bytes_packed);
+    qld_ctx_mfree(ctx, t->bc, t->bytes_bc);
+    qld_ctx_mfree(ctx, t->X, t->bytes_x);
+    delete t;
+}
+
+static int table_create(qld_ctx *ctx, const void *rows_src, const void *bc_src,
+                        hipMemcpyKind kind, int rows, int words, int snp_size,
+                        qld_table **out)
+{
+    if (!ctx || !out || !rows_src || !bc_src || rows <= 0 || words <= 0 ||
+        snp_size <= 0 || snp_size >= QLD_MAX_SNP_SIZE ||
+        (int64_t)words * 64 < (int64_t)snp_size - 63) {
+        if (ctx) qld_set_error(ctx, "qld_table_create: bad arguments (rows=%d words=%d N=%d)",
+                               rows, words, snp_size);
+        return QLD_ERR_INVALID;
+    }
+    *out = nullptr;
+    hipSetDevice(ctx->device);
+    qld_table *t = new (std::nothrow) qld_table();
+    if (!t) return QLD_ERR_NOMEM;
+    t->rows = rows;
+    t->words = words;
+    t->snp_size = snp_size;
+    t->engine = ctx->cfg.engine == QLD_ENGINE_MFMA_I8 ? QLD_ENGINE_MFMA_I8 : QLD_ENGINE_MFMA_FP4;
+    t->row_bytes = mfma_row_bytes(t->engine, words);
+    t->bytes_packed = (size_t)rows * words * 8;
+    t->bytes_bc = (size_t)rows * 4;
+    t->bytes_x = (size_t)rows * t->row_bytes;
+    int rc;
+    if ((rc = table_alloc(ctx, (void **)&t->packed, t->bytes_packed)) ||
+        (rc = table_alloc(ctx, (void **)&t->bc, t->bytes_bc)) ||
+        (rc = table_alloc(ctx, (void **)&t->X, t->bytes_x))) {
+        table_release(ctx, t);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(t->packed, rows_src, t->bytes_packed, kind, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->bc, bc_src, t->bytes_bc, kind, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_expand(t->engine, t->packed, rows, words, t->X, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        table_release(ctx, t);
+        return qld_hip_fail(ctx, e, "qld_table_create");
+    }
+    *out = t;
+    return QLD_OK;
+}
+
+extern "C" int qld_table_create(qld_ctx *ctx, const uint64_t *rows_host,
+                                const uint32_t *bc_host, int rows, int words,
+                                int snp_size, qld_table **out)
+{
+    return table_create(ctx, rows_host, bc_host, hipMemcpyHostToDevice, rows, words,
+                        snp_size, out);
+}
+
+extern "C" int qld_table_create_dev(qld_ctx *ctx, const uint64_t *rows_dev,
+                                    const uint32_t *bc_dev, int rows, int words,
+                                    int snp_size, qld_table **out)
+{
+    return table_create(ctx, rows_dev, bc_dev, hipMemcpyDeviceToDevice, rows, words,
+                        snp_size, out);
+}
+
+extern "C" int qld_table_free(qld_ctx *ctx, qld_table *t)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    if (!t) return QLD_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    table_release(ctx, t);
+    return QLD_OK;
+}
+
+extern "C" int qld_table_rows(const qld_table *t) { return t ? t->rows : 0; }
+
+extern "C" size_t qld_table_bytes(const qld_table *t)
+{
+    return t ? t->bytes_packed + t->bytes_bc + t->bytes_x : 0;
+}
+
+// ---- plans ------------------------------------------------------------------
+
+int qld_window_tile(const qld_ctx *ctx, int size)
+{
+    return size >= ctx->tile_cutoff ? QLD_TILE_LARGE : QLD_TILE_SMALL;
+}
+
+static int plan_upload(qld_ctx *ctx, qld_plan *p, void **dst, const void *src, size_t bytes)
+{
+    int rc = qld_ctx_malloc(ctx, bytes ? bytes : 8, dst);
+    if (rc) return rc;
+    p->owned.push_back({*dst, bytes ? bytes : 8});
+    if (!bytes) return QLD_OK;
+    hipError_t e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "plan upload");
+}
+
+static int plan_alloc(qld_ctx *ctx, qld_plan *p, void **dst, size_t bytes)
+{
+    int rc = qld_ctx_malloc(ctx, bytes, dst);
+    if (rc == QLD_OK) p->owned.push_back({*dst, bytes});
+    return rc;
+}
+
+static void plan_release(qld_ctx *ctx, qld_plan *p)
+{
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    for (auto &o : p->owned) qld_ctx_mfree(ctx, o.first, o.second);
+    delete p;
+}
+
+static int persistent_grid(const qld_ctx *ctx, int tile)
+{
+    // 256-tiles use 128 KiB of LDS (one workgroup per CU); 128-tiles use
+    // 64 KiB and ~216 registers per lane (two per CU).
+    return ctx->num_cu * (tile == QLD_TILE_LARGE ? 1 : 2);
+}
+
+// Queue the plan's kernels on the context stream (direct, or under capture).
+static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
+{
+    hipError_t e = hipSuccess;
+    WorkDesc none;
+    memset(&none, 0, sizeof(none));
+    if (p->mode == QLD_BATCH_PERSISTENT) {
+        for (auto &c : p->classes) {
+            if (!c.nitems) continue;
+            e = launch_mfma_items(p->engine, p->r2_mode, c.tile, c.d_descs, none, c.d_items,
+                                  (int64_t)c.nitems, c.grid, c.d_queue, ctx->stream);
+            if (e != hipSuccess) return e;
+        }
+    } else {
+        for (auto &l : p->plain) {
+            e = launch_mfma_items(p->engine, p->r2_mode, l.tile, nullptr, l.desc,
+                                  p->d_items_all + l.item_off, l.item_cnt, 0, nullptr,
+                                  ctx->stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return e;
+}
+
+extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
+                                int nwin, float *out, int mode, qld_plan **plan)
+{
+    if (!ctx || !t || !plan || nwin < 0 || (nwin && (!wins || !out)) ||
+        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT)) {
+        if (ctx) qld_set_error(ctx, "qld_plan_windows: bad arguments");
+        return QLD_ERR_INVALID;
+    }
+    *plan = nullptr;
+    for (int k = 0; k < nwin; ++k) {
+        const qld_window &w = wins[k];
+        if (w.start < 0 || w.size < 0 || (int64_t)w.start + w.size > t->rows ||
+            w.out_offset < 0 || w.size > 65535 * QLD_TILE_SMALL) {
+            qld_set_error(ctx, "window %d (start %d size %d) outside table of %d rows",
+                          k, w.start, w.size, t->rows);
+            return QLD_ERR_INVALID;
+        }
+    }
+    hipSetDevice(ctx->device);
+    qld_plan *p = new (std::nothrow) qld_plan();
+    if (!p) return QLD_ERR_NOMEM;
+    p->mode = mode;
+    p->engine = t->engine;
+    p->r2_mode = ctx->cfg.r2_mode;
+
+    // Windows in table order, so that each XCD's contiguous share of the
+    // work list covers one stretch of the chromosome.
+    std::vector<int> order(nwin);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return wins[a].start < wins[b].start; });
+
+    auto desc_of = [&](const qld_window &w) {
+        WorkDesc d;
+        const uint8_t *X = t->X + (int64_t)w.start * t->row_bytes;
+        const uint32_t *bc = t->bc + w.start;
+        fill_work_desc(d, X, X, bc, bc, w.size, w.size, t->row_bytes, 1, t->snp_size,
+                       out + w.out_offset, w.size, nullptr, w.size);
+        return d;
+    };
+
+    int rc = QLD_OK;
+    if (mode == QLD_BATCH_PERSISTENT) {
+        for (int tile : {QLD_TILE_LARGE, QLD_TILE_SMALL}) {
+            std::vector<WorkDesc> descs;
+            std::vector<WorkItem> items;
+            for (int k : order) {
+                const qld_window &w = wins[k];
+                if (w.size < 2 || qld_window_tile(ctx, w.size) != tile) continue;
+                size_t cnt = items.size();
+                items.resize(cnt + job_tile_count(w.size, w.size, 1, tile));
+                append_job_tiles(items.data(), &cnt, (uint32_t)descs.size(), w.size, w.size,
+                                 1, tile);
+                descs.push_back(desc_of(w));
+                p->pairs += (int64_t)w.size * (w.size - 1) / 2;
+            }
+            if (items.empty()) continue;
+            PlanClass c;
+            c.tile = tile;
+            c.ndesc = descs.size();
+            c.nitems = items.size();
+            c.grid = std::min<int64_t>(persistent_grid(ctx, tile), (int64_t)items.size());
+            if ((rc = plan_upload(ctx, p, (void **)&c.d_descs, descs.data(),
+                                  descs.size() * sizeof(WorkDesc))) ||
+                (rc = plan_upload(ctx, p, (void **)&c.d_items, items.data(),
+                                  items.size() * sizeof(WorkItem))) ||
+                (rc = plan_alloc(ctx, p, (void **)&c.d_queue, 16 * sizeof(uint32_t)))) {
+                plan_release(ctx, p);
+                return rc;
+            }
+            p->tiles += (int64_t)items.size();
+            p->launches += 1;
+            p->classes.push_back(c);
+        }
+    } else {
+        std::vector<WorkItem> items;
+        for (int k : order) {
+            const qld_window &w = wins[k];
+            if (w.size < 2) continue;
+            const int tile = qld_window_tile(ctx, w.size);
+            PlainLaunch l;
+            l.tile = tile;
+            l.item_off = (int64_t)items.size();
+            size_t cnt = items.size();
+            items.resize(cnt + job_tile_count(w.size, w.size, 1, tile));
+            append_job_tiles(items.data(), &cnt, 0, w.size, w.size, 1, tile);
+            l.item_cnt = (int64_t)cnt - l.item_off;
+            l.desc = desc_of(w);
+            p->plain.push_back(l);
+            p->pairs += (int64_t)w.size * (w.size - 1) / 2;
+        }
+        p->tiles = (int64_t)items.size();
+        p->launches = (int)p->plain.size();
+        if ((rc = plan_upload(ctx, p, (void **)&p->d_items_all, items.data(),
+                              items.size() * sizeof(WorkItem)))) {
+            plan_release(ctx, p);
+            return rc;
+        }
+    }
+
+    // Capture the launch sequence once; replays cost one graph launch.
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t ei = plan_issue(ctx, p);
+        e = hipStreamEndCapture(ctx->stream, &p->graph);
+        if (ei != hipSuccess) e = ei;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p->use_graph = false;     // direct launches still work
+        if (p->exec) hipGraphExecDestroy(p->exec);
+        if (p->graph) hipGraphDestroy(p->graph);
+        p->exec = nullptr;
+        p->graph = nullptr;
+    }
+    *plan = p;
+    return QLD_OK;
+}
+
+extern "C" int qld_plan_run(qld_ctx *ctx, qld_plan *p)
+{
+    if (!ctx || !p) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    if (ctx->cfg.timing) hipEventRecord(ctx->ev0, ctx->stream);
+    if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
+    hipError_t e = (p->use_graph && p->exec) ? hipGraphLaunch(p->exec, ctx->stream)
+                                             : plan_issue(ctx, p);
+    if (e != hipSuccess) return qld_hip_fail(ctx, e, "qld_plan_run");
+    if (ctx->cfg.timing) {
+        hipEventRecord(ctx->kev1, ctx->stream);
+        hipEventRecord(ctx->ev1, ctx->stream);
+        ctx->timed = true;
+    }
+    ctx->last_engine = p->engine;
+    return QLD_OK;
+}
+
+extern "C" int qld_plan_use_graph(qld_plan *p, int use_graph)
+{
+    if (!p) return QLD_ERR_INVALID;
+    if (use_graph && !p->exec) return QLD_ERR_NOTFOUND;
+    p->use_graph = use_graph != 0;
+    return QLD_OK;
+}
+
+extern "C" int qld_plan_stats(const qld_plan *p, int64_t *tiles, int64_t *pairs,
+                              int *launches)
+{
+    if (!p) return QLD_ERR_INVALID;
+    if (tiles) *tiles = p->tiles;
+    if (pairs) *pairs = p->pairs;
+    if (launches) *launches = p->launches;
+    return QLD_OK;
+}
+
+extern "C" int qld_plan_free(qld_ctx *ctx, qld_plan *p)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    if (!p) return QLD_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    plan_release(ctx, p);
+    return QLD_OK;
+}
+
+// ---- tile-shape autotuner ---------------------------------------------------
+
+extern "C" int qld_set_tile_cutoff(qld_ctx *ctx, int cutoff)
+{
+    if (!ctx || cutoff < 0) return QLD_ERR_INVALID;
+    ctx->tile_cutoff = cutoff;
+    return QLD_OK;
+}
+
+extern "C" int qld_get_tile_cutoff(const qld_ctx *ctx)
+{
+    return ctx ? ctx->tile_cutoff : -1;
+}
+
+// Time the window set as persistent plans under each candidate cutoff
+// (windows of at least `cutoff` SNPs on 256-tiles, the rest on 128-tiles)
+// and keep the fastest for later plans of this context.
+extern "C" int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
+                                    int nwin, float *out, int reps, int *best_cutoff,
+                                    float *best_ms)
+{
+    if (!ctx || !t || nwin <= 0 || !wins || !out) return QLD_ERR_INVALID;
+    if (reps <= 0) reps = 3;
+    static const int candidates[] = {0, 256, 384, 512, 768, 1024, 1536, 2048, 1 << 30};
+    const int saved = ctx->tile_cutoff;
+    int best = saved;
+    float best_t = 3.4e38f;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return QLD_ERR_DEVICE;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return QLD_ERR_DEVICE;
+    }
+    int rc = QLD_OK;
+    for (int cut : candidates) {
+        ctx->tile_cutoff = cut;
+        qld_plan *p = nullptr;
+        if ((rc = qld_plan_windows(ctx, t, wins, nwin, out, QLD_BATCH_PERSISTENT, &p))) break;
+        if ((rc = qld_plan_run(ctx, p)) == QLD_OK) {            // warm-up
+            hipEventRecord(e0, ctx->stream);
+            for (int r = 0; r < reps && rc == QLD_OK; ++r) rc = qld_plan_run(ctx, p);
+            hipEventRecord(e1, ctx->stream);
+            float ms = 0.0f;
+            if (rc == QLD_OK && hipEventSynchronize(e1) == hipSuccess &&
+                hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+                ms /= reps;
+                if (ms < best_t) {
+                    best_t = ms;
+                    best = cut;
+                }
+            }
+        }
+        qld_plan_free(ctx, p);
+        if (rc) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    ctx->tile_cutoff = rc == QLD_OK ? best : saved;
+    if (best_cutoff) *best_cutoff = ctx->tile_cutoff;
+    if (best_ms) *best_ms = best_t;
+    return rc;
+}

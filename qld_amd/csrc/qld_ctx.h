@@ -2,6 +2,7 @@
 #pragma once
 
 #include "qld_internal.h"
+#include "ld_mfma.h"
 
 #include <new>
 #include <unordered_map>
@@ -32,9 +33,11 @@ struct qld_ctx {
     std::unordered_map<void *, size_t> allocs;   // qld_dev_alloc'd blocks
     DevBuf host_stage;   // staging for the host-buffer entry point
     DevBuf ws;           // engine workspace (expanded operands)
-    DevBuf tiles;        // device tile list of the MFMA engines
-    std::vector<uint32_t> tiles_host;
-    int tiles_key[3] = {-1, -1, -1};             // (ti_count, tj_count, tri)
+    DevBuf tiles;        // device tile list of the last plain MFMA shape
+    std::vector<WorkItem> tiles_host;
+    int tiles_key[4] = {-1, -1, -1, -1};         // (ni, nj, tri, tile edge)
+    int tile_cutoff = 1024;   // windows of at least this many SNPs use 256-tiles
+    int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape
     char err[512] = {0};
 };
 
@@ -45,3 +48,4 @@ void qld_ctx_mfree(qld_ctx *ctx, void *p, size_t bytes);
 int qld_fill_async(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32);
 int qld_run_job(qld_ctx *ctx, const LdJob &job);
 int qld_pick_engine(const qld_ctx *ctx, const LdJob &job);
+int qld_window_tile(const qld_ctx *ctx, int size);

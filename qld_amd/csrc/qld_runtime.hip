@@ -94,6 +94,12 @@ extern "C" int qld_gpu_init(qld_ctx **out, const qld_cfg *cfg)
         delete ctx;
         return QLD_ERR_DEVICE;
     }
+    static hipError_t attr_status = mfma_init_attributes();
+    if (attr_status != hipSuccess) {
+        qld_set_error(nullptr, "kernel attributes: %s", hipGetErrorString(attr_status));
+        qld_gpu_release(ctx);
+        return QLD_ERR_DEVICE;
+    }
     size_t freeb = 0, totalb = 0;
     hipMemGetInfo(&freeb, &totalb);
     ctx->budget = c.hbm_budget ? c.hbm_budget : (size_t)(0.9 * (double)freeb);
@@ -294,32 +300,22 @@ int qld_pick_engine(const qld_ctx *ctx, const LdJob &j)
     return QLD_ENGINE_MFMA_FP4;
 }
 
-// Tile order of the MFMA launches: 8 x 8 super-tiles, column-major inside,
-// so that the ~32 tiles one XCD runs at a time share 4 column panels and
-// 8 row panels in its L2.  Triangles keep only tiles with tj >= ti.
-static void build_tile_list(std::vector<uint32_t> &out, int ti_count, int tj_count, int tri)
-{
-    const int S = 8;
-    out.clear();
-    const int si = (ti_count + S - 1) / S, sj = (tj_count + S - 1) / S;
-    for (int J = 0; J < sj; ++J)
-        for (int I = 0; I < si; ++I) {
-            if (tri && I > J) continue;
-            for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj)
-                for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti)
-                    if (!tri || ti <= tj)
-                        out.push_back((uint32_t)ti | ((uint32_t)tj << 16));
-        }
-}
-
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Tile edge for one plain job: the 256 x 256 tiles halve operand traffic per
+// pair, but a job needs enough of them to occupy every CU.
+static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job)
+{
+    if (ctx->plain_tile) return ctx->plain_tile;
+    size_t big = job_tile_count(job.n, job.m, job.tri, QLD_TILE_LARGE);
+    return big >= (size_t)ctx->num_cu ? QLD_TILE_LARGE : QLD_TILE_SMALL;
+}
 
 static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
 {
-    const int T = mfma_tile_edge();
-    const int ti_count = (job.n + T - 1) / T, tj_count = (job.m + T - 1) / T;
-    if (ti_count > 65535 || tj_count > 65535) {
-        qld_set_error(ctx, "too many rows for the MFMA tile list (m=%d n=%d)", job.m, job.n);
+    const int T = plain_tile_edge(ctx, job);
+    if ((job.n + T - 1) / T > 65535 || (job.m + T - 1) / T > 65535) {
+        qld_set_error(ctx, "too many rows for one launch (m=%d n=%d)", job.m, job.n);
         return QLD_ERR_INVALID;
     }
     const int64_t rb = mfma_row_bytes(engine, job.words);
@@ -331,32 +327,35 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     uint8_t *XA = (uint8_t *)ctx->ws.ptr;
     uint8_t *XB = same ? XA : XA + bytesA;
 
-    if (ctx->tiles_key[0] != ti_count || ctx->tiles_key[1] != tj_count ||
-        ctx->tiles_key[2] != job.tri) {
-        build_tile_list(ctx->tiles_host, ti_count, tj_count, job.tri);
-        size_t tb = ctx->tiles_host.size() * sizeof(uint32_t);
-        if ((rc = ctx->tiles.reserve(ctx, tb ? tb : 4))) return rc;
+    if (ctx->tiles_key[0] != job.n || ctx->tiles_key[1] != job.m ||
+        ctx->tiles_key[2] != job.tri || ctx->tiles_key[3] != T) {
+        size_t cnt = 0;
+        ctx->tiles_host.resize(job_tile_count(job.n, job.m, job.tri, T));
+        append_job_tiles(ctx->tiles_host.data(), &cnt, 0, job.n, job.m, job.tri, T);
+        size_t tb = cnt * sizeof(WorkItem);
+        ctx->tiles_key[0] = -1;
+        if ((rc = ctx->tiles.reserve(ctx, tb ? tb : 8))) return rc;
         hipError_t e = hipMemcpyAsync(ctx->tiles.ptr, ctx->tiles_host.data(), tb,
                                       hipMemcpyHostToDevice, ctx->stream);
         if (e != hipSuccess) return qld_hip_fail(ctx, e, "tile list upload");
-        ctx->tiles_key[0] = ti_count;
-        ctx->tiles_key[1] = tj_count;
+        ctx->tiles_key[0] = job.n;
+        ctx->tiles_key[1] = job.m;
         ctx->tiles_key[2] = job.tri;
+        ctx->tiles_key[3] = T;
     }
 
     hipError_t e = launch_expand(engine, job.A, job.m, job.words, XA, ctx->stream);
     if (e == hipSuccess && !same)
         e = launch_expand(engine, job.B, job.n, job.words, XB, ctx->stream);
     if (e != hipSuccess) return qld_hip_fail(ctx, e, "expand kernel");
-    MfmaJob mj;
-    mj.job = job;
-    mj.XA = XA;
-    mj.XB = XB;
-    mj.row_bytes = rb;
-    mj.tiles = (const uint32_t *)ctx->tiles.ptr;
-    mj.ntiles = (int64_t)ctx->tiles_host.size();
+    // Job rows i come from table B, columns j from table A.
+    WorkDesc d;
+    fill_work_desc(d, XB, XA, job.bcB, job.bcA, job.n, job.m, rb, job.tri, job.snp_size,
+                   job.res, job.ldr, job.cnt, job.ldc);
     if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
-    e = launch_mfma(engine, ctx->cfg.r2_mode, mj, ctx->stream);
+    e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, nullptr, d,
+                          (const WorkItem *)ctx->tiles.ptr,
+                          (int64_t)ctx->tiles_host.size(), 0, nullptr, ctx->stream);
     if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "MFMA kernel launch");
 }
@@ -508,6 +507,14 @@ extern "C" int qld_timer_stop(qld_ctx *ctx, float *ms)
     if (e == hipSuccess) e = hipEventSynchronize(ctx->tev1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->tev0, ctx->tev1);
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_timer_stop");
+}
+
+extern "C" int qld_set_plain_tile(qld_ctx *ctx, int tile)
+{
+    if (!ctx || (tile != 0 && tile != QLD_TILE_LARGE && tile != QLD_TILE_SMALL))
+        return QLD_ERR_INVALID;
+    ctx->plain_tile = tile;
+    return QLD_OK;
 }
 
 extern "C" int qld_last_engine(const qld_ctx *ctx)

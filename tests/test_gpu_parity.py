@@ -18,6 +18,9 @@ from conftest import golden_cases, load_case
 pytestmark = pytest.mark.gpu
 
 ENGINES = ["popcount", "mfma_fp4", "mfma_i8"]
+# Shapes large enough for 256 x 256 tiles on every CU (the AUTO engine path).
+BIG_SHAPES = [(1100, 900), (600, 3000)]
+TILES = [256, 128]
 TOL_FP64 = 1e-6
 
 
@@ -197,3 +200,25 @@ def test_errors_are_status_codes(engine_factory):
     # the context stays usable after errors
     res, _ = e.correlate(A, bc, snp_size=100)
     assert res.shape == (10, 10)
+
+
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_i8"])
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+@pytest.mark.parametrize("tile", TILES)
+def test_forced_tile_shapes(engine, mode, tile, engine_factory):
+    """Both tile edges on ragged rectangles and a triangle."""
+    e = engine_factory(engine=engine, r2_mode=mode)
+    e.set_plain_tile(tile)
+    for N in (16622, 100000):
+        for m, n in BIG_SHAPES:
+            say("rect", engine, mode, tile, N, m, n)
+            A, bA, B, bB = qld_amd.synth_regions(31 + m + N, m, n, N)
+            res, cnt = e.correlate(A, bA, B, bB, snp_size=N, triangular=False, counts=True)
+            _check(res, cnt, A, bA, B, bB, N, False, mode)
+    A, bc = qld_amd.synth_blocks(8, 4200, 16622)
+    res, cnt = e.correlate(A, bc, snp_size=16622, triangular=True, counts=True)
+    rows = np.random.default_rng(0).choice(4200, 40, replace=False)
+    want_c = oracle.counts_naive(A, A[rows])
+    for k, i in enumerate(rows):
+        j = np.arange(i + 1, 4200)
+        assert (cnt[i, j] == want_c[k, j]).all()

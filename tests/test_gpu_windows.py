@@ -1,0 +1,121 @@
+"""Batched windows on a resident table: plain (one launch per window) and
+persistent (one launch per tile shape) plans, direct and graph-replayed,
+against the oracle window by window.  Needs an MI355X."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import qld_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _windows(rng, rows, k, lo, hi):
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), k)).round().astype(np.int64)
+    sizes = np.clip(sizes, 1, rows)
+    starts = rng.integers(0, rows - sizes + 1)
+    return qld_amd.window_layout(starts, sizes)
+
+
+def _check_windows(out, wins, A, bc, N, mode="reference"):
+    f = oracle.r2_reference if mode == "reference" else oracle.r2_exact
+    for w in wins:
+        s, n, off = int(w["start"]), int(w["size"]), int(w["out_offset"])
+        if n < 2:
+            continue
+        got = out[off:off + n * n].reshape(n, n)
+        sub = np.ascontiguousarray(A[s:s + n])
+        want = f(bc[s:s + n], bc[s:s + n], oracle.counts_naive(sub), N)
+        sel = np.triu(np.ones((n, n), dtype=bool), 1)
+        bad = int((got.view(np.uint32)[sel] != want.view(np.uint32)[sel]).sum())
+        assert bad == 0, f"window start={s} size={n}: {bad} cells differ"
+
+
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_i8"])
+@pytest.mark.parametrize("mode", [qld_amd.BATCH_PERSISTENT, qld_amd.BATCH_PLAIN])
+@pytest.mark.parametrize("cutoff", [300, 0, 1 << 30])
+def test_window_plans(engine, mode, cutoff, engine_factory):
+    N, rows = 16622, 4000
+    A, bc = qld_amd.synth_blocks(4242, rows, N)
+    rng = np.random.default_rng(cutoff % 97 + mode)
+    wins = _windows(rng, rows, 40, 2, 700)
+    wins[0]["size"] = 1                     # a window without pairs
+    wins = qld_amd.window_layout(wins["start"], wins["size"])
+    total = int((wins["size"].astype(np.int64) ** 2).sum())
+    e = engine_factory(engine=engine, r2_mode="reference")
+    e.set_tile_cutoff(cutoff)
+    t = e.table(A, bc, N)
+    out = e.alloc(total * 4)
+    plan = e.plan_windows(t, wins, out, mode)
+    st = plan.stats()
+    say("plan", engine, mode, cutoff, st)
+    assert st["pairs"] == int(sum(int(s) * (int(s) - 1) // 2 for s in wins["size"]))
+    for use_graph in (True, False, True):
+        out.fill(0x7FC00000)                # NaN everywhere before each run
+        plan.use_graph(use_graph)
+        plan.run()
+        e.sync()
+        _check_windows(out.download(total, np.float32), wins, A, bc, N)
+    plan.free()
+    t.free()
+    out.free()
+
+
+def test_window_plan_exact_mode_and_samples_100k(engine_factory):
+    N, rows = 100000, 900
+    A, bc = qld_amd.synth_blocks(99, rows, N)
+    wins = _windows(np.random.default_rng(1), rows, 12, 20, 600)
+    total = int((wins["size"].astype(np.int64) ** 2).sum())
+    e = engine_factory(engine="auto", r2_mode="exact")
+    e.set_tile_cutoff(400)
+    t = e.table(A, bc, N)
+    out = e.alloc(total * 4)
+    plan = e.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
+    plan.run()
+    e.sync()
+    _check_windows(out.download(total, np.float32), wins, A, bc, N, mode="exact")
+    plan.free()
+    t.free()
+    out.free()
+
+
+def test_window_validation(engine_factory):
+    e = engine_factory(engine="auto")
+    A, bc = qld_amd.synth_blocks(5, 100, 1000)
+    t = e.table(A, bc, 1000)
+    out = e.alloc(4 * 100 * 100)
+    bad = qld_amd.window_layout([90], [20])          # runs past the table
+    with pytest.raises(qld_amd.QldError) as x:
+        e.plan_windows(t, bad, out)
+    assert x.value.code == qld_amd.QLD_ERR_INVALID
+    t.free()
+    out.free()
+
+
+def test_autotuner_picks_a_candidate(engine_factory):
+    N, rows = 16622, 6000
+    A, bc = qld_amd.synth_blocks(7, rows, N)
+    wins = _windows(np.random.default_rng(2), rows, 64, 50, 2000)
+    total = int((wins["size"].astype(np.int64) ** 2).sum())
+    e = engine_factory(engine="auto")
+    t = e.table(A, bc, N)
+    out = e.alloc(total * 4)
+    best, ms = e.autotune_windows(t, wins, out, reps=2)
+    say("autotune best cutoff", best, "ms", ms)
+    assert best in (0, 256, 384, 512, 768, 1024, 1536, 2048, 1 << 30)
+    assert e.tile_cutoff == best and ms > 0
+    # the tuned plan still computes the right answer
+    plan = e.plan_windows(t, wins, out)
+    plan.run()
+    e.sync()
+    _check_windows(out.download(total, np.float32), wins[:16], A, bc, N)
+    plan.free()
+    t.free()
+    out.free()
