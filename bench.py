@@ -85,6 +85,64 @@ def cpu_baseline(A, bc, N, budget_s=10.0):
                       f"({int(pairs)} pairs), counts + r^2, {t:.2f} s"}
 
 
+def windows_bench(args, steps):
+    """4,096 windows, sizes round(exp(U(ln 50, ln 2000))), starts uniform,
+    over a 500,000-SNP x 16,622-sample chromosome: one persistent launch per
+    tile shape (graph-replayed) against one launch per window."""
+    import qld_amd
+    N, rows, nwin = args.samples, args.win_rows, args.windows
+    seed = 20261203 + 4
+    log(f"[bench] windows: generating {rows} SNPs x {N} samples (seed {seed})")
+    A, bc = qld_amd.synth_blocks(seed, rows, N)
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(50), np.log(2000), nwin))).astype(np.int64)
+    starts = rng.integers(0, rows - sizes + 1)
+    wins = qld_amd.window_layout(starts, sizes)
+    total = int((sizes * sizes).sum())
+    pairs = int((sizes * (sizes - 1) // 2).sum())
+    eng = qld_amd.Engine(engine=args.engine, r2_mode=args.r2_mode, timing=True)
+    t = eng.table(A, bc, N)
+    out = eng.alloc(total * 4)
+    del A
+    log("[bench] windows: autotuning tile shapes")
+    cutoff, _ = eng.autotune_windows(t, wins, out, reps=2)
+    res = {"workload": f"{nwin} windows, sizes LogUniform(50, 2000), over {rows} SNPs x "
+                       f"{N} samples, per-window lower-triangle r2",
+           "windows": nwin, "pairs": pairs, "tile_cutoff": cutoff}
+
+    def timed(plan, use_graph, k):
+        plan.use_graph(use_graph)
+        plan.run()
+        eng.sync()
+        eng.timer_start()
+        for _ in range(k):
+            plan.run()
+        return eng.timer_stop() / k
+
+    batched = eng.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
+    ms_b = timed(batched, True, steps)
+    st = batched.stats()
+    batched.free()
+    plain = eng.plan_windows(t, wins, out, qld_amd.BATCH_PLAIN)
+    ms_pg = timed(plain, True, max(2, steps // 3))
+    ms_pd = timed(plain, False, max(2, steps // 3))
+    plain.free()
+    t.free()
+    out.free()
+    eng.close()
+    res.update({"batched_ms": round(ms_b, 3), "batched_gpairs_s": round(pairs / ms_b / 1e6, 2),
+                "batched_launches": st["launches"], "tiles": st["tiles"],
+                "plain_graph_ms": round(ms_pg, 3),
+                "plain_graph_gpairs_s": round(pairs / ms_pg / 1e6, 2),
+                "plain_direct_ms": round(ms_pd, 3),
+                "plain_direct_gpairs_s": round(pairs / ms_pd / 1e6, 2),
+                "speedup_vs_plain": round(ms_pd / ms_b, 2),
+                "speedup_vs_plain_graph": round(ms_pg / ms_b, 2)})
+    log(f"[bench] windows: batched {ms_b:.3f} ms, plain graph {ms_pg:.3f} ms, "
+        f"plain direct {ms_pd:.3f} ms (cutoff {cutoff})")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +154,9 @@ def main():
     ap.add_argument("--r2-mode", default="reference")
     ap.add_argument("--seed", type=int, default=20261203 + 2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-windows", action="store_true")
+    ap.add_argument("--windows", type=int, default=4096)
+    ap.add_argument("--win-rows", type=int, default=500000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -164,6 +225,9 @@ def main():
     for b in (dA, dbc, dres):
         b.free()
     eng.close()
+
+    if not args.no_windows:
+        out["batched_windows"] = windows_bench(args, max(3, args.steps))
 
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle restatement of the BLIS path)")

@@ -128,8 +128,8 @@ int qld_last_contract_ms(qld_ctx *ctx, float *ms);
 int qld_timer_start(qld_ctx *ctx);
 int qld_timer_stop(qld_ctx *ctx, float *ms);
 
-/* Tile edge of the MFMA engines for plain calls: 256, 128, or 0 to choose
- * by shape (256 once a launch has a 256-tile for every CU). */
+/* Tile edge of the MFMA engines for plain calls: 256, 128, or 0 for the
+ * default (128, measured fastest on MI355X). */
 int qld_set_plain_tile(qld_ctx *ctx, int tile);
 
 /* Engine the last call used (QLD_ENGINE_*), after AUTO resolution. */

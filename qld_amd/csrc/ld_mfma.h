@@ -21,7 +21,7 @@ struct WorkDesc {
     uint32_t *cnt;
     int64_t ldr;
     int64_t ldc;
-    int64_t row_bytes;   // multiple of the 128-byte k stage
+    int64_t row_bytes;   // multiple of 128 bytes (whole k stages)
     int ni, nj;
     int tri;
     int snp_size;

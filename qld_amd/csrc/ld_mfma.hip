@@ -2,8 +2,10 @@
 //
 // Each packed haplotype bit is first expanded to one matrix element:
 //   FP4 engine : e2m1 nibble 0x2 (= 1.0) or 0x0, two samples per byte,
-//                v_mfma_scale_f32_16x16x128_f8f6f4 (scale 2^0), fp32 sums
-//   I8 engine  : byte 0x01 or 0x00, v_mfma_i32_16x16x64_i8, int32 sums
+//                v_mfma_scale_f32_32x32x64_f8f6f4 (scale 2^0), fp32 sums
+//   I8 engine  : byte 0x01 or 0x00, v_mfma_i32_32x32x32_i8, int32 sums
+// (On gfx950 the 32x32 forms retire more work per cycle than the 16x16
+// ones: measured 8.1 vs 4.8 PFLOP/s for fp4, 3.8 vs 2.9 POP/s for int8.)
 // A count is a sum of at most N products 1*1, so fp32 accumulators are exact
 // for N < 2^24 and the counts equal the popcount engine's bit for bit.
 // The pairing of samples to K positions only has to be the same for every
@@ -11,15 +13,14 @@
 //
 // Work decomposition (both engines), tile edge T = 256 or 128
 //   workgroup : 4 wave64 (2 x 2), one T x T tile of pairs of one job
-//   wave      : T/2 x T/2 = (T/32)^2 MFMA blocks of 16 x 16 (256 or 64
+//   wave      : T/2 x T/2 = (T/64)^2 MFMA blocks of 32 x 32 (256 or 64
 //               fp32/int32 accumulators per lane)
-//   k stage   : 128 bytes of every tile row (256 fp4 / 128 int8 samples),
-//               two MFMA k-steps; stages are double-buffered in LDS and
-//               filled with buffer_load_dwordx4 ... lds (no register
-//               staging; out-of-range rows read as 0 through the buffer
-//               descriptor's bound)
-// LDS rows are 8 chunks of 16 bytes stored at chunk ^ (row & 7) so that the
-// ds_read_b128 of 16 consecutive rows by an MFMA operand fetch spreads over
+//   k stage   : SB bytes of every tile row (SB/32 MFMA k-steps) in a ring
+//               of NSTAGE LDS buffers, filled with buffer_load_dwordx4 ... lds
+//               (no register staging; out-of-range rows read as 0 through
+//               the buffer descriptor's bound); one barrier per stage
+// LDS rows are stored as 16-byte chunks permuted per row (Geo::swz) so that
+// the ds_read_b128 of consecutive rows by an MFMA operand fetch spreads over
 // all banks.
 #include "qld_internal.h"
 #include "ld_epilogue.h"
@@ -30,41 +31,75 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
-typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int SB = 128;          // bytes per row per k stage
 constexpr int THREADS = 256;     // 4 wave64 in a 2 x 2 arrangement
+constexpr int LDS_RING = 131072; // bytes of LDS for the k-stage ring (256-tiles)
 
-// Geometry of a TILE x TILE workgroup tile.
-template <int TILE>
+// Geometry of a TILE x TILE workgroup tile whose k stages carry SBY bytes of
+// every row (SBY/32 MFMA k-steps) in a ring of NST LDS buffers.
+template <int TILE, int SBY, int NST>
 struct Geo {
+    static constexpr int SB = SBY;
+    static constexpr int NSTAGE = NST;
+    static constexpr int KSTEPS = SB / 32;
+    static constexpr int CHUNKS = SB / 16;           // 16-byte chunks per LDS row
+    static constexpr int ROWS_PER_LOAD = 1024 / SB;  // rows per 1 KiB wave load
     static constexpr int WT = TILE / 2;              // wave tile edge
-    static constexpr int NB = WT / 16;               // 16 x 16 MFMA blocks per wave edge
+    static constexpr int NB = WT / 32;               // 32 x 32 MFMA blocks per wave edge
     static constexpr int ROWS_PER_WAVE = TILE / 4;   // rows each wave stages per operand
-    static constexpr int LOADS = ROWS_PER_WAVE / 8;  // 1 KiB wave loads per operand
+    static constexpr int LOADS = ROWS_PER_WAVE / ROWS_PER_LOAD;
     static constexpr int TILE_BYTES = TILE * SB;     // one operand, one stage
-    static constexpr int LDS_BYTES = 4 * TILE_BYTES; // 2 operands x 2 stages
+    static constexpr int STAGE_BYTES = 2 * TILE_BYTES;
+    static constexpr int LDS_BYTES = NSTAGE * STAGE_BYTES;
+    // Epilogue: each wave stages 32 rows of its counts at a time, plus the
+    // r^2 terms of its WT rows.
+    static constexpr int EP_PITCH = WT + 4;          // words; rows land 4 banks apart
+    static constexpr int EP_WAVE_BYTES = 32 * EP_PITCH * 4;
+    static constexpr int EP_TERMS_OFF = 4 * EP_WAVE_BYTES;
+    static constexpr int LANES_PER_ROW = WT / 4;     // each lane owns 4 columns
+    static constexpr int ROWS_PER_IT = 64 / LANES_PER_ROW;
+    static_assert(EP_TERMS_OFF + 4 * WT * 16 <= LDS_BYTES, "epilogue LDS");
+    static_assert(LOADS >= 1 && NSTAGE >= 2, "stage geometry");
+
+    // Physical 16-byte chunk of logical chunk c of an LDS row: consecutive
+    // rows reading the same logical chunk hit distinct bank groups (8 rows
+    // per 128 bytes of banks).  Its own inverse.
+    static __device__ __forceinline__ int swz(int row, int c)
+    {
+        return CHUNKS == 8 ? (c ^ (row & 7)) : (c ^ ((row >> 1) & 3));
+    }
 };
 
+// Kernel configurations: (tile, stage bytes, ring depth).
+#define QLD_CFG_L256 256, 128, 2
+#define QLD_CFG_S128 128, 128, 2
+
+// v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (scale 2^0): on gfx950
+// it retires twice the flops per cycle of the 16x16x128 form.
 struct FP4 {
-    typedef v4f acc_t;
+    typedef v16f acc_t;
     static __device__ __forceinline__ acc_t mma(v4i a, v4i b, acc_t c)
     {
         v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
         v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
-        return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0,
-                                                                 127, 0, 127);
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, 127,
+                                                                0, 127);
     }
-    static __device__ __forceinline__ float count(float v) { return v; }
+    // Counts travel through LDS as raw 32-bit words.
+    static __device__ __forceinline__ uint32_t bits(float v) { return __float_as_uint(v); }
+    static __device__ __forceinline__ float count(uint32_t w) { return __uint_as_float(w); }
 };
 
 struct I8 {
-    typedef v4i acc_t;
+    typedef v16i acc_t;
     static __device__ __forceinline__ acc_t mma(v4i a, v4i b, acc_t c)
     {
-        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ float count(int v) { return (float)v; }
+    static __device__ __forceinline__ uint32_t bits(int v) { return (uint32_t)v; }
+    static __device__ __forceinline__ float count(uint32_t w) { return (float)(int)w; }
 };
 
 struct KArgs {
@@ -90,21 +125,20 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t panel_rsrc(const uint8_t *base
 }
 
 // Queue one k stage of one operand: wave w fills tile rows
-// [w*R, w*R + R), R = TILE/4.  Lane L of load q lands in LDS slot
-// (w*R + 8q)*8 + L, i.e. row w*R + 8q + L/8, physical chunk L%8, which holds
-// logical chunk (L%8) ^ (L/8).
-template <int TILE>
+// [w*R, w*R + R), R = TILE/4.  A wave load covers 1 KiB = ROWS_PER_LOAD rows
+// x CHUNKS chunks; lane L lands at row L/CHUNKS, physical chunk L%CHUNKS,
+// which holds logical chunk swz(row, L%CHUNKS).
+template <class G>
 __device__ __forceinline__ void stage_load(__amdgpu_buffer_rsrc_t rs, uint8_t *lds_tile,
                                            int64_t row_bytes, int k0, int wave, int lane)
 {
-    typedef Geo<TILE> G;
-    const int r_in = lane >> 3, pc = lane & 7;
-    const int c = pc ^ r_in;
+    const int r_in = lane / G::CHUNKS, pc = lane % G::CHUNKS;
+    const int c = G::swz(r_in, pc);
 #pragma unroll
     for (int q = 0; q < G::LOADS; ++q) {
-        int row = wave * G::ROWS_PER_WAVE + q * 8 + r_in;
+        int row = wave * G::ROWS_PER_WAVE + q * G::ROWS_PER_LOAD + r_in;
         uint32_t voff = (uint32_t)((int64_t)row * row_bytes + k0 + c * 16);
-        uint8_t *dst = lds_tile + (wave * G::ROWS_PER_WAVE + q * 8) * SB;
+        uint8_t *dst = lds_tile + (wave * G::ROWS_PER_WAVE + q * G::ROWS_PER_LOAD) * G::SB;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)dst, 16, voff, 0, 0, 0);
     }
 }
@@ -115,6 +149,15 @@ __device__ __forceinline__ void wait_vm()
     asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
+// Wait until at most `ahead` later stages (2*LOADS loads each) are pending.
+template <int PER_STAGE>
+__device__ __forceinline__ void wait_stage(int ahead)
+{
+    if (ahead >= 2) wait_vm<2 * PER_STAGE>();
+    else if (ahead == 1) wait_vm<PER_STAGE>();
+    else wait_vm<0>();
+}
+
 // Workgroup barrier that also orders LDS traffic: this wave's outstanding
 // ds_reads complete first, and the compiler may not move memory accesses
 // across it (the bare s_barrier builtin orders neither).
@@ -123,15 +166,95 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// One tile of one job: k loop over double-buffered LDS stages, then the
-// fused epilogue straight to the job's result rows.
-template <class OP, int MODE, int TILE>
+// r^2 of 4 consecutive cells of one row.
+template <int MODE>
+struct Epi;
+
+// Row terms travel through LDS as 16 bytes.
+union TermBits {
+    uint4 u;
+    struct {
+        float f;
+        uint32_t z;
+        double d;
+    } s;
+};
+
+template <>
+struct Epi<QLD_R2_REFERENCE> {
+    typedef RefTerm term;
+    static __device__ __forceinline__ term make(uint32_t bc, const WorkDesc &d)
+    {
+        return ref_term(bc, d.fN);
+    }
+    static __device__ __forceinline__ uint4 pack(const term &t)
+    {
+        TermBits b;
+        b.s.f = t.p;
+        b.s.z = t.zero;
+        b.s.d = t.omp;
+        return b.u;
+    }
+    static __device__ __forceinline__ term unpack(uint4 u)
+    {
+        TermBits b;
+        b.u = u;
+        term t;
+        t.p = b.s.f;
+        t.zero = b.s.z != 0;
+        t.omp = b.s.d;
+        return t;
+    }
+    static __device__ __forceinline__ float r2(float c, const term &col, const term &row,
+                                               const WorkDesc &d)
+    {
+        return r2_reference(c, col, row, d.invN);
+    }
+};
+
+template <>
+struct Epi<QLD_R2_EXACT> {
+    typedef ExactTerm term;
+    static __device__ __forceinline__ term make(uint32_t bc, const WorkDesc &d)
+    {
+        return exact_term(bc, d.dN);
+    }
+    // bc fits a float exactly (< 2^24).
+    static __device__ __forceinline__ uint4 pack(const term &t)
+    {
+        TermBits b;
+        b.s.f = (float)t.bc;
+        b.s.z = t.zero;
+        b.s.d = t.rv;
+        return b.u;
+    }
+    static __device__ __forceinline__ term unpack(uint4 u)
+    {
+        TermBits b;
+        b.u = u;
+        term t;
+        t.bc = (double)b.s.f;
+        t.zero = b.s.z != 0;
+        t.rv = b.s.d;
+        return t;
+    }
+    static __device__ __forceinline__ float r2(float c, const term &col, const term &row,
+                                               const WorkDesc &d)
+    {
+        return r2_exact(c, col, row, d.dN);
+    }
+};
+
+// One tile of one job: k loop over a ring of LDS stages, then the fused
+// epilogue straight to the job's result rows.
+template <class OP, int MODE, class G>
 __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uint8_t *smem)
 {
-    typedef Geo<TILE> G;
+    typedef Epi<MODE> E;
+    constexpr int TILE = 2 * G::WT, SB = G::SB, NSTAGE = G::NSTAGE, KSTEPS = G::KSTEPS;
     constexpr int NB = G::NB, WT = G::WT;
-    auto sA = [&](int b) { return smem + (2 * b) * G::TILE_BYTES; };
-    auto sB = [&](int b) { return smem + (2 * b + 1) * G::TILE_BYTES; };
+    auto sA = [&](int b) { return smem + b * G::STAGE_BYTES; };
+    auto sB = [&](int b) { return smem + b * G::STAGE_BYTES + G::TILE_BYTES; };
 
     const int i0 = ti * TILE, j0 = tj * TILE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -143,107 +266,141 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uin
     const __amdgpu_buffer_rsrc_t rsI = panel_rsrc<TILE>(d.XI, i0, d.ni, rb);
     const __amdgpu_buffer_rsrc_t rsJ = panel_rsrc<TILE>(d.XJ, j0, d.nj, rb);
 
+    // Bit counts of this lane's epilogue rows/columns, fetched now so their
+    // latency hides under the k loop.
+    constexpr int RPL = WT / 64 > 0 ? WT / 64 : 1;      // rows per lane (WT = 128 or 64)
+    uint32_t bc_row[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        int i = i0 + wm * WT + q * 64 + lane;
+        bc_row[q] = (lane + q * 64 < WT && i < d.ni) ? d.bcI[i] : 0u;
+    }
+    const int cq = lane % G::LANES_PER_ROW, rq = lane / G::LANES_PER_ROW;
+    const int jc = j0 + wn * WT + 4 * cq;               // first of this lane's 4 columns
+    uint32_t bc_col[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? d.bcJ[jc + q] : 0u;
+
     typename OP::acc_t acc[NB][NB];
 #pragma unroll
     for (int a = 0; a < NB; ++a)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = typename OP::acc_t{0, 0, 0, 0};
-
-    stage_load<TILE>(rsI, sA(0), rb, 0, wave, lane);
-    stage_load<TILE>(rsJ, sB(0), rb, 0, wave, lane);
-
-    const int r16 = lane & 15, g = lane >> 4;
-    for (int s = 0; s < stages; ++s) {
-        const int cur = s & 1;
-        if (s + 1 < stages) {
-            stage_load<TILE>(rsI, sA(cur ^ 1), rb, (s + 1) * SB, wave, lane);
-            stage_load<TILE>(rsJ, sB(cur ^ 1), rb, (s + 1) * SB, wave, lane);
-            wait_vm<2 * G::LOADS>();
-        } else {
-            wait_vm<0>();
-        }
-        lds_barrier();
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            const int c = ks * 4 + g;
-            v4i bv[NB];
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+
+#pragma unroll
+    for (int p = 0; p < NSTAGE - 1; ++p)
+        if (p < stages) {
+            stage_load<G>(rsI, sA(p), rb, p * SB, wave, lane);
+            stage_load<G>(rsJ, sB(p), rb, p * SB, wave, lane);
+        }
+
+    // 32x32 operand fetch: lane l reads 16 bytes of row l%32, chunk
+    // 2*ks + l/32 of the stage (the same chunk for both operands).
+    const int r32 = lane & 31, half = lane >> 5;
+    for (int s = 0; s < stages; ++s) {
+        const int cur = s % NSTAGE;
+        int ahead = stages - 1 - s;
+        wait_stage<2 * G::LOADS>(ahead < NSTAGE - 2 ? ahead : NSTAGE - 2);
+        lds_barrier();
+        if (s + NSTAGE - 1 < stages) {
+            const int nxt = (s + NSTAGE - 1) % NSTAGE;
+            stage_load<G>(rsI, sA(nxt), rb, (s + NSTAGE - 1) * SB, wave, lane);
+            stage_load<G>(rsJ, sB(nxt), rb, (s + NSTAGE - 1) * SB, wave, lane);
+        }
+        v4i av[KSTEPS][NB], bv[KSTEPS][NB];
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            const int c = 2 * ks + half;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                int row = wn * WT + b * 16 + r16;
-                bv[b] = *(const v4i *)(sB(cur) + row * SB + ((c ^ (row & 7)) << 4));
+                int row = wn * WT + b * 32 + r32;
+                bv[ks][b] = *(const v4i *)(sB(cur) + row * SB + (G::swz(row, c) << 4));
             }
 #pragma unroll
             for (int a = 0; a < NB; ++a) {
-                int row = wm * WT + a * 16 + r16;
-                v4i av = *(const v4i *)(sA(cur) + row * SB + ((c ^ (row & 7)) << 4));
-#pragma unroll
-                for (int b = 0; b < NB; ++b) acc[a][b] = OP::mma(av, bv[b], acc[a][b]);
+                int row = wm * WT + a * 32 + r32;
+                av[ks][a] = *(const v4i *)(sA(cur) + row * SB + (G::swz(row, c) << 4));
             }
         }
-        lds_barrier();
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks)
+#pragma unroll
+            for (int a = 0; a < NB; ++a)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    acc[a][b] = OP::mma(av[ks][a], bv[ks][b], acc[a][b]);
     }
+    lds_barrier();       // every wave is done with the ring before the epilogue reuses it
 
-    // Fused epilogue.  Block (a, b) register v holds the pair
-    // i = i0 + wm*WT + a*16 + 4*(lane/16) + v,  j = j0 + wn*WT + b*16 + lane%16.
-    const int jb = j0 + wn * WT + r16;
-    if (MODE == QLD_R2_REFERENCE) {
-        RefTerm tj_[NB];
+    // Fused epilogue.  Block (a, b) register r holds the pair
+    //   row wm*WT + a*32 + 8*(r/4) + 4*(lane/32) + r%4, col wn*WT + b*32 + lane%32
+    // of the tile.  The r^2 terms of the wave's WT rows go to LDS once; then
+    // for each a the wave stages those 32 rows of counts in its own LDS
+    // region and every lane turns 4 consecutive counts of a row into r^2,
+    // written with one 16-byte store (rows of WT floats coalesce).
+    uint32_t *ep = (uint32_t *)(smem + wave * G::EP_WAVE_BYTES);
+    uint4 *terms = (uint4 *)(smem + G::EP_TERMS_OFF) + wave * WT;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            int j = jb + b * 16;
-            tj_[b] = ref_term(j < d.nj ? d.bcJ[j] : 0u, d.fN);
-        }
+    for (int q = 0; q < RPL; ++q)
+        if (lane + q * 64 < WT) terms[q * 64 + lane] = E::pack(E::make(bc_row[q], d));
+    typename E::term col[4];
 #pragma unroll
-        for (int a = 0; a < NB; ++a)
+    for (int q = 0; q < 4; ++q) col[q] = E::make(bc_col[q], d);
+    const bool vec_ok = d.res && (d.ldr & 3) == 0 && (((uintptr_t)d.res) & 15) == 0;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int i = i0 + wm * WT + a * 16 + 4 * g + v;
-                if (i >= d.ni) continue;
-                RefTerm ti_ = ref_term(d.bcI[i], d.fN);
+    for (int a = 0; a < NB; ++a) {
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    int j = jb + b * 16;
-                    if (j >= d.nj || (d.tri && j <= i)) continue;
-                    float c = OP::count(acc[a][b][v]);
-                    if (d.res) d.res[(int64_t)i * d.ldr + j] = r2_reference(c, tj_[b], ti_, d.invN);
-                    if (d.cnt) d.cnt[(int64_t)i * d.ldc + j] = (uint32_t)c;
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                int row = 8 * (r >> 2) + 4 * half + (r & 3);
+                ep[row * G::EP_PITCH + b * 32 + r32] = OP::bits(acc[a][b][r]);
+            }
+        const int ibase = i0 + wm * WT + a * 32;
+#pragma unroll 2
+        for (int it = 0; it < 32 / G::ROWS_PER_IT; ++it) {
+            const int rl = it * G::ROWS_PER_IT + rq;
+            const int i = ibase + rl;
+            const uint4 c4 = *(const uint4 *)(ep + rl * G::EP_PITCH + 4 * cq);
+            const typename E::term rowt = E::unpack(terms[a * 32 + rl]);
+            if (i >= d.ni) continue;
+            const float cv[4] = {OP::count(c4.x), OP::count(c4.y), OP::count(c4.z),
+                                 OP::count(c4.w)};
+            float rv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rv[q] = E::r2(cv[q], col[q], rowt, d);
+            const bool full = jc + 3 < d.nj && (!d.tri || jc > i);
+            if (d.res) {
+                float *dst = d.res + (int64_t)i * d.ldr + jc;
+                if (full && vec_ok) {
+                    *(float4 *)dst = make_float4(rv[0], rv[1], rv[2], rv[3]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (jc + q < d.nj && (!d.tri || jc + q > i)) dst[q] = rv[q];
                 }
             }
-    } else {
-        ExactTerm tj_[NB];
+            if (d.cnt) {
+                uint32_t *dst = d.cnt + (int64_t)i * d.ldc + jc;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            int j = jb + b * 16;
-            tj_[b] = exact_term(j < d.nj ? d.bcJ[j] : 0u, d.dN);
-        }
-#pragma unroll
-        for (int a = 0; a < NB; ++a)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int i = i0 + wm * WT + a * 16 + 4 * g + v;
-                if (i >= d.ni) continue;
-                ExactTerm ti_ = exact_term(d.bcI[i], d.dN);
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    int j = jb + b * 16;
-                    if (j >= d.nj || (d.tri && j <= i)) continue;
-                    float c = OP::count(acc[a][b][v]);
-                    if (d.res) d.res[(int64_t)i * d.ldr + j] = r2_exact(c, tj_[b], ti_, d.dN);
-                    if (d.cnt) d.cnt[(int64_t)i * d.ldc + j] = (uint32_t)c;
-                }
+                for (int q = 0; q < 4; ++q)
+                    if (jc + q < d.nj && (!d.tri || jc + q > i)) dst[q] = (uint32_t)cv[q];
             }
+        }
     }
 }
 
-template <class OP, int MODE, int TILE>
+template <class OP, int MODE, class G>
 __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, uint8_t *smem)
 {
     const WorkItem it = p.items[idx];
     if (p.descs) {
         const WorkDesc d = p.descs[it.desc];
-        mfma_tile<OP, MODE, TILE>(d, it.ti, it.tj, smem);
+        mfma_tile<OP, MODE, G>(d, it.ti, it.tj, smem);
     } else {
-        mfma_tile<OP, MODE, TILE>(p.inline_desc, it.ti, it.tj, smem);
+        mfma_tile<OP, MODE, G>(p.inline_desc, it.ti, it.tj, smem);
     }
 }
 
@@ -252,12 +409,12 @@ __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, uint8_t *s
 // whose items are a contiguous range of the list, then steals from the
 // other queues.  Every claim advances a head, so the loop ends once all
 // eight ranges are exhausted.
-template <class OP, int MODE, int TILE, int PERSIST>
+template <class OP, int MODE, class G, int PERSIST>
 __global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(KArgs p)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     if (!PERSIST) {
-        run_item<OP, MODE, TILE>(p, xcd_logical(blockIdx.x, gridDim.x), smem);
+        run_item<OP, MODE, G>(p, xcd_logical(blockIdx.x, gridDim.x), smem);
         return;
     }
     __shared__ int64_t claimed;
@@ -275,7 +432,7 @@ __global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(KArgs p)
             const int64_t idx = claimed;
             __syncthreads();
             if (idx < 0) break;
-            run_item<OP, MODE, TILE>(p, idx, smem);
+            run_item<OP, MODE, G>(p, idx, smem);
         }
     }
 }
@@ -337,7 +494,7 @@ __global__ void expand_i8_kernel(const uint64_t *T, int rows, int W, int Wp,
 
 int mfma_padded_words(int engine, int words)
 {
-    // A k stage is 128 bytes of a row: 4 packed words as fp4, 2 as int8.
+    // Rows hold whole 128-byte k stages: 4 packed words as fp4, 2 as int8.
     int q = engine == QLD_ENGINE_MFMA_FP4 ? 4 : 2;
     return (words + q - 1) / q * q;
 }
@@ -418,22 +575,25 @@ __global__ void zero_queue_kernel(uint32_t *q)
     if (threadIdx.x < 16) q[threadIdx.x] = 0u;
 }
 
-template <class OP, int MODE, int TILE, int PERSIST>
+template <class OP, int MODE, class G, int PERSIST>
 static hipError_t set_lds_limit()
 {
-    return hipFuncSetAttribute((const void *)ld_mfma_kernel<OP, MODE, TILE, PERSIST>,
+    return hipFuncSetAttribute((const void *)ld_mfma_kernel<OP, MODE, G, PERSIST>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)Geo<TILE>::LDS_BYTES);
+                               (int)G::LDS_BYTES);
 }
+
+typedef Geo<QLD_CFG_L256> GeoL;
+typedef Geo<QLD_CFG_S128> GeoS;
 
 template <class OP, int MODE>
 static hipError_t set_lds_limits_mode()
 {
     hipError_t e;
-    if ((e = set_lds_limit<OP, MODE, QLD_TILE_LARGE, 0>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<OP, MODE, QLD_TILE_LARGE, 1>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<OP, MODE, QLD_TILE_SMALL, 0>()) != hipSuccess) return e;
-    return set_lds_limit<OP, MODE, QLD_TILE_SMALL, 1>();
+    if ((e = set_lds_limit<OP, MODE, GeoL, 0>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, GeoL, 1>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, GeoS, 0>()) != hipSuccess) return e;
+    return set_lds_limit<OP, MODE, GeoS, 1>();
 }
 
 // Raise the dynamic LDS limit of every kernel variant (done once per
@@ -447,27 +607,27 @@ hipError_t mfma_init_attributes()
     return set_lds_limits_mode<I8, QLD_R2_EXACT>();
 }
 
-template <class OP, int MODE, int TILE, int PERSIST>
+template <class OP, int MODE, class G, int PERSIST>
 static hipError_t launch_one(const KArgs &a, unsigned grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE, TILE, PERSIST>), dim3(grid), dim3(THREADS),
-                       (size_t)Geo<TILE>::LDS_BYTES, s, a);
+    hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE, G, PERSIST>), dim3(grid), dim3(THREADS),
+                       (size_t)G::LDS_BYTES, s, a);
     return hipGetLastError();
 }
 
-template <class OP, int MODE, int TILE>
+template <class OP, int MODE, class G>
 static hipError_t launch_mode(const KArgs &a, int persistent, hipStream_t s)
 {
     if (persistent > 0)
-        return launch_one<OP, MODE, TILE, 1>(a, (unsigned)persistent, s);
-    return launch_one<OP, MODE, TILE, 0>(a, (unsigned)a.nitems, s);
+        return launch_one<OP, MODE, G, 1>(a, (unsigned)persistent, s);
+    return launch_one<OP, MODE, G, 0>(a, (unsigned)a.nitems, s);
 }
 
 template <class OP, int MODE>
 static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStream_t s)
 {
-    if (tile == QLD_TILE_SMALL) return launch_mode<OP, MODE, QLD_TILE_SMALL>(a, persistent, s);
-    return launch_mode<OP, MODE, QLD_TILE_LARGE>(a, persistent, s);
+    if (tile == QLD_TILE_SMALL) return launch_mode<OP, MODE, GeoS>(a, persistent, s);
+    return launch_mode<OP, MODE, GeoL>(a, persistent, s);
 }
 
 hipError_t launch_mfma_items(int engine, int r2_mode, int tile,
