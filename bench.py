@@ -143,6 +143,117 @@ def windows_bench(args, steps):
     return res
 
 
+def rect_bench(args, steps):
+    """Region A 8,000 SNPs x region B 12,000 SNPs over 100,000 samples
+    (Beta(0.5, 2) MAF, 5% of B copied from A with 2% bit flips): full
+    rectangle of FP32 r2 tiles."""
+    import qld_amd
+    N, m, n = 100000, 8000, 12000
+    seed = 20261203 + 3
+    log(f"[bench] rect: generating {m} x {n} SNPs over {N} samples (seed {seed})")
+    A, bA, B, bB = qld_amd.synth_regions(seed, m, n, N)
+    W = A.shape[1]
+    eng = qld_amd.Engine(engine=args.engine, r2_mode=args.r2_mode, timing=True)
+    dA = eng.alloc(A.nbytes).upload(A)
+    dB = eng.alloc(B.nbytes).upload(B)
+    dbA = eng.alloc(bA.nbytes).upload(bA)
+    dbB = eng.alloc(bB.nbytes).upload(bB)
+    dres = eng.alloc(m * n * 4)
+
+    def step():
+        eng.correlate_dev(dA, dbA, m, dB, dbB, n, W, N, False, dres, m)
+    step()
+    eng.sync()
+    eng.timer_start()
+    for _ in range(steps):
+        step()
+    ms = eng.timer_stop() / steps
+    kms = []
+    for _ in range(3):
+        step()
+        kms.append(eng.last_contract_ms())
+    engine = eng.last_engine
+    for b in (dA, dB, dbA, dbB, dres):
+        b.free()
+    eng.close()
+    pairs = m * n
+    k_s = float(np.mean(kms)) / 1e3
+    log(f"[bench] rect: {ms:.3f} ms/step ({engine})")
+    return {"workload": f"region A {m} SNPs x region B {n} SNPs over {N} samples, "
+                        "full rectangle FP32 r2", "pairs": pairs, "engine": engine,
+            "ms_per_step": round(ms, 3), "gpairs_s": round(pairs / ms / 1e6, 2),
+            "roofline": roofline(engine, pairs, W, N, k_s)}
+
+
+def serving_bench(args):
+    """Poisson arrivals of LD queries (window size LogUniform(20, 5000), a
+    uniformly chosen resident chromosome, uniform start) against three
+    16,622-sample and two 200,000-sample chromosomes; per rate: p50/p99
+    latency (submit -> r2 ready in HBM) and pairs/s."""
+    import qld_amd
+    rates = [float(x) for x in args.serving_rates.split(",")]
+    nreq = args.serving_requests
+    rows16, rows200 = args.serving_rows16, args.serving_rows200
+    eng = qld_amd.Engine(engine="auto", r2_mode=args.r2_mode)
+    tabs = []
+    specs = [(16622, rows16)] * 3 + [(200000, rows200)] * 2
+    for k, (N, rows) in enumerate(specs):
+        log(f"[bench] serving: chromosome {k}: {rows} SNPs x {N} samples")
+        A, bc = qld_amd.synth_blocks(20261203 + 50 + k, rows, N)
+        tabs.append((eng.table(A, bc, N), rows, N))
+        del A
+    srv = qld_amd.Server(eng, out_budget=8 << 30, max_batch=128, streams=3)
+    rng = np.random.default_rng(20261203 + 5)
+    results = []
+    # warm-up
+    ids = [srv.submit(tabs[0][0], 0, 1000, qld_amd.REQ_DISCARD) for _ in range(4)]
+    for i in ids:
+        srv.wait(i)
+        srv.release(i)
+    for rate in rates:
+        gaps = rng.exponential(1.0 / rate, nreq)
+        which = rng.integers(0, len(tabs), nreq)
+        sizes = np.rint(np.exp(rng.uniform(np.log(20), np.log(5000), nreq))).astype(int)
+        sizes = np.minimum(sizes, [tabs[w][1] for w in which])
+        starts = [int(rng.integers(0, tabs[w][1] - z + 1)) for w, z in zip(which, sizes)]
+        ids = []
+        t0 = time.perf_counter()
+        due = t0
+        for k in range(nreq):
+            due += gaps[k]
+            while time.perf_counter() < due:
+                pass
+            ids.append(srv.submit(tabs[which[k]][0], starts[k], int(sizes[k]),
+                                  qld_amd.REQ_DISCARD))
+        lat = []
+        for i in ids:
+            lat.append(srv.wait(i))
+            srv.release(i)
+        span = time.perf_counter() - t0
+        lat = np.array(lat)
+        pairs = int((sizes * (sizes - 1) // 2).sum())
+        pairs16 = int(sum(z * (z - 1) // 2 for w, z in zip(which, sizes) if tabs[w][2] == 16622))
+        res = {"rate_req_s": rate, "requests": nreq,
+               "achieved_req_s": round(nreq / span, 1),
+               "p50_ms": round(float(np.percentile(lat, 50)), 3),
+               "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "mean_ms": round(float(lat.mean()), 3),
+               "gpairs_s": round(pairs / span / 1e9, 3),
+               "gpairs_s_16622": round(pairs16 / span / 1e9, 3)}
+        log(f"[bench] serving {rate:g} req/s: p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, "
+            f"{res['achieved_req_s']} req/s")
+        results.append(res)
+    st = srv.stats()
+    srv.close()
+    for t, _, _ in tabs:
+        t.free()
+    eng.close()
+    return {"workload": f"Poisson LD queries, windows LogUniform(20, 5000) over 3 x {rows16} "
+                        f"SNPs x 16,622 and 2 x {rows200} SNPs x 200,000 samples, resident; "
+                        "latency = submit to r2 ready in HBM",
+            "rates": results, "batches": st["batches"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +266,12 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203 + 2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-windows", action="store_true")
+    ap.add_argument("--no-rect", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-rates", default="100,1000,5000")
+    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-rows16", type=int, default=100000)
+    ap.add_argument("--serving-rows200", type=int, default=20000)
     ap.add_argument("--windows", type=int, default=4096)
     ap.add_argument("--win-rows", type=int, default=500000)
     args = ap.parse_args()
@@ -228,6 +345,10 @@ def main():
 
     if not args.no_windows:
         out["batched_windows"] = windows_bench(args, max(3, args.steps))
+    if not args.no_rect:
+        out["rect_100k"] = rect_bench(args, max(3, args.steps // 2))
+    if not args.no_serving:
+        out["serving"] = serving_bench(args)
 
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle restatement of the BLIS path)")

@@ -189,6 +189,55 @@ int qld_plan_stats(const qld_plan *plan, int64_t *tiles, int64_t *pairs,
                    int *launches);
 int qld_plan_free(qld_ctx *ctx, qld_plan *plan);
 
+/* ---- serving ----------------------------------------------------------------- */
+/* A request loop for LD queries against resident tables.  Each request is a
+ * window (rows [start, start + size) of one table) whose r^2 triangle is
+ * computed on the device.  A dispatcher thread admits pending requests in
+ * arrival order while their outputs fit the output budget, packs them into
+ * one launch (one job descriptor per request), and keeps up to `streams`
+ * such batches in flight on separate HIP streams, so requests that arrive
+ * while others run are admitted as slots and memory free up.  Each result
+ * is bit-identical whatever it was batched with.  No counterpart in the
+ * reference (its tasks run one by one, correlate_function_gpu.c:55-81). */
+typedef struct qld_server qld_server;
+
+typedef struct qld_server_cfg {
+    size_t out_budget;   /* device bytes for outputs in flight (0 = 4 GiB)   */
+    int    max_batch;    /* requests per launch (0 = 256)                    */
+    int    streams;      /* batches in flight (0 = 3)                        */
+    int    reserved;
+} qld_server_cfg;
+
+/* Request flags. */
+#define QLD_REQ_KEEP    0  /* result stays on the device until fetched      */
+#define QLD_REQ_COPY    1  /* copy into host_out (pinned memory from
+                              qld_host_alloc is fastest), then release     */
+#define QLD_REQ_DISCARD 2  /* release at completion (latency measurement)  */
+
+int qld_server_create(qld_ctx *ctx, const qld_server_cfg *cfg, qld_server **out);
+/* Stop accepting, finish everything in flight, free all results. */
+int qld_server_destroy(qld_server *s);
+/* Queue a request (thread-safe, non-blocking).  QLD_ERR_BUDGET if its
+ * output (size*size floats) can never fit the budget. */
+int qld_server_submit(qld_server *s, const qld_table *t, int start, int size,
+                      int flags, float *host_out, int64_t *req_id);
+/* Wait up to timeout_ms (< 0: forever) for completion; QLD_ERR_NOTREADY on
+ * timeout.  latency_ms (optional) = submit to result ready. */
+int qld_server_wait(qld_server *s, int64_t req_id, double timeout_ms,
+                    double *latency_ms);
+/* Copy a completed QLD_REQ_KEEP result (size*size floats, cells j > i
+ * defined) to host memory and release it. */
+int qld_server_fetch(qld_server *s, int64_t req_id, float *dst);
+/* Forget a completed request (and release a kept result). */
+int qld_server_release(qld_server *s, int64_t req_id);
+/* Totals since creation: requests completed, batches launched, pairs. */
+int qld_server_stats(const qld_server *s, int64_t *completed, int64_t *batches,
+                     int64_t *pairs);
+
+/* Pinned host memory for QLD_REQ_COPY destinations and fast copies. */
+int qld_host_alloc(qld_ctx *ctx, size_t bytes, void **ptr);
+int qld_host_free(qld_ctx *ctx, void *ptr);
+
 /* Tile-shape policy of window plans: windows of at least `cutoff` SNPs use
  * 256 x 256 tiles, smaller ones 128 x 128 (0 = all large). */
 int qld_set_tile_cutoff(qld_ctx *ctx, int cutoff);

@@ -83,11 +83,32 @@ GPU_SYMBOLS = [
     ("qld_plan_stats", C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int)]),
     ("qld_plan_free", C.c_int, [_vp, _vp]),
+    ("qld_server_create", C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    ("qld_server_destroy", C.c_int, [_vp]),
+    ("qld_server_submit", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                    C.POINTER(C.c_int64)]),
+    ("qld_server_wait", C.c_int, [_vp, C.c_int64, C.c_double, C.POINTER(C.c_double)]),
+    ("qld_server_fetch", C.c_int, [_vp, C.c_int64, _vp]),
+    ("qld_server_release", C.c_int, [_vp, C.c_int64]),
+    ("qld_server_stats", C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64)]),
+    ("qld_host_alloc", C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    ("qld_host_free", C.c_int, [_vp, _vp]),
     ("qld_set_tile_cutoff", C.c_int, [_vp, C.c_int]),
     ("qld_get_tile_cutoff", C.c_int, [_vp]),
     ("qld_autotune_windows", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int,
                                        C.POINTER(C.c_int), _f32p]),
 ]
+
+REQ_KEEP = 0
+REQ_COPY = 1
+REQ_DISCARD = 2
+
+
+class ServerCfg(C.Structure):
+    _fields_ = [("out_budget", C.c_size_t), ("max_batch", C.c_int),
+                ("streams", C.c_int), ("reserved", C.c_int)]
+
 
 WINDOW_DTYPE = np.dtype([("start", "<i4"), ("size", "<i4"), ("out_offset", "<i8")])
 BATCH_PLAIN = 0
@@ -416,3 +437,51 @@ def window_layout(starts, sizes) -> np.ndarray:
     w["size"] = sizes
     w["out_offset"] = np.concatenate([[0], np.cumsum(sizes * sizes)[:-1]]) if len(sizes) else []
     return w
+
+
+class Server:
+    """Continuous-batching request loop over resident tables (qld_server_*)."""
+
+    def __init__(self, eng: Engine, out_budget=0, max_batch=0, streams=0):
+        self.eng = eng
+        cfg = ServerCfg(out_budget, max_batch, streams, 0)
+        p = C.c_void_p()
+        eng._check(hip_lib().qld_server_create(eng._ctx, C.byref(cfg), C.byref(p)))
+        self._s = p
+
+    def submit(self, table: Table, start: int, size: int, flags=REQ_KEEP, host_out=None):
+        rid = C.c_int64()
+        rc = hip_lib().qld_server_submit(self._s, table._t, int(start), int(size), int(flags),
+                                         host_out, C.byref(rid))
+        if rc != QLD_OK:
+            raise QldError(rc, f"qld_server_submit({start}, {size}) failed")
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: float = -1.0) -> float:
+        lat = C.c_double()
+        rc = hip_lib().qld_server_wait(self._s, rid, timeout_ms, C.byref(lat))
+        if rc != QLD_OK:
+            raise QldError(rc, f"qld_server_wait({rid}) failed")
+        return lat.value
+
+    def fetch(self, rid: int, size: int) -> np.ndarray:
+        out = np.empty((size, size), dtype=np.float32)
+        rc = hip_lib().qld_server_fetch(self._s, rid, _ptr(out))
+        if rc != QLD_OK:
+            raise QldError(rc, f"qld_server_fetch({rid}) failed")
+        return out
+
+    def release(self, rid: int):
+        rc = hip_lib().qld_server_release(self._s, rid)
+        if rc != QLD_OK:
+            raise QldError(rc, f"qld_server_release({rid}) failed")
+
+    def stats(self):
+        c, b, p = C.c_int64(), C.c_int64(), C.c_int64()
+        rc = hip_lib().qld_server_stats(self._s, C.byref(c), C.byref(b), C.byref(p))
+        return {"completed": c.value, "batches": b.value, "pairs": p.value, "status": rc}
+
+    def close(self):
+        if self._s:
+            hip_lib().qld_server_destroy(self._s)
+            self._s = None

@@ -17,15 +17,6 @@
 #include <string.h>
 #include <vector>
 
-struct qld_table {
-    int rows = 0, words = 0, snp_size = 0, engine = QLD_ENGINE_MFMA_FP4;
-    int64_t row_bytes = 0;
-    uint64_t *packed = nullptr;
-    uint32_t *bc = nullptr;
-    uint8_t *X = nullptr;
-    size_t bytes_packed = 0, bytes_bc = 0, bytes_x = 0;
-};
-
 struct PlanClass {
     int tile = QLD_TILE_LARGE;
     int grid = 0;                   // persistent workgroups

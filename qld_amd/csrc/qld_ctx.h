@@ -4,6 +4,7 @@
 #include "qld_internal.h"
 #include "ld_mfma.h"
 
+#include <mutex>
 #include <new>
 #include <unordered_map>
 #include <vector>
@@ -30,6 +31,7 @@ struct qld_ctx {
     bool timed = false;
     int last_engine = QLD_ENGINE_AUTO;
     size_t budget = 0, used = 0;
+    std::mutex alloc_mu;     // guards `used`: a server's dispatcher allocates too
     std::unordered_map<void *, size_t> allocs;   // qld_dev_alloc'd blocks
     DevBuf host_stage;   // staging for the host-buffer entry point
     DevBuf ws;           // engine workspace (expanded operands)
@@ -39,6 +41,16 @@ struct qld_ctx {
     int tile_cutoff = 1024;   // windows of at least this many SNPs use 256-tiles
     int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape
     char err[512] = {0};
+};
+
+// A resident table: packed rows, bit counts and the expanded MFMA operand.
+struct qld_table {
+    int rows = 0, words = 0, snp_size = 0, engine = QLD_ENGINE_MFMA_FP4;
+    int64_t row_bytes = 0;
+    uint64_t *packed = nullptr;
+    uint32_t *bc = nullptr;
+    uint8_t *X = nullptr;
+    size_t bytes_packed = 0, bytes_bc = 0, bytes_x = 0;
 };
 
 void qld_set_error(qld_ctx *ctx, const char *fmt, ...);

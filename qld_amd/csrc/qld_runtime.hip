@@ -139,6 +139,7 @@ extern "C" const char *qld_gpu_last_error(const qld_ctx *ctx)
 
 int qld_ctx_malloc(qld_ctx *ctx, size_t bytes, void **p)
 {
+    std::lock_guard<std::mutex> lk(ctx->alloc_mu);
     if (ctx->used + bytes > ctx->budget) {
         qld_set_error(ctx, "allocation of %zu bytes exceeds HBM budget "
                       "(%zu of %zu used)", bytes, ctx->used, ctx->budget);
@@ -157,6 +158,7 @@ int qld_ctx_malloc(qld_ctx *ctx, size_t bytes, void **p)
 void qld_ctx_mfree(qld_ctx *ctx, void *p, size_t bytes)
 {
     if (!p) return;
+    std::lock_guard<std::mutex> lk(ctx->alloc_mu);
     hipFree(p);
     ctx->used -= bytes;
 }
