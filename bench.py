@@ -208,7 +208,7 @@ def serving_bench(args):
     # warm-up
     ids = [srv.submit(tabs[0][0], 0, 1000, qld_amd.REQ_DISCARD) for _ in range(4)]
     for i in ids:
-        srv.wait(i)
+        srv.wait(i, 120000.0)
         srv.release(i)
     for rate in rates:
         gaps = rng.exponential(1.0 / rate, nreq)
@@ -227,7 +227,7 @@ def serving_bench(args):
                                   qld_amd.REQ_DISCARD))
         lat = []
         for i in ids:
-            lat.append(srv.wait(i))
+            lat.append(srv.wait(i, 120000.0))
             srv.release(i)
         span = time.perf_counter() - t0
         lat = np.array(lat)

@@ -103,6 +103,8 @@ struct qld_server {
     uint8_t *arena_mem = nullptr;
     Arena arena;
     bool stop = false;
+    uint64_t gen = 0;        // bumped by every submit and release: either may
+                             // let a blocked admission proceed
     int64_t next_id = 1;
     int64_t completed = 0, batches = 0, pairs = 0;
     int error = QLD_OK;
@@ -242,14 +244,28 @@ static void dispatcher(qld_server *s)
         }
         bool busy = false;
         for (Slot &sl : s->slots) busy |= sl.busy;
-        if (s->stop && !busy && s->pending.empty()) break;
+        if (s->stop && !busy) {
+            // Shutting down: requests that could not be admitted fail.
+            for (int64_t id : s->pending) {
+                Request &r = s->reqs[id];
+                r.state = FAILED;
+                r.t_done = now_s();
+            }
+            s->pending.clear();
+            s->cv_done.notify_all();
+            break;
+        }
         if (!progress) {
-            // Poll device completions every 20 us while work is in flight;
-            // sleep until a submit otherwise.
-            if (busy)
+            // Poll device completions every 20 us while work is in flight.
+            // Otherwise sleep until a submit or a release changes what can
+            // be admitted (waiting on `gen`, not on `pending`, which may be
+            // non-empty but blocked on memory).
+            if (busy) {
                 s->cv_work.wait_for(lk, std::chrono::microseconds(20));
-            else
-                s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+            } else {
+                const uint64_t g = s->gen;
+                s->cv_work.wait(lk, [&] { return s->stop || s->gen != g; });
+            }
         }
     }
 }
@@ -336,6 +352,7 @@ extern "C" int qld_server_submit(qld_server *s, const qld_table *t, int start, i
         const int64_t id = s->next_id++;
         s->reqs.emplace(id, r);
         s->pending.push_back(id);
+        s->gen++;
         *req_id = id;
     }
     s->cv_work.notify_one();
@@ -394,6 +411,7 @@ extern "C" int qld_server_release(qld_server *s, int64_t id)
         if (r.state != DONE && r.state != FAILED) return QLD_ERR_NOTREADY;
         if (r.out_off >= 0) s->arena.release(r.out_off, (int64_t)r.out_bytes);
         s->reqs.erase(it);
+        s->gen++;
     }
     s->cv_work.notify_one();    // freed memory may admit waiting requests
     return QLD_OK;

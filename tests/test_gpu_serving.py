@@ -35,6 +35,9 @@ def _want(A, bc, N, start, size):
                                oracle.counts_naive(sub), N)
 
 
+WAIT_MS = 60000.0     # every wait is bounded: a stuck server fails the test
+
+
 def _tri(x):
     n = x.shape[0]
     return x[np.triu(np.ones((n, n), dtype=bool), 1)]
@@ -57,7 +60,7 @@ def test_server_matches_oracle(tables):
     reqs = _requests(np.random.default_rng(0), tabs, 40, 2, 500)
     ids = [srv.submit(tabs[t][0], s, n) for t, s, n in reqs]
     for rid, (t, s, n) in zip(ids, reqs):
-        lat = srv.wait(rid)
+        lat = srv.wait(rid, WAIT_MS)
         assert lat > 0
         got = srv.fetch(rid, n)
         want = _want(tabs[t][1], tabs[t][2], tabs[t][3], s, n)
@@ -74,17 +77,17 @@ def test_alone_equals_in_mix(tables):
     probe = (123, 450)
     srv = qld_amd.Server(e, out_budget=256 << 20, max_batch=8, streams=2)
     rid = srv.submit(t, *probe)
-    srv.wait(rid)
+    srv.wait(rid, WAIT_MS)
     alone = srv.fetch(rid, probe[1])
     # the same request in the middle of a burst of others on both tables
     mix = _requests(np.random.default_rng(1), tabs, 30, 20, 600)
     ids = [srv.submit(tabs[ti][0], s, n, qld_amd.REQ_DISCARD) for ti, s, n in mix[:15]]
     rid = srv.submit(t, *probe)
     ids += [srv.submit(tabs[ti][0], s, n, qld_amd.REQ_DISCARD) for ti, s, n in mix[15:]]
-    srv.wait(rid)
+    srv.wait(rid, WAIT_MS)
     in_mix = srv.fetch(rid, probe[1])
     for i in ids:
-        srv.wait(i)
+        srv.wait(i, WAIT_MS)
         srv.release(i)
     assert (_tri(alone).view(np.uint32) == _tri(in_mix).view(np.uint32)).all()
     assert (_tri(alone).view(np.uint32) == _tri(_want(A, bc, N, *probe)).view(np.uint32)).all()
@@ -105,7 +108,7 @@ def test_budget_admission(tables):
     starts = list(range(0, 2400, 200))
     ids = [srv.submit(t, s, size) for s in starts]   # at most 2 in flight
     for rid, s in zip(ids, starts):
-        srv.wait(rid)
+        srv.wait(rid, WAIT_MS)
         got = srv.fetch(rid, size)
         assert (_tri(got).view(np.uint32) == _tri(_want(A, bc, N, s, size)).view(np.uint32)).all()
     srv.close()
@@ -119,7 +122,7 @@ def test_copy_mode_pinned(tables):
     assert qld_amd.hip_lib().qld_host_alloc(e._ctx, size * size * 4, C.byref(p)) == 0
     srv = qld_amd.Server(e, out_budget=64 << 20)
     rid = srv.submit(t, 50, size, qld_amd.REQ_COPY, p)
-    srv.wait(rid)
+    srv.wait(rid, WAIT_MS)
     got = np.ctypeslib.as_array((C.c_float * (size * size)).from_address(p.value)).reshape(size, size).copy()
     assert (_tri(got).view(np.uint32) == _tri(_want(A, bc, N, 50, size)).view(np.uint32)).all()
     srv.release(rid)
