@@ -268,6 +268,9 @@ def main():
     ap.add_argument("--no-windows", action="store_true")
     ap.add_argument("--no-rect", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run one device copy of this many MiB (a known-size "
+                         "kernel for calibrating HBM traffic counters)")
     ap.add_argument("--serving-rates", default="100,1000,5000")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--serving-rows16", type=int, default=100000)
@@ -339,6 +342,13 @@ def main():
         "device_ms_per_step": round(dev_ms / args.steps, 4),
         "roofline": roofline(engine, pairs, W, N, kernel_s),
     }
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        src, dst = eng.alloc(nb).fill(7), eng.alloc(nb)
+        eng._check(qld_amd.hip_lib().qld_dev_copy(eng._ctx, dst.ptr, src.ptr, nb))
+        eng.sync()
+        src.free()
+        dst.free()
     for b in (dA, dbc, dres):
         b.free()
     eng.close()

@@ -100,6 +100,10 @@ int qld_dev_free(qld_ctx *ctx, void *dptr);
 int qld_h2d(qld_ctx *ctx, void *dst, const void *src, size_t bytes);
 int qld_d2h(qld_ctx *ctx, void *dst, const void *src, size_t bytes);
 int qld_dev_fill(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32);
+/* Device-to-device copy by a plain streaming kernel on the context's stream
+ * (asynchronous); `bytes` must be a multiple of 16.  Also the known-size
+ * reference when calibrating HBM traffic counters. */
+int qld_dev_copy(qld_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Wait for all work queued on the context's stream. */
 int qld_sync(qld_ctx *ctx);
 

@@ -61,6 +61,7 @@ GPU_SYMBOLS = [
     ("qld_h2d", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     ("qld_d2h", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     ("qld_dev_fill", C.c_int, [_vp, _vp, C.c_uint32, C.c_size_t]),
+    ("qld_dev_copy", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     ("qld_sync", C.c_int, [_vp]),
     ("qld_correlate_dev", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int,
                                     C.c_int, C.c_int, C.c_int, _vp, C.c_int64,

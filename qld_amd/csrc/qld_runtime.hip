@@ -253,6 +253,27 @@ extern "C" int qld_dev_fill(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u
     return qld_fill_async(ctx, dptr, value, n_u32);
 }
 
+__global__ void qld_copy_u128(uint4 *dst, const uint4 *src, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int qld_dev_copy(qld_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes & 15) || (bytes && (!dst || !src))) return QLD_ERR_INVALID;
+    if (!bytes) return QLD_OK;
+    hipSetDevice(ctx->device);
+    size_t n = bytes / 16;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(qld_copy_u128, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       (uint4 *)dst, (const uint4 *)src, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_dev_copy");
+}
+
 extern "C" int qld_sync(qld_ctx *ctx)
 {
     if (!ctx) return QLD_ERR_INVALID;
