@@ -72,6 +72,7 @@ GPU_SYMBOLS = [
     ("qld_timer_stop", C.c_int, [_vp, _f32p]),
     ("qld_last_engine", C.c_int, [_vp]),
     ("qld_set_plain_tile", C.c_int, [_vp, C.c_int]),
+    ("qld_set_fp4_operands", C.c_int, [_vp, C.c_int]),
     ("qld_table_create", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     ("qld_table_create_dev", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(_vp)]),
@@ -350,6 +351,9 @@ class Engine:
     def plan_windows(self, table: "Table", windows: np.ndarray, out,
                      mode=BATCH_PERSISTENT) -> "Plan":
         return Plan(self, table, windows, out, mode)
+
+    def set_fp4_operands(self, packed: bool):
+        self._check(hip_lib().qld_set_fp4_operands(self._ctx, 1 if packed else 0))
 
     def set_plain_tile(self, tile: int):
         self._check(hip_lib().qld_set_plain_tile(self._ctx, int(tile)))

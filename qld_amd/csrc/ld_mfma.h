@@ -9,6 +9,8 @@
 
 #include "qld_internal.h"
 
+#include <vector>
+
 // One all-pairs job on expanded operands.  Row i of the job reads expanded
 // row XI + i*row_bytes and bit count bcI[i]; column j reads XJ / bcJ.  Cell
 // (i, j) goes to res[i*ldr + j] (and cnt[i*ldc + j]); tri != 0 keeps j > i.
@@ -29,6 +31,12 @@ struct WorkDesc {
     float pad_;
     double dN;
     double invN;
+    // Packed-operand launches (fp4 expanded inside the kernel): the packed
+    // rows themselves, `words` uint64 per row.
+    const uint64_t *PI;
+    const uint64_t *PJ;
+    int words;
+    int pad2_;
 };
 
 // One tile: rows [ti*T, ti*T + T) x columns [tj*T, tj*T + T) of job `desc`.
@@ -41,6 +49,9 @@ void fill_work_desc(WorkDesc &d, const uint8_t *XI, const uint8_t *XJ,
                     const uint32_t *bcI, const uint32_t *bcJ, int ni, int nj,
                     int64_t row_bytes, int tri, int snp_size, float *res,
                     int64_t ldr, uint32_t *cnt, int64_t ldc);
+
+// Packed-operand variant: the same job read straight from packed rows.
+void set_packed_operands(WorkDesc &d, const uint64_t *PI, const uint64_t *PJ, int words);
 
 // Packed words per row after padding to whole 128-byte k stages.
 int mfma_padded_words(int engine, int words);
@@ -57,6 +68,11 @@ void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int n
                       int tri, int T);
 size_t job_tile_count(int ni, int nj, int tri, int T);
 
+// Tiles of one job for a persistent launch, grouped per XCD queue:
+// items = queue 0's tiles, then queue 1's, ...; qbeg[0..8] = queue bounds.
+void build_xcd_schedule(std::vector<WorkItem> &items, int64_t *qbeg, uint32_t desc,
+                        int ni, int nj, int tri, int T);
+
 // Expand packed rows to fp4 nibbles / int8 bytes (rows x row_bytes).
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
                          uint8_t *X, hipStream_t s);
@@ -69,8 +85,12 @@ hipError_t mfma_init_attributes();
 //               for every item (single-job launches need no upload)
 //   persistent  0: one workgroup per item;  >0: that many workgroups pull
 //               items from eight per-XCD queues (queue = 16 device uint32,
-//               zeroed by this call on the stream before the kernel)
-hipError_t launch_mfma_items(int engine, int r2_mode, int tile,
+//               zeroed by this call on the stream before the kernel); queue
+//               x owns items [qbeg[x], qbeg[x+1]) (qbeg NULL: equal shares)
+//   packed      nonzero: fp4 engine reading PI/PJ packed rows (no expanded
+//               operand), 128-tiles only
+hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              const WorkDesc *descs, const WorkDesc &inline_desc,
                              const WorkItem *items, int64_t nitems,
-                             int persistent, uint32_t *queue, hipStream_t s);
+                             int persistent, uint32_t *queue, const int64_t *qbeg,
+                             hipStream_t s);

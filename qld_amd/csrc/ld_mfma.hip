@@ -27,6 +27,8 @@
 #include "ld_tiles.h"
 #include "ld_mfma.h"
 
+#include <vector>
+
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -108,6 +110,7 @@ struct KArgs {
     const WorkItem *items;
     int64_t nitems;
     uint32_t *queue;          // persistent launches: 8 per-XCD queue heads
+    int64_t qbeg[QLD_NUM_XCD + 1];   // queue x owns items [qbeg[x], qbeg[x+1])
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -245,18 +248,16 @@ struct Epi<QLD_R2_EXACT> {
     }
 };
 
-// One tile of one job: k loop over a ring of LDS stages, then the fused
-// epilogue straight to the job's result rows.
-template <class OP, int MODE, class G>
-__device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uint8_t *smem)
+// k loop over expanded operands: a ring of LDS stages filled by DMA.
+template <class OP, class G>
+__device__ __forceinline__ void kloop_dma(const WorkDesc &d, int i0, int j0,
+                                          typename OP::acc_t (&acc)[G::NB][G::NB],
+                                          uint8_t *smem)
 {
-    typedef Epi<MODE> E;
     constexpr int TILE = 2 * G::WT, SB = G::SB, NSTAGE = G::NSTAGE, KSTEPS = G::KSTEPS;
     constexpr int NB = G::NB, WT = G::WT;
     auto sA = [&](int b) { return smem + b * G::STAGE_BYTES; };
     auto sB = [&](int b) { return smem + b * G::STAGE_BYTES + G::TILE_BYTES; };
-
-    const int i0 = ti * TILE, j0 = tj * TILE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t rb = d.row_bytes;
@@ -265,29 +266,6 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uin
     // MFMA operand "A" = job rows i, operand "B" = job columns j.
     const __amdgpu_buffer_rsrc_t rsI = panel_rsrc<TILE>(d.XI, i0, d.ni, rb);
     const __amdgpu_buffer_rsrc_t rsJ = panel_rsrc<TILE>(d.XJ, j0, d.nj, rb);
-
-    // Bit counts of this lane's epilogue rows/columns, fetched now so their
-    // latency hides under the k loop.
-    constexpr int RPL = WT / 64 > 0 ? WT / 64 : 1;      // rows per lane (WT = 128 or 64)
-    uint32_t bc_row[RPL];
-#pragma unroll
-    for (int q = 0; q < RPL; ++q) {
-        int i = i0 + wm * WT + q * 64 + lane;
-        bc_row[q] = (lane + q * 64 < WT && i < d.ni) ? d.bcI[i] : 0u;
-    }
-    const int cq = lane % G::LANES_PER_ROW, rq = lane / G::LANES_PER_ROW;
-    const int jc = j0 + wn * WT + 4 * cq;               // first of this lane's 4 columns
-    uint32_t bc_col[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? d.bcJ[jc + q] : 0u;
-
-    typename OP::acc_t acc[NB][NB];
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
 
 #pragma unroll
     for (int p = 0; p < NSTAGE - 1; ++p)
@@ -332,14 +310,138 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uin
                 for (int b = 0; b < NB; ++b)
                     acc[a][b] = OP::mma(av[ks][a], bv[ks][b], acc[a][b]);
     }
-    lds_barrier();       // every wave is done with the ring before the epilogue reuses it
+}
 
-    // Fused epilogue.  Block (a, b) register r holds the pair
-    //   row wm*WT + a*32 + 8*(r/4) + 4*(lane/32) + r%4, col wn*WT + b*32 + lane%32
-    // of the tile.  The r^2 terms of the wave's WT rows go to LDS once; then
-    // for each a the wave stages those 32 rows of counts in its own LDS
-    // region and every lane turns 4 consecutive counts of a row into r^2,
-    // written with one 16-byte store (rows of WT floats coalesce).
+// e2m1 expansion of 32 packed samples into one 16-byte MFMA chunk: dword k
+// holds, in nibble n, 1.0 (0x2) iff bit 4n+k of x is set.  The sample -> K
+// mapping only has to agree between the two operands, which it does.
+__device__ __forceinline__ v4i fp4_chunk(uint32_t x)
+{
+    const uint32_t M = 0x22222222u;
+    v4i c;
+    c[0] = (int)((x << 1) & M);
+    c[1] = (int)(x & M);
+    c[2] = (int)((x >> 1) & M);
+    c[3] = (int)((x >> 2) & M);
+    return c;
+}
+
+// k loop over packed operands (fp4 only): lane t stages row t%TILE of
+// operand t/TILE.  Each 128-byte LDS stage row is the expansion of 4 packed
+// words (chunk c = packed dword c); packed words are fetched three stages
+// ahead into registers, expanded and written to the LDS stage the MFMAs
+// read next.  Memory traffic is 1 bit per sample instead of 4.
+template <class G>
+__device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
+                                             v16f (&acc)[G::NB][G::NB], uint8_t *smem)
+{
+    constexpr int TILE = 2 * G::WT, SB = G::SB, KSTEPS = G::KSTEPS;
+    constexpr int NB = G::NB, WT = G::WT;
+    static_assert(G::NSTAGE == 2 && SB == 128 && 2 * TILE == THREADS, "packed geometry");
+    auto sA = [&](int b) { return smem + b * G::STAGE_BYTES; };
+    auto sB = [&](int b) { return smem + b * G::STAGE_BYTES + G::TILE_BYTES; };
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int W = d.words;
+    const int stages = (W + 3) / 4;
+
+    // Staging role of this lane.
+    const int opnd = threadIdx.x / TILE, srow = threadIdx.x % TILE;
+    const int grow = (opnd ? j0 : i0) + srow;
+    const bool rvalid = grow < (opnd ? d.nj : d.ni);
+    const uint64_t *src = (opnd ? d.PJ : d.PI) + (int64_t)(rvalid ? grow : 0) * W;
+    uint8_t *dst_row = (opnd ? TILE * SB : 0) + srow * SB + smem;
+
+    auto fetch = [&](uint64_t (&r)[4], int s) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int k = 4 * s + w;
+            r[w] = (rvalid && k < W) ? src[k] : 0ull;
+        }
+    };
+    auto expand = [&](const uint64_t (&r)[4], int buf) {
+        uint8_t *row = dst_row + buf * G::STAGE_BYTES;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const uint32_t x = (uint32_t)(r[c >> 1] >> (32 * (c & 1)));
+            *(v4i *)(row + (G::swz(srow, c) << 4)) = fp4_chunk(x);
+        }
+    };
+
+    const int r32 = lane & 31, half = lane >> 5;
+    auto compute = [&](int buf) {
+        v4i av[KSTEPS][NB], bv[KSTEPS][NB];
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            const int c = 2 * ks + half;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                int row = wn * WT + b * 32 + r32;
+                bv[ks][b] = *(const v4i *)(sB(buf) + row * SB + (G::swz(row, c) << 4));
+            }
+#pragma unroll
+            for (int a = 0; a < NB; ++a) {
+                int row = wm * WT + a * 32 + r32;
+                av[ks][a] = *(const v4i *)(sA(buf) + row * SB + (G::swz(row, c) << 4));
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks)
+#pragma unroll
+            for (int a = 0; a < NB; ++a)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    acc[a][b] = FP4::mma(av[ks][a], bv[ks][b], acc[a][b]);
+    };
+
+    // Three register sets in rotation (unrolled so each set is static):
+    // stage s+1 is expanded while stage s+2 and s+3 are in flight.
+    uint64_t R0[4], R1[4], R2[4];
+    fetch(R0, 0);
+    fetch(R1, 1);
+    fetch(R2, 2);
+    expand(R0, 0);
+    fetch(R0, 3);
+    // One step: barrier, expand stage s+1 from Rn, refill Rn with stage s+4
+    // (Rn is free once expanded), compute stage s.
+    auto step = [&](int s, uint64_t (&Rn)[4]) {
+        lds_barrier();
+        if (s + 1 < stages) {
+            expand(Rn, (s + 1) & 1);
+            fetch(Rn, s + 4);
+        }
+        compute(s & 1);
+    };
+    int s = 0;
+    for (; s + 3 <= stages; s += 3) {
+        step(s, R1);
+        step(s + 1, R2);
+        step(s + 2, R0);
+    }
+    if (s < stages) step(s, R1);
+    if (s + 1 < stages) step(s + 1, R2);
+}
+
+// Fused epilogue.  Block (a, b) register r holds the pair
+//   row wm*WT + a*32 + 8*(r/4) + 4*(lane/32) + r%4, col wn*WT + b*32 + lane%32
+// of the tile.  The r^2 terms of the wave's WT rows go to LDS once; then
+// for each a the wave stages those 32 rows of counts in its own LDS region
+// and every lane turns 4 consecutive counts of a row into r^2, written with
+// one 16-byte store (rows of WT floats coalesce).
+template <class OP, int MODE, class G>
+__device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
+                                              typename OP::acc_t (&acc)[G::NB][G::NB],
+                                              const uint32_t *bc_row, const uint32_t *bc_col,
+                                              uint8_t *smem)
+{
+    typedef Epi<MODE> E;
+    constexpr int NB = G::NB, WT = G::WT;
+    constexpr int RPL = WT / 64 > 0 ? WT / 64 : 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int r32 = lane & 31, half = lane >> 5;
+    const int cq = lane % G::LANES_PER_ROW, rq = lane / G::LANES_PER_ROW;
+    const int jc = j0 + wn * WT + 4 * cq;               // first of this lane's 4 columns
     uint32_t *ep = (uint32_t *)(smem + wave * G::EP_WAVE_BYTES);
     uint4 *terms = (uint4 *)(smem + G::EP_TERMS_OFF) + wave * WT;
 #pragma unroll
@@ -392,37 +494,75 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uin
     }
 }
 
-template <class OP, int MODE, class G>
+// One tile of one job: k loop (expanded operands by DMA, or packed operands
+// expanded in the kernel), then the fused epilogue.
+template <class OP, int MODE, class G, int PACKED>
+__device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uint8_t *smem)
+{
+    constexpr int NB = G::NB, WT = G::WT, TILE = 2 * G::WT;
+    constexpr int RPL = WT / 64 > 0 ? WT / 64 : 1;      // epilogue rows per lane
+    const int i0 = ti * TILE, j0 = tj * TILE;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    // Bit counts of this lane's epilogue rows/columns, fetched now so their
+    // latency hides under the k loop.
+    uint32_t bc_row[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        int i = i0 + wm * WT + q * 64 + lane;
+        bc_row[q] = (lane + q * 64 < WT && i < d.ni) ? d.bcI[i] : 0u;
+    }
+    const int jc = j0 + wn * WT + 4 * (lane % G::LANES_PER_ROW);
+    uint32_t bc_col[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? d.bcJ[jc + q] : 0u;
+
+    typename OP::acc_t acc[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+
+    if constexpr (PACKED) kloop_packed<G>(d, i0, j0, acc, smem);
+    else kloop_dma<OP, G>(d, i0, j0, acc, smem);
+    lds_barrier();       // every wave is done with the ring before the epilogue reuses it
+    tile_epilogue<OP, MODE, G>(d, i0, j0, acc, bc_row, bc_col, smem);
+}
+
+template <class OP, int MODE, class G, int PACKED>
 __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, uint8_t *smem)
 {
     const WorkItem it = p.items[idx];
     if (p.descs) {
         const WorkDesc d = p.descs[it.desc];
-        mfma_tile<OP, MODE, G>(d, it.ti, it.tj, smem);
+        mfma_tile<OP, MODE, G, PACKED>(d, it.ti, it.tj, smem);
     } else {
-        mfma_tile<OP, MODE, G>(p.inline_desc, it.ti, it.tj, smem);
+        mfma_tile<OP, MODE, G, PACKED>(p.inline_desc, it.ti, it.tj, smem);
     }
 }
 
-// One workgroup per item (PERSIST == 0) or a fixed grid draining eight
-// per-XCD queues: workgroup b starts on queue b % 8 -- the XCD it runs on --
-// whose items are a contiguous range of the list, then steals from the
-// other queues.  Every claim advances a head, so the loop ends once all
-// eight ranges are exhausted.
-template <class OP, int MODE, class G, int PERSIST>
+// One workgroup per item (PERSIST == 0; workgroup b takes item b) or a
+// fixed grid draining eight per-XCD queues: workgroup b starts on queue
+// b % 8 -- the XCD it runs on -- whose items are a contiguous range of the
+// list, then steals from the other queues.  Every claim advances a head, so
+// the loop ends once all eight ranges are exhausted.
+template <class OP, int MODE, class G, int PERSIST, int PACKED>
 __global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(KArgs p)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     if (!PERSIST) {
-        run_item<OP, MODE, G>(p, xcd_logical(blockIdx.x, gridDim.x), smem);
+        run_item<OP, MODE, G, PACKED>(p, (int64_t)blockIdx.x, smem);
         return;
     }
     __shared__ int64_t claimed;
     const int x0 = (int)(blockIdx.x % QLD_NUM_XCD);
     for (int q = 0; q < QLD_NUM_XCD; ++q) {
         const int x = (x0 + q) % QLD_NUM_XCD;
-        const int64_t begin = p.nitems * x / QLD_NUM_XCD;
-        const int64_t end = p.nitems * (x + 1) / QLD_NUM_XCD;
+        const int64_t begin = p.qbeg[x];
+        const int64_t end = p.qbeg[x + 1];
         for (;;) {
             if (threadIdx.x == 0) {
                 int64_t k = (int64_t)atomicAdd(&p.queue[x], 1u);
@@ -432,7 +572,7 @@ __global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(KArgs p)
             const int64_t idx = claimed;
             __syncthreads();
             if (idx < 0) break;
-            run_item<OP, MODE, G>(p, idx, smem);
+            run_item<OP, MODE, G, PACKED>(p, idx, smem);
         }
     }
 }
@@ -528,6 +668,17 @@ void fill_work_desc(WorkDesc &d, const uint8_t *XI, const uint8_t *XJ,
     d.pad_ = 0.0f;
     d.dN = e.dN;
     d.invN = e.invN;
+    d.PI = nullptr;
+    d.PJ = nullptr;
+    d.words = 0;
+    d.pad2_ = 0;
+}
+
+void set_packed_operands(WorkDesc &d, const uint64_t *PI, const uint64_t *PJ, int words)
+{
+    d.PI = PI;
+    d.PJ = PJ;
+    d.words = words;
 }
 
 size_t job_tile_count(int ni, int nj, int tri, int T)
@@ -553,6 +704,37 @@ void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int n
     *count = k;
 }
 
+void build_xcd_schedule(std::vector<WorkItem> &items, int64_t *qbeg, uint32_t desc,
+                        int ni, int nj, int tri, int T)
+{
+    // The persistent grid keeps 2 workgroups per CU = 64 per XCD busy.  The
+    // tiles are cut into super-blocks of 16 x 32 tiles; inside one, XCD x
+    // owns the 8 x 8 sub-block (x / 4, x % 4).  Every queue walks the
+    // super-blocks in the same order, so at any time all XCDs share one
+    // super-block's 48 panels in the Infinity Cache while each XCD reuses
+    // its own 16 panels in its L2.
+    const int SI = 16, SJ = 32, SUB = 8;
+    const int tic = (ni + T - 1) / T, tjc = (nj + T - 1) / T;
+    std::vector<std::vector<WorkItem>> q(QLD_NUM_XCD);
+    for (int J = 0; J < (tjc + SJ - 1) / SJ; ++J)
+        for (int I = 0; I < (tic + SI - 1) / SI; ++I) {
+            if (tri && I * SI > (J + 1) * SJ - 1) continue;
+            for (int x = 0; x < QLD_NUM_XCD; ++x) {
+                const int bi = I * SI + (x / 4) * SUB, bj = J * SJ + (x % 4) * SUB;
+                for (int tj = bj; tj < bj + SUB && tj < tjc; ++tj)
+                    for (int ti = bi; ti < bi + SUB && ti < tic; ++ti)
+                        if (!tri || ti <= tj)
+                            q[x].push_back(WorkItem{desc, (uint16_t)ti, (uint16_t)tj});
+            }
+        }
+    items.clear();
+    for (int x = 0; x < QLD_NUM_XCD; ++x) {
+        qbeg[x] = (int64_t)items.size();
+        items.insert(items.end(), q[x].begin(), q[x].end());
+    }
+    qbeg[QLD_NUM_XCD] = (int64_t)items.size();
+}
+
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
                          uint8_t *X, hipStream_t s)
 {
@@ -575,10 +757,10 @@ __global__ void zero_queue_kernel(uint32_t *q)
     if (threadIdx.x < 16) q[threadIdx.x] = 0u;
 }
 
-template <class OP, int MODE, class G, int PERSIST>
+template <class OP, int MODE, class G, int PERSIST, int PACKED = 0>
 static hipError_t set_lds_limit()
 {
-    return hipFuncSetAttribute((const void *)ld_mfma_kernel<OP, MODE, G, PERSIST>,
+    return hipFuncSetAttribute((const void *)ld_mfma_kernel<OP, MODE, G, PERSIST, PACKED>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)G::LDS_BYTES);
 }
@@ -596,6 +778,14 @@ static hipError_t set_lds_limits_mode()
     return set_lds_limit<OP, MODE, GeoS, 1>();
 }
 
+template <int MODE>
+static hipError_t set_lds_limits_packed()
+{
+    hipError_t e;
+    if ((e = set_lds_limit<FP4, MODE, GeoS, 0, 1>()) != hipSuccess) return e;
+    return set_lds_limit<FP4, MODE, GeoS, 1, 1>();
+}
+
 // Raise the dynamic LDS limit of every kernel variant (done once per
 // process, before any launch -- and so before any graph capture).
 hipError_t mfma_init_attributes()
@@ -604,23 +794,25 @@ hipError_t mfma_init_attributes()
     if ((e = set_lds_limits_mode<FP4, QLD_R2_REFERENCE>()) != hipSuccess) return e;
     if ((e = set_lds_limits_mode<FP4, QLD_R2_EXACT>()) != hipSuccess) return e;
     if ((e = set_lds_limits_mode<I8, QLD_R2_REFERENCE>()) != hipSuccess) return e;
-    return set_lds_limits_mode<I8, QLD_R2_EXACT>();
+    if ((e = set_lds_limits_mode<I8, QLD_R2_EXACT>()) != hipSuccess) return e;
+    if ((e = set_lds_limits_packed<QLD_R2_REFERENCE>()) != hipSuccess) return e;
+    return set_lds_limits_packed<QLD_R2_EXACT>();
 }
 
-template <class OP, int MODE, class G, int PERSIST>
+template <class OP, int MODE, class G, int PERSIST, int PACKED = 0>
 static hipError_t launch_one(const KArgs &a, unsigned grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE, G, PERSIST>), dim3(grid), dim3(THREADS),
-                       (size_t)G::LDS_BYTES, s, a);
+    hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE, G, PERSIST, PACKED>), dim3(grid),
+                       dim3(THREADS), (size_t)G::LDS_BYTES, s, a);
     return hipGetLastError();
 }
 
-template <class OP, int MODE, class G>
+template <class OP, int MODE, class G, int PACKED = 0>
 static hipError_t launch_mode(const KArgs &a, int persistent, hipStream_t s)
 {
     if (persistent > 0)
-        return launch_one<OP, MODE, G, 1>(a, (unsigned)persistent, s);
-    return launch_one<OP, MODE, G, 0>(a, (unsigned)a.nitems, s);
+        return launch_one<OP, MODE, G, 1, PACKED>(a, (unsigned)persistent, s);
+    return launch_one<OP, MODE, G, 0, PACKED>(a, (unsigned)a.nitems, s);
 }
 
 template <class OP, int MODE>
@@ -630,10 +822,11 @@ static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStrea
     return launch_mode<OP, MODE, GeoL>(a, persistent, s);
 }
 
-hipError_t launch_mfma_items(int engine, int r2_mode, int tile,
+hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              const WorkDesc *descs, const WorkDesc &inline_desc,
                              const WorkItem *items, int64_t nitems,
-                             int persistent, uint32_t *queue, hipStream_t s)
+                             int persistent, uint32_t *queue, const int64_t *qbeg,
+                             hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
     KArgs a;
@@ -642,11 +835,19 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile,
     a.items = items;
     a.nitems = nitems;
     a.queue = queue;
+    for (int x = 0; x <= QLD_NUM_XCD; ++x)
+        a.qbeg[x] = qbeg ? qbeg[x] : nitems * x / QLD_NUM_XCD;
     if (persistent > 0) {
         if (!queue) return hipErrorInvalidValue;
         hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, queue);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (packed) {
+        if (engine != QLD_ENGINE_MFMA_FP4 || tile != QLD_TILE_SMALL) return hipErrorInvalidValue;
+        return r2_mode == QLD_R2_EXACT
+                   ? launch_mode<FP4, QLD_R2_EXACT, GeoS, 1>(a, persistent, s)
+                   : launch_mode<FP4, QLD_R2_REFERENCE, GeoS, 1>(a, persistent, s);
     }
     if (engine == QLD_ENGINE_MFMA_FP4)
         return r2_mode == QLD_R2_EXACT ? launch_tile<FP4, QLD_R2_EXACT>(a, tile, persistent, s)

@@ -186,14 +186,14 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
     if (p->mode == QLD_BATCH_PERSISTENT) {
         for (auto &c : p->classes) {
             if (!c.nitems) continue;
-            e = launch_mfma_items(p->engine, p->r2_mode, c.tile, c.d_descs, none, c.d_items,
-                                  (int64_t)c.nitems, c.grid, c.d_queue, ctx->stream);
+            e = launch_mfma_items(p->engine, p->r2_mode, c.tile, 0, c.d_descs, none, c.d_items,
+                                  (int64_t)c.nitems, c.grid, c.d_queue, nullptr, ctx->stream);
             if (e != hipSuccess) return e;
         }
     } else {
         for (auto &l : p->plain) {
-            e = launch_mfma_items(p->engine, p->r2_mode, l.tile, nullptr, l.desc,
-                                  p->d_items_all + l.item_off, l.item_cnt, 0, nullptr,
+            e = launch_mfma_items(p->engine, p->r2_mode, l.tile, 0, nullptr, l.desc,
+                                  p->d_items_all + l.item_off, l.item_cnt, 0, nullptr, nullptr,
                                   ctx->stream);
             if (e != hipSuccess) return e;
         }

@@ -36,10 +36,12 @@ struct qld_ctx {
     DevBuf host_stage;   // staging for the host-buffer entry point
     DevBuf ws;           // engine workspace (expanded operands)
     DevBuf tiles;        // device tile list of the last plain MFMA shape
+    DevBuf queue;        // per-XCD queue heads of persistent plain launches
     std::vector<WorkItem> tiles_host;
     int tiles_key[4] = {-1, -1, -1, -1};         // (ni, nj, tri, tile edge)
     int tile_cutoff = 1024;   // windows of at least this many SNPs use 256-tiles
     int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape
+    int fp4_packed = 0;       // fp4 engine reads packed rows (1) or an expanded copy (0)
     char err[512] = {0};
 };
 

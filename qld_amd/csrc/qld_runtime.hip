@@ -119,6 +119,7 @@ extern "C" void qld_gpu_release(qld_ctx *ctx)
     ctx->allocs.clear();
     ctx->ws.release();
     ctx->tiles.release();
+    ctx->queue.release();
     ctx->host_stage.release();
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
@@ -337,51 +338,74 @@ static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job)
     return QLD_TILE_SMALL;
 }
 
+static int upload_plain_tiles(qld_ctx *ctx, const LdJob &job, int T)
+{
+    if (ctx->tiles_key[0] == job.n && ctx->tiles_key[1] == job.m &&
+        ctx->tiles_key[2] == job.tri && ctx->tiles_key[3] == T)
+        return QLD_OK;
+    size_t cnt = 0;
+    ctx->tiles_host.resize(job_tile_count(job.n, job.m, job.tri, T));
+    append_job_tiles(ctx->tiles_host.data(), &cnt, 0, job.n, job.m, job.tri, T);
+    size_t tb = cnt * sizeof(WorkItem);
+    ctx->tiles_key[0] = -1;
+    int rc = ctx->tiles.reserve(ctx, tb ? tb : 8);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(ctx->tiles.ptr, ctx->tiles_host.data(), tb,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return qld_hip_fail(ctx, e, "tile list upload");
+    ctx->tiles_key[0] = job.n;
+    ctx->tiles_key[1] = job.m;
+    ctx->tiles_key[2] = job.tri;
+    ctx->tiles_key[3] = T;
+    return QLD_OK;
+}
+
 static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
 {
-    const int T = plain_tile_edge(ctx, job);
+    // The fp4 engine reads packed rows directly (expanding in the kernel,
+    // 128-tiles) unless the context asks for an expanded operand copy.
+    const bool packed = engine == QLD_ENGINE_MFMA_FP4 && ctx->fp4_packed;
+    const int T = packed ? QLD_TILE_SMALL : plain_tile_edge(ctx, job);
     if ((job.n + T - 1) / T > 65535 || (job.m + T - 1) / T > 65535) {
         qld_set_error(ctx, "too many rows for one launch (m=%d n=%d)", job.m, job.n);
         return QLD_ERR_INVALID;
     }
-    const int64_t rb = mfma_row_bytes(engine, job.words);
-    const bool same = job.A == job.B && job.m == job.n;
-    const size_t bytesA = align256((size_t)job.m * rb);
-    const size_t bytesB = same ? 0 : align256((size_t)job.n * rb);
-    int rc = ctx->ws.reserve(ctx, bytesA + bytesB);
+    int rc = upload_plain_tiles(ctx, job, T);
     if (rc) return rc;
-    uint8_t *XA = (uint8_t *)ctx->ws.ptr;
-    uint8_t *XB = same ? XA : XA + bytesA;
-
-    if (ctx->tiles_key[0] != job.n || ctx->tiles_key[1] != job.m ||
-        ctx->tiles_key[2] != job.tri || ctx->tiles_key[3] != T) {
-        size_t cnt = 0;
-        ctx->tiles_host.resize(job_tile_count(job.n, job.m, job.tri, T));
-        append_job_tiles(ctx->tiles_host.data(), &cnt, 0, job.n, job.m, job.tri, T);
-        size_t tb = cnt * sizeof(WorkItem);
-        ctx->tiles_key[0] = -1;
-        if ((rc = ctx->tiles.reserve(ctx, tb ? tb : 8))) return rc;
-        hipError_t e = hipMemcpyAsync(ctx->tiles.ptr, ctx->tiles_host.data(), tb,
-                                      hipMemcpyHostToDevice, ctx->stream);
-        if (e != hipSuccess) return qld_hip_fail(ctx, e, "tile list upload");
-        ctx->tiles_key[0] = job.n;
-        ctx->tiles_key[1] = job.m;
-        ctx->tiles_key[2] = job.tri;
-        ctx->tiles_key[3] = T;
-    }
-
-    hipError_t e = launch_expand(engine, job.A, job.m, job.words, XA, ctx->stream);
-    if (e == hipSuccess && !same)
-        e = launch_expand(engine, job.B, job.n, job.words, XB, ctx->stream);
-    if (e != hipSuccess) return qld_hip_fail(ctx, e, "expand kernel");
-    // Job rows i come from table B, columns j from table A.
+    hipError_t e = hipSuccess;
     WorkDesc d;
-    fill_work_desc(d, XB, XA, job.bcB, job.bcA, job.n, job.m, rb, job.tri, job.snp_size,
-                   job.res, job.ldr, job.cnt, job.ldc);
+    if (packed) {
+        // Job rows i come from table B, columns j from table A.
+        fill_work_desc(d, nullptr, nullptr, job.bcB, job.bcA, job.n, job.m, 0, job.tri,
+                       job.snp_size, job.res, job.ldr, job.cnt, job.ldc);
+        set_packed_operands(d, job.B, job.A, job.words);
+    } else {
+        const int64_t rb = mfma_row_bytes(engine, job.words);
+        const bool same = job.A == job.B && job.m == job.n;
+        const size_t bytesA = align256((size_t)job.m * rb);
+        const size_t bytesB = same ? 0 : align256((size_t)job.n * rb);
+        if ((rc = ctx->ws.reserve(ctx, bytesA + bytesB))) return rc;
+        uint8_t *XA = (uint8_t *)ctx->ws.ptr;
+        uint8_t *XB = same ? XA : XA + bytesA;
+        e = launch_expand(engine, job.A, job.m, job.words, XA, ctx->stream);
+        if (e == hipSuccess && !same)
+            e = launch_expand(engine, job.B, job.n, job.words, XB, ctx->stream);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "expand kernel");
+        fill_work_desc(d, XB, XA, job.bcB, job.bcA, job.n, job.m, rb, job.tri, job.snp_size,
+                       job.res, job.ldr, job.cnt, job.ldc);
+    }
+    // Persistent launch: two workgroups per CU (128-tiles) or one (256)
+    // drain per-XCD queues over equal shares of the tile list; measured
+    // 3.47 ms vs 4.11 ms for one workgroup per tile on the 30,000-SNP
+    // triangle.
+    if ((rc = ctx->queue.reserve(ctx, 16 * sizeof(uint32_t)))) return rc;
+    const int64_t ntiles = (int64_t)ctx->tiles_host.size();
+    int64_t grid = (int64_t)ctx->num_cu * (T == QLD_TILE_SMALL ? 2 : 1);
+    if (grid > ntiles) grid = ntiles;
     if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
-    e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, nullptr, d,
-                          (const WorkItem *)ctx->tiles.ptr,
-                          (int64_t)ctx->tiles_host.size(), 0, nullptr, ctx->stream);
+    e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, packed ? 1 : 0, nullptr, d,
+                          (const WorkItem *)ctx->tiles.ptr, ntiles, (int)grid,
+                          (uint32_t *)ctx->queue.ptr, nullptr, ctx->stream);
     if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "MFMA kernel launch");
 }
@@ -533,6 +557,13 @@ extern "C" int qld_timer_stop(qld_ctx *ctx, float *ms)
     if (e == hipSuccess) e = hipEventSynchronize(ctx->tev1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->tev0, ctx->tev1);
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "qld_timer_stop");
+}
+
+extern "C" int qld_set_fp4_operands(qld_ctx *ctx, int mode)
+{
+    if (!ctx || (mode != QLD_FP4_PACKED && mode != QLD_FP4_EXPANDED)) return QLD_ERR_INVALID;
+    ctx->fp4_packed = mode == QLD_FP4_PACKED;
+    return QLD_OK;
 }
 
 extern "C" int qld_set_plain_tile(qld_ctx *ctx, int tile)

@@ -169,9 +169,9 @@ static int launch_batch(qld_server *s, Slot &sl)
     WorkDesc none;
     memset(&none, 0, sizeof(none));
     if (e == hipSuccess)
-        e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, (const WorkDesc *)sl.d_stage, none,
+        e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, 0, (const WorkDesc *)sl.d_stage, none,
                               (const WorkItem *)((char *)sl.d_stage + desc_bytes),
-                              (int64_t)cnt, 0, nullptr, sl.stream);
+                              (int64_t)cnt, 0, nullptr, nullptr, sl.stream);
     for (int64_t id : sl.reqs) {
         Request &r = s->reqs[id];
         if (e == hipSuccess && (r.flags & QLD_REQ_COPY) && r.host_out)

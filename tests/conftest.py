@@ -51,7 +51,14 @@ def engine_factory():
     made = []
 
     def make(**kw):
+        # "mfma_fp4p" = the fp4 engine reading packed rows (in-kernel
+        # expansion) instead of the default expanded operand copy.
+        packed = kw.get("engine") == "mfma_fp4p"
+        if packed:
+            kw["engine"] = "mfma_fp4"
         e = qld_amd.Engine(**kw)
+        if packed:
+            e.set_fp4_operands(True)
         made.append(e)
         return e
     yield make
