@@ -44,7 +44,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def roofline(engine, pairs, words, samples, kernel_s):
+def measured_traffic(engine, r2_mode, snps, samples):
+    """HBM bytes per launch of the contraction kernel for this workload, read
+    from the committed rocprofv3 PMC summary (FETCH_SIZE x2 + WRITE_SIZE x1,
+    scaled by a copy kernel of known size); None when no pass covers it."""
+    key = f"engine={engine} mode={r2_mode} snps={snps} samples={samples} "
+    try:
+        for path in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
+            with open(os.path.join(ROOT, "profiles", path)) as f:
+                for line in f:
+                    if line.startswith("traffic_bytes ") and key in line + " ":
+                        kv = dict(t.split("=", 1) for t in line.split()[1:])
+                        return {"bytes": int(kv["fetch"]) + int(kv["write"]),
+                                "fetch": int(kv["fetch"]), "write": int(kv["write"]),
+                                "source": f"profiles/{path}"}
+    except OSError:
+        pass
+    return None
+
+
+def roofline(engine, pairs, words, samples, kernel_s, traffic=None, rows=None):
     if engine == "popcount":
         ops = pairs * 4.0 * words          # 2x v_and_b32 + 2x v_bcnt per word pair
         ach = ops / kernel_s
@@ -55,9 +74,18 @@ def roofline(engine, pairs, words, samples, kernel_s):
     flops = pairs * 2.0 * samples
     ach = flops / kernel_s
     pk = PEAK[engine]
-    return {"bound": "mfma", "achieved": ach / 1e12, "peak": pk / 1e12,
-            "unit": "TFLOP/s", "frac": ach / pk, "traffic": None,
-            "flops_per_pair": 2 * samples, "kernel_ms": kernel_s * 1e3}
+    out = {"bound": "mfma", "achieved": ach / 1e12, "peak": pk / 1e12,
+           "unit": "TFLOP/s", "frac": ach / pk,
+           "traffic": traffic["bytes"] if traffic else None,
+           "flops_per_pair": 2 * samples, "kernel_ms": kernel_s * 1e3}
+    if traffic:
+        out["traffic_detail"] = traffic
+        if rows:
+            # each operand row read once (fp4: 4 bits per sample, padded to the
+            # 64-sample word) + 4 B of r2 written per pair
+            opb = {"mfma_fp4": 32, "mfma_i8": 64}.get(engine, 8)
+            out["algorithmic_bytes"] = rows * words * opb + pairs * 4
+    return out
 
 
 def cpu_baseline(A, bc, N, budget_s=10.0):
@@ -361,7 +389,8 @@ def main():
         "device_ms_per_step": round(dev_ms / args.steps, 4),
         f"{other}_mode": {"ms_per_step": round(other_ms, 4),
                           "gpairs_s": round(pairs / other_ms / 1e6, 3)},
-        "roofline": roofline(engine, pairs, W, N, kernel_s),
+        "roofline": roofline(engine, pairs, W, N, kernel_s,
+                             measured_traffic(engine, args.r2_mode, m, N), rows=m),
     }
     if args.copy_probe:
         nb = args.copy_probe << 20
