@@ -85,6 +85,66 @@ void qld_synth_regions(uint64_t seed, int m_a, int n_b, int n_samples,
 void qld_bitcounts(const uint64_t *table, int rows, int words, uint32_t *bc,
                    int nthreads);
 
+/* ---- MDF input directories -------------------------------------------- */
+
+#define QLD_PATH_MAX 4096
+
+/* Message of the last failed host-library call on this thread. */
+const char *qld_host_last_error(void);
+
+/* What readHeaderFile (correlate_IO.c:79-255) takes from a directory: the
+ * first regular file whose name contains "header"; its name up to the first
+ * '_' is the alignment id; the line after its `#CHROM` line holds
+ * `id snipsPerFile snipSize totalSnips posMin posMax` (written by
+ * parse_vcf.c:620-634); `samples` counts the #CHROM line's sample columns. */
+typedef struct qld_mdf_header {
+    char id[256];
+    char header_path[QLD_PATH_MAX];
+    int snips_per_file, snip_size, total_snips, pos_min, pos_max;
+    int samples;
+} qld_mdf_header;
+
+int qld_mdf_read_header(const char *dir, qld_mdf_header *h);
+
+/* Chunk files of `id` covering the window [pos_wmin, pos_wmax], in chunk
+ * order, as findFiles (correlate_IO.c:294-376) selects them from names
+ * `<id>_<first>_<last>_<posMin>_<posMax>.mdf`.  *files is malloc'd; release
+ * with qld_mdf_free_files.  Where the reference would rescan forever (no
+ * chunk continues the index sequence) this returns QLD_ERR_NOTFOUND. */
+int qld_mdf_find_files(const char *dir, const char *id, int snips_per_file,
+                       int total_snips, int pos_wmin, int pos_wmax,
+                       char ***files, int *nfiles);
+void qld_mdf_free_files(char **files, int nfiles);
+
+/* A loaded table: one SNP per row, `words` uint64 per row, with the POS, ID
+ * and BC columns of the MDF rows (table_x64, correlate_MEM.h:66-75). */
+typedef struct qld_snp_table {
+    int rows, words;
+    uint64_t *snp;
+    uint32_t *pos, *bc;
+    char **id;
+    char *idbuf;
+} qld_snp_table;
+
+/* Load the rows of `files` as readTable_x64 does for MDF input
+ * (correlate_IOx64.c:27-122,351-461): per file, two header lines are
+ * skipped, then lines are read while pos < pos_wmax (only the first line
+ * when pos_wset == 0) and every line whose POS lies in [pos_wmin, pos_wmax]
+ * becomes a row; the `#CHROM` line itself is such a line with POS 0.  Each
+ * row must carry exactly `words` packed words (QLD_ERR_INVALID otherwise).
+ * Files are read and rows parsed on `nthreads` threads. */
+int qld_mdf_read_table(char *const *files, int nfiles, int pos_wmin, int pos_wmax,
+                       int pos_wset, int words, int nthreads, qld_snp_table *t);
+void qld_snp_table_free(qld_snp_table *t);
+
+/* Write one MDF chunk file exactly as the MDF tool does
+ * (process_data.c:447-460): `##fileformat=MDFv0.1`, `##SNPs:<rows>`, the
+ * `#CHROM POS ID BC compSNP_k...` line, then one tab-separated row per SNP,
+ * no trailing newline.  id may be NULL ("."). */
+int qld_mdf_write(const char *path, const char *chrom, const uint64_t *snp,
+                  const uint32_t *pos, char *const *id, const uint32_t *bc,
+                  int rows, int words);
+
 #ifdef __cplusplus
 }
 #endif

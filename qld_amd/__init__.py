@@ -133,7 +133,33 @@ HOST_SYMBOLS = [
                                  C.c_double, C.c_double, _vp, _vp, _vp, _vp,
                                  C.c_int]),
     ("qld_bitcounts", None, [_vp, C.c_int, C.c_int, _vp, C.c_int]),
+    ("qld_host_last_error", C.c_char_p, []),
+    ("qld_mdf_read_header", C.c_int, [C.c_char_p, _vp]),
+    ("qld_mdf_find_files", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _vp, _vp]),
+    ("qld_mdf_free_files", None, [_vp, C.c_int]),
+    ("qld_mdf_read_table", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _vp]),
+    ("qld_snp_table_free", None, [_vp]),
+    ("qld_mdf_write", C.c_int, [C.c_char_p, C.c_char_p, _vp, _vp, _vp, _vp, C.c_int,
+                                C.c_int]),
 ]
+
+PATH_MAX = 4096
+
+
+class MdfHeader(C.Structure):
+    """qld_mdf_header (include/qld_host.h)."""
+    _fields_ = [("id", C.c_char * 256), ("header_path", C.c_char * PATH_MAX),
+                ("snips_per_file", C.c_int), ("snip_size", C.c_int),
+                ("total_snips", C.c_int), ("pos_min", C.c_int), ("pos_max", C.c_int),
+                ("samples", C.c_int)]
+
+
+class _SnpTable(C.Structure):
+    _fields_ = [("rows", C.c_int), ("words", C.c_int), ("snp", C.c_void_p),
+                ("pos", C.c_void_p), ("bc", C.c_void_p), ("id", C.c_void_p),
+                ("idbuf", C.c_void_p)]
 
 _hip = None
 _host = None
@@ -206,6 +232,69 @@ def synth_regions(seed: int, m_a: int, n_b: int, n_samples: int,
                                  maf_hi, copy_prob, copy_frac, flip_prob,
                                  _ptr(ta), _ptr(ba), _ptr(tb), _ptr(bb), nthreads)
     return ta, ba, tb, bb
+
+
+# ---- MDF directories (host C reader/writer) --------------------------------
+
+def _host_check(rc: int, what: str) -> None:
+    if rc != QLD_OK:
+        msg = host_lib().qld_host_last_error() or b""
+        raise QldError(rc, f"{what}: {msg.decode(errors='replace')}")
+
+
+def mdf_read_header(directory: str) -> MdfHeader:
+    h = MdfHeader()
+    _host_check(host_lib().qld_mdf_read_header(directory.encode(), C.byref(h)),
+                f"header of {directory}")
+    return h
+
+
+def mdf_find_files(directory: str, hid: str, snips_per_file: int, total_snips: int,
+                   pos_wmin: int, pos_wmax: int) -> list[str]:
+    files = C.POINTER(C.c_char_p)()
+    n = C.c_int()
+    _host_check(host_lib().qld_mdf_find_files(directory.encode(), hid.encode(),
+                                              snips_per_file, total_snips, pos_wmin,
+                                              pos_wmax, C.byref(files), C.byref(n)),
+                f"chunks of {directory}")
+    out = [files[i].decode() for i in range(n.value)]
+    host_lib().qld_mdf_free_files(C.cast(files, C.c_void_p), n.value)
+    return out
+
+
+def mdf_read_table(files: list[str], pos_wmin: int, pos_wmax: int, words: int,
+                   pos_wset: int = 1, nthreads: int = 0) -> dict:
+    """Rows of `files` in [pos_wmin, pos_wmax] -> dict(snp, pos, bc, id)."""
+    arr = (C.c_char_p * max(1, len(files)))(*[f.encode() for f in files])
+    t = _SnpTable()
+    _host_check(host_lib().qld_mdf_read_table(C.cast(arr, C.c_void_p), len(files), pos_wmin,
+                                              pos_wmax, pos_wset, words,
+                                              nthreads or (os.cpu_count() or 1),
+                                              C.byref(t)), "reading MDF rows")
+    try:
+        r, w = t.rows, t.words
+        snp = np.ctypeslib.as_array(C.cast(t.snp, C.POINTER(C.c_uint64)), (max(r, 1) * w,))
+        pos = np.ctypeslib.as_array(C.cast(t.pos, C.POINTER(C.c_uint32)), (max(r, 1),))
+        bc = np.ctypeslib.as_array(C.cast(t.bc, C.POINTER(C.c_uint32)), (max(r, 1),))
+        idp = C.cast(t.id, C.POINTER(C.c_char_p))
+        out = {"snp": snp[:r * w].reshape(r, w).copy(), "pos": pos[:r].copy(),
+               "bc": bc[:r].copy(), "id": [idp[i].decode() for i in range(r)]}
+    finally:
+        host_lib().qld_snp_table_free(C.byref(t))
+    return out
+
+
+def mdf_write(path: str, chrom: str, snp: np.ndarray, pos, ids, bc) -> None:
+    snp = np.ascontiguousarray(snp, dtype=np.uint64)
+    pos = np.ascontiguousarray(pos, dtype=np.uint32)
+    bc = np.ascontiguousarray(bc, dtype=np.uint32)
+    rows, words = snp.shape
+    idarr = None
+    if ids is not None:
+        idarr = (C.c_char_p * max(1, rows))(*[i.encode() for i in ids])
+    _host_check(host_lib().qld_mdf_write(path.encode(), chrom.encode(), _ptr(snp), _ptr(pos),
+                                         None if idarr is None else C.cast(idarr, C.c_void_p),
+                                         _ptr(bc), rows, words), f"writing {path}")
 
 
 def format_float(val: float, precision: int = 9) -> str:

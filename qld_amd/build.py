@@ -52,7 +52,19 @@ def build_host() -> str:
     out = os.path.join(LIBDIR, "libqld_host.so")
     tmp = out + ".tmp"
     _run(["gcc", *HOST_FLAGS, f"-I{os.path.join(ROOT, 'include')}", "-o", tmp,
-          *srcs, f"-L{LIBDIR}", "-lqld_hip", f"-Wl,-rpath,$ORIGIN", "-lm"])
+          *srcs, f"-L{LIBDIR}", "-lqld_hip", f"-Wl,-rpath,$ORIGIN", "-lz", "-lm"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cli() -> str:
+    """qLD-compute: the drop-in command line over libqld_host + libqld_hip."""
+    out = os.path.join(LIBDIR, "qLD-compute")
+    tmp = out + ".tmp"
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter",
+          f"-I{os.path.join(ROOT, 'include')}", "-o", tmp,
+          os.path.join(PKG, "cli", "qld_compute.c"), f"-L{LIBDIR}", "-lqld_host",
+          "-lqld_hip", "-Wl,-rpath,$ORIGIN", "-lz", "-lm"])
     os.replace(tmp, out)
     return out
 
@@ -60,9 +72,10 @@ def build_host() -> str:
 def build_all() -> None:
     build_hip()
     build_host()
+    build_cli()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "host": build_host}[w]()
+        {"all": build_all, "hip": build_hip, "host": build_host, "cli": build_cli}[w]()

@@ -1,0 +1,460 @@
+/*
+ * qld_compute.c -- `qLD-compute` for the MI355X engine.
+ *
+ * Same command line, input directories and report files as the reference
+ * correlator (correlate_our_kernel.c:146-472 parses, load_balancer.c:122-357
+ * prepares a task, input_list.c:205-323 expands -inputList, and
+ * correlate_function.c:60-290 loads the tables and correlates), with the
+ * pairwise counts and r^2 computed on the GPU through qld_correlate_report().
+ * Input is MDF (-mdf); every task is computed (the reference's thread split
+ * can drop tasks, correlate_function.c:53-57; here there is no split).
+ */
+#define _GNU_SOURCE
+#include <dirent.h>
+#include <libgen.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include <unistd.h>
+
+#include "qld_host.h"
+
+typedef struct {
+    char input[QLD_PATH_MAX], input2[QLD_PATH_MAX], output[QLD_PATH_MAX];
+    int wmin1, wmax1, wmin2, wmax2;
+    int in2set, wset1, wset2;
+    qld_mdf_header h1, h2;
+    char **files1, **files2;
+    int nfiles1, nfiles2;
+} task;
+
+typedef struct {
+    char input[QLD_PATH_MAX], input2[QLD_PATH_MAX], output[QLD_PATH_MAX];
+    char list[QLD_PATH_MAX];
+    int in2set, wset1, wset2, listset, outset, pathset;
+    int wmin1, wmax1, wmin2, wmax2;
+    int ploidy, threads, mdf, gpu, device, dump;
+    float r2limit;
+    int engine, r2mode;
+} options;
+
+static void die(const char *fmt, const char *a)
+{
+    fprintf(stderr, fmt, a);
+    exit(1);
+}
+
+static double now(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return (double)tv.tv_sec + 1e-6 * (double)tv.tv_usec;
+}
+
+static void with_slash(char *out, const char *in)
+{
+    snprintf(out, QLD_PATH_MAX - 1, "%s", in);
+    size_t n = strlen(out);
+    if (n && out[n - 1] != '/') { out[n] = '/'; out[n + 1] = '\0'; }
+}
+
+static void usage(void)
+{
+    printf("qLD-compute (MI355X engine)\n"
+           "  -input DIR          MDF directory of the first region\n"
+           "  -input2 DIR         MDF directory of the second region (optional)\n"
+           "  -output FILE        report file (with -inputList: FILE stem, one\n"
+           "                      report per list line, FILE_<line>.txt)\n"
+           "  -ploidy P           haploid | phased_diploid | unphased_diploid\n"
+           "  -posWmin1/-posWmax1 POS   window of the first region\n"
+           "  -posWmin2/-posWmax2 POS   window of the second region\n"
+           "  -inputList FILE     lines input,posWmin1,posWmax1,input2,posWmin2,posWmax2\n"
+           "  -r2limit R          report pairs with r2 >= R (default 0.2)\n"
+           "  -mdf                MDF input (required: this build reads MDF)\n"
+           "  -gpu                accepted; the engine always runs on the GPU\n"
+           "  -blis, -sorted, -competing, -rTable, -threads T, -seed S\n"
+           "                      accepted, no effect (MDF files are read on all\n"
+           "                      host cores, up to 32)\n"
+           "  -engine E           auto | popcount | mfma_fp4 | mfma_i8 (default auto)\n"
+           "  -r2mode M           reference (bit-identical to the reference's float\n"
+           "                      r2, default) | exact (within 1 ulp of FP64)\n"
+           "  -device D           GPU ordinal (default 0)\n"
+           "  -dump-tables        write the tables each task would correlate to\n"
+           "                      <output>.tables instead of computing (no GPU)\n");
+}
+
+static int need_arg(int i, int argc) { return i != argc - 1; }
+
+static void parse_args(int argc, char **argv, options *o)
+{
+    memset(o, 0, sizeof *o);
+    o->r2limit = 0.2f;
+    o->threads = 1;
+    o->engine = QLD_ENGINE_AUTO;
+    o->r2mode = QLD_R2_REFERENCE;
+    if (argc < 2) die("qLD-compute needs some arguments to run. Run \"qLD-compute -help\" for more info%s\n", "");
+    if (strcmp(argv[1], "-help") && strcmp(argv[1], "-h") && strcmp(argv[1], "--help"))
+        printf("qLD-compute running with arguments:\n");
+    for (int i = 1; i < argc; ++i) {
+        const char *a = argv[i];
+        if (!strcmp(a, "-help") || !strcmp(a, "-h") || !strcmp(a, "--help")) {
+            usage();
+            exit(0);
+        }
+        if (!strcmp(a, "-input") || !strcmp(a, "-input2")) {
+            if (need_arg(i, argc)) {
+                printf("\t%s %s\n", a, argv[i + 1]);
+                char *dst = !strcmp(a, "-input") ? o->input : o->input2;
+                with_slash(dst, argv[++i]);
+                DIR *d = opendir(dst);
+                if (!d) die("\n ERROR: Directory %s does not exist.\n\n", dst);
+                closedir(d);
+                if (dst == o->input) o->pathset = 1;
+                else o->in2set = 1;
+            }
+            continue;
+        }
+        if (!strcmp(a, "-inputList")) {
+            if (!need_arg(i, argc)) die("\n ERROR: -inputList needs a file%s\n\n", "");
+            printf("\t%s %s\n", a, argv[i + 1]);
+            snprintf(o->list, sizeof o->list, "%s", argv[++i]);
+            o->listset = 1;
+            continue;
+        }
+        if (!strcmp(a, "-output")) {
+            if (need_arg(i, argc)) {
+                printf("\t%s %s\n", a, argv[i + 1]);
+                snprintf(o->output, sizeof o->output, "%s", argv[++i]);
+                o->outset = 1;
+            }
+            continue;
+        }
+        if (!strcmp(a, "-sampleList") || !strcmp(a, "-sampleList2"))
+            die("\n ERROR: %s: sample selection applies to VCF input; MDF rows are "
+                "already packed for their samples\n\n", a);
+        if (!strcmp(a, "-ploidy")) {
+            if (!need_arg(i, argc)) die("Please insert supported ploidy (see --help for more info)%s\n", "");
+            const char *p = argv[++i];
+            if (!strcmp(p, "haploid")) o->ploidy = 1;
+            else if (!strcmp(p, "phased_diploid") || !strcmp(p, "unphased_diploid")) o->ploidy = 2;
+            else die("Please insert supported ploidy (see --help for more info)%s\n", "");
+            continue;
+        }
+        int *iv = NULL, *set = NULL;
+        if (!strcmp(a, "-posWmin1")) { iv = &o->wmin1; set = &o->wset1; }
+        else if (!strcmp(a, "-posWmax1")) { iv = &o->wmax1; set = &o->wset1; }
+        else if (!strcmp(a, "-posWmin2")) { iv = &o->wmin2; set = &o->wset2; }
+        else if (!strcmp(a, "-posWmax2")) { iv = &o->wmax2; set = &o->wset2; }
+        if (iv) {
+            if (need_arg(i, argc)) { *iv = atoi(argv[++i]); *set = 1; }
+            continue;
+        }
+        if (!strcmp(a, "-r2limit")) {
+            if (need_arg(i, argc)) o->r2limit = (float)atof(argv[++i]);
+            continue;
+        }
+        if (!strcmp(a, "-threads")) {
+            if (need_arg(i, argc)) ++i;
+            continue;
+        }
+        if (!strcmp(a, "-device")) {
+            if (need_arg(i, argc)) o->device = atoi(argv[++i]);
+            continue;
+        }
+        if (!strcmp(a, "-seed")) {
+            if (need_arg(i, argc)) ++i;
+            continue;
+        }
+        if (!strcmp(a, "-engine") && need_arg(i, argc)) {
+            const char *e = argv[++i];
+            if (!strcmp(e, "auto")) o->engine = QLD_ENGINE_AUTO;
+            else if (!strcmp(e, "popcount")) o->engine = QLD_ENGINE_POPCOUNT;
+            else if (!strcmp(e, "mfma_fp4")) o->engine = QLD_ENGINE_MFMA_FP4;
+            else if (!strcmp(e, "mfma_i8")) o->engine = QLD_ENGINE_MFMA_I8;
+            else die("\nERROR: unknown engine %s\n\n", e);
+            continue;
+        }
+        if (!strcmp(a, "-r2mode") && need_arg(i, argc)) {
+            const char *e = argv[++i];
+            if (!strcmp(e, "reference")) o->r2mode = QLD_R2_REFERENCE;
+            else if (!strcmp(e, "exact")) o->r2mode = QLD_R2_EXACT;
+            else die("\nERROR: unknown r2 mode %s\n\n", e);
+            continue;
+        }
+        if (!strcmp(a, "-mdf")) { o->mdf = 1; continue; }
+        if (!strcmp(a, "-dump-tables")) { o->dump = 1; continue; }
+        if (!strcmp(a, "-gpu")) { o->gpu = 1; continue; }
+        if (!strcmp(a, "-blis") || !strcmp(a, "-sorted") || !strcmp(a, "-competing") ||
+            !strcmp(a, "-rTable"))
+            continue;
+        die("\nERROR: %s is not a valid command line parameter\n\n", a);
+    }
+    if (!o->ploidy) die("Please insert supported ploidy (see --help for more info)%s\n", "");
+    if (o->listset) {
+        o->pathset = 1;
+        o->wset1 = 1;
+        o->in2set = 1;
+        o->wset2 = 1;
+    }
+    if (!o->pathset) die("\n ERROR: Please specify a path for the input with -input%s\n\n", "");
+    if (!o->outset) die("\n ERROR: Please specify an alignment folder with -output%s\n\n", "");
+    if (!o->mdf)
+        die("\n ERROR: this build reads MDF input; add -mdf (VCF chunks are converted to "
+            "MDF by the MDF tool first)%s\n\n", "");
+    long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+    o->threads = ncpu < 1 ? 1 : ncpu > 32 ? 32 : (int)ncpu;
+}
+
+static void check(int rc, const char *what)
+{
+    if (rc != QLD_OK) {
+        fprintf(stderr, "\n ERROR: %s: %s\n\n", what, qld_host_last_error());
+        exit(1);
+    }
+}
+
+/* Window and task set-up of preprocess_data (load_balancer.c:122-357). */
+static void prepare_task(task *t, FILE *rep, int id)
+{
+    check(qld_mdf_read_header(t->input, &t->h1), "header of -input");
+    if (t->in2set) {
+        check(qld_mdf_read_header(t->input2, &t->h2), "header of -input2");
+        if (t->h1.snip_size != t->h2.snip_size) {
+            fprintf(stderr, "\n ERROR: The two inputs have different snip sizes.\n\t%d != %d\n",
+                    t->h1.snip_size, t->h2.snip_size);
+            exit(1);
+        }
+    } else {
+        t->h2 = t->h1;
+    }
+    const int pmin = t->h1.pos_min, pmax = t->h1.pos_max;
+    const int pmin2 = t->h2.pos_min, pmax2 = t->h2.pos_max;
+    if (t->wset1) {
+        if (t->wmin1 < pmin || t->wmin1 > pmax) t->wmin1 = pmin;
+        if (t->wmax1 < pmin || t->wmax1 > pmax) t->wmax1 = pmax;
+        if (t->wset2 && t->in2set) {
+            if (t->wmin2 < pmin2 || t->wmin2 > pmax2) t->wmin2 = pmin2;
+            if (t->wmax2 < pmin2 || t->wmax2 > pmax2) t->wmax2 = pmax2;
+        } else if (t->wset2) {
+            if (t->wmin2 < pmin || t->wmin2 > pmax) t->wmin2 = pmin;
+            if (t->wmax2 < pmin || t->wmax2 > pmax) t->wmax2 = pmax;
+        } else if (t->in2set) {
+            t->wset2 = 1;
+            t->wmin2 = (t->wmin1 < pmin2 || t->wmin1 > pmax2) ? pmin2 : t->wmin1;
+            t->wmax2 = (t->wmax1 < pmin2 || t->wmax1 > pmax2) ? pmax2 : t->wmax1;
+        } else {
+            if (t->wmin2 < pmin || t->wmin2 > pmax) t->wmin2 = pmin;
+            if (t->wmax2 < pmin || t->wmax2 > pmax) t->wmax2 = pmax;
+        }
+    } else {
+        t->wset1 = 1;
+        t->wmin1 = pmin;
+        t->wmax1 = pmax;
+        t->wmin2 = pmin;
+        t->wmax2 = pmax;
+    }
+    if (t->in2set && t->wset1 && t->wset2 && t->wmin1 == t->wmin2 && t->wmax1 == t->wmax2)
+        t->wset2 = 0;
+    int spf = t->h1.snips_per_file, spf2 = t->h2.snips_per_file;
+    if (spf < 0 || spf > t->h1.total_snips) spf = t->h1.total_snips;
+    if (t->in2set && (spf2 < 0 || spf2 > t->h2.total_snips)) spf2 = t->h2.total_snips;
+    check(qld_mdf_find_files(t->input, t->h1.id, spf, t->h1.total_snips, t->wmin1, t->wmax1,
+                             &t->files1, &t->nfiles1), "chunks of -input");
+    check(qld_mdf_find_files(t->in2set ? t->input2 : t->input, t->h2.id, spf2,
+                             t->h2.total_snips, t->wmin2, t->wmax2, &t->files2, &t->nfiles2),
+          "chunks of the second region");
+    if (rep) {
+        /* the task line of input_report.txt (input_list.c:43-59) */
+        fprintf(rep, "%d)\n\tSamples:  %10d\t%10d\n\tSites:    %10d\t%10d\n", id + 1,
+                t->h1.snip_size, t->h2.snip_size, t->wmax1 - t->wmin1, t->wmax2 - t->wmin2);
+        fflush(rep);
+    }
+}
+
+static void strip_ext(char *f)
+{
+    char *end = f + strlen(f);
+    while (end > f && *end != '.' && *end != '\\' && *end != '/') --end;
+    if (end > f && *end == '.' && end[-1] != '\\' && end[-1] != '/') *end = '\0';
+}
+
+/* -inputList expansion (input_list.c:205-323): one task per valid line. */
+static task *list_tasks(options *o, int *ntask, FILE *rep)
+{
+    FILE *fp = fopen(o->list, "r");
+    if (!fp) die("ERROR: Input list file not existing!%s\n", "");
+    strip_ext(o->output);
+    task *ts = NULL;
+    int n = 0, line_no = 0;
+    char line[4096];
+    while (++line_no) {
+        if (!fgets(line, sizeof line, fp)) break;
+        char in1[1024], a[1024], b[1024], in2[1024], c[1024], d[1024];
+        if (sscanf(line, "%1023[^,],%1023[^,],%1023[^,],%1023[^,],%1023[^,],%1023[^\n]",
+                   in1, a, b, in2, c, d) != 6) {
+            printf("ABORTING: Line %d ---Invalid structure---\n", line_no);
+            continue;
+        }
+        task t;
+        memset(&t, 0, sizeof t);
+        with_slash(t.input, in1);
+        with_slash(t.input2, in2);
+        t.wmin1 = atoi(a);
+        t.wmax1 = atoi(b);
+        if (t.wmin1 >= t.wmax1) {
+            printf("ABORTING: Line %d ---posWmin greater than posWmax---\n", line_no);
+            continue;
+        }
+        t.wmin2 = atoi(c);
+        t.wmax2 = atoi(d);
+        if (t.wmin2 >= t.wmax2) {
+            printf("ABORTING: Line %d ---posWmin2 greater than posWmax2---\n", line_no);
+            continue;
+        }
+        snprintf(t.output, sizeof t.output, "%s_%d.txt", o->output, line_no);
+        if (access(t.input, F_OK) == -1 || access(t.input2, F_OK) == -1) {
+            printf("ABORTING: Line %d ---Input files do not exist---\n", line_no);
+            continue;
+        }
+        t.in2set = o->in2set;
+        t.wset1 = o->wset1;
+        t.wset2 = o->wset2;
+        prepare_task(&t, rep, n);
+        task *nt = (task *)realloc(ts, (size_t)(n + 1) * sizeof(task));
+        if (!nt) die("ERROR: out of host memory%s\n", "");
+        ts = nt;
+        ts[n++] = t;
+    }
+    fclose(fp);
+    if (!n) die("ERROR: All files in the list are invalid!%s\n", "");
+    *ntask = n;
+    return ts;
+}
+
+static void dump_table(FILE *f, const qld_snp_table *t)
+{
+    int32_t hdr[2] = {t->rows, t->words};
+    fwrite(hdr, sizeof hdr, 1, f);
+    fwrite(t->pos, sizeof(uint32_t), (size_t)t->rows, f);
+    fwrite(t->bc, sizeof(uint32_t), (size_t)t->rows, f);
+    fwrite(t->snp, sizeof(uint64_t), (size_t)t->rows * (size_t)t->words, f);
+    for (int r = 0; r < t->rows; r++) fwrite(t->id[r], 1, strlen(t->id[r]) + 1, f);
+}
+
+/* Binary dump of one task's tables: "QLDT", int32 single, posWset2,
+ * snp_size, then table A and (unless single) table B as int32 rows, words,
+ * uint32 pos[rows], uint32 bc[rows], uint64 words[rows*words] and the IDs as
+ * NUL-terminated strings. */
+static void dump_tables(const task *t, int single, const qld_snp_table *A,
+                        const qld_snp_table *B)
+{
+    char path[QLD_PATH_MAX + 8];
+    snprintf(path, sizeof path, "%s.tables", t->output);
+    FILE *f = fopen(path, "wb");
+    if (!f) die("\n ERROR: cannot create %s\n\n", path);
+    int32_t hdr[3] = {single, t->wset2, t->h1.snip_size};
+    fwrite("QLDT", 4, 1, f);
+    fwrite(hdr, sizeof hdr, 1, f);
+    dump_table(f, A);
+    if (!single) dump_table(f, B);
+    if (fclose(f) != 0) die("\n ERROR: write error on %s\n\n", path);
+}
+
+/* Table loading and the correlate call of correlate_function.c:60-290. */
+static void run_task(qld_ctx *ctx, const options *o, task *t)
+{
+    FILE *fp = o->dump ? NULL : fopen(t->output, "wb");
+    if (!o->dump && !fp) die("\n ERROR: cannot create output file %s\n\n", t->output);
+    int in2set = t->in2set && strcmp(t->input, t->input2) != 0;
+    const int words1 = (t->h1.samples * o->ploidy + 63) / 64;
+    const int words2 = (t->h2.samples * o->ploidy + 63) / 64;
+    qld_snp_table A, B;
+    memset(&B, 0, sizeof B);
+    check(qld_mdf_read_table(t->files1, t->nfiles1, t->wmin1, t->wmax1, t->wset1, words1,
+                             o->threads, &A), "reading the first region");
+    const int single = !t->wset2 && !in2set;
+    if (in2set)
+        check(qld_mdf_read_table(t->files2, t->nfiles2, t->wmin2, t->wmax2, t->wset2, words2,
+                                 o->threads, &B), "reading the second region");
+    else if (t->wset2)
+        check(qld_mdf_read_table(t->files2, t->nfiles2, t->wmin2, t->wmax2, t->wset2, words1,
+                                 o->threads, &B), "reading the second region");
+    const qld_snp_table *b = single ? &A : &B;
+    if (!single && b->words != A.words) {
+        fprintf(stderr, "\n ERROR: the two regions pack %d and %d words per SNP\n\n",
+                A.words, b->words);
+        exit(1);
+    }
+    if (o->dump) {
+        dump_tables(t, single, &A, b);
+        qld_snp_table_free(&A);
+        qld_snp_table_free(&B);
+        return;
+    }
+    int rc = qld_correlate_report(ctx, fp, A.snp, A.pos, A.id, A.bc, A.rows,
+                                  b->snp, b->pos, b->id, b->bc, b->rows, A.words,
+                                  t->h1.snip_size, t->wset2, o->r2limit);
+    if (rc != QLD_OK) {
+        fprintf(stderr, "\n ERROR: GPU correlate failed (%d): %s\n\n", rc,
+                qld_gpu_last_error(ctx));
+        exit(1);
+    }
+    if (fclose(fp) != 0) die("\n ERROR: write error on %s\n\n", t->output);
+    printf("%s: %d x %d SNPs\n", t->output, A.rows, b->rows);
+    qld_snp_table_free(&A);
+    qld_snp_table_free(&B);
+}
+
+int main(int argc, char **argv)
+{
+    options o;
+    parse_args(argc, argv, &o);
+    if (o.gpu) printf("GPU kernel active\n");
+    printf("MDF input active\n");
+
+    FILE *rep = fopen("input_report.txt", "w");
+    task *tasks;
+    int ntask = 0;
+    if (o.listset) {
+        tasks = list_tasks(&o, &ntask, rep);
+    } else {
+        tasks = (task *)calloc(1, sizeof(task));
+        if (!tasks) die("ERROR: out of host memory%s\n", "");
+        snprintf(tasks[0].input, QLD_PATH_MAX, "%s", o.input);
+        snprintf(tasks[0].input2, QLD_PATH_MAX, "%s", o.input2);
+        snprintf(tasks[0].output, QLD_PATH_MAX, "%s", o.output);
+        tasks[0].in2set = o.in2set;
+        tasks[0].wset1 = o.wset1;
+        tasks[0].wset2 = o.wset2;
+        tasks[0].wmin1 = o.wmin1;
+        tasks[0].wmax1 = o.wmax1;
+        tasks[0].wmin2 = o.wmin2;
+        tasks[0].wmax2 = o.wmax2;
+        prepare_task(&tasks[0], rep, 0);
+        ntask = 1;
+    }
+    if (rep) fclose(rep);
+
+    qld_cfg cfg;
+    qld_cfg_default(&cfg);
+    cfg.device = o.device;
+    cfg.engine = o.engine;
+    cfg.r2_mode = o.r2mode;
+    qld_ctx *ctx = NULL;
+    if (!o.dump && qld_gpu_init(&ctx, &cfg) != QLD_OK) {
+        fprintf(stderr, "\n ERROR: GPU init failed: %s\n\n", qld_gpu_last_error(NULL));
+        return 1;
+    }
+    double t0 = now();
+    for (int i = 0; i < ntask; i++) {
+        run_task(ctx, &o, &tasks[i]);
+        qld_mdf_free_files(tasks[i].files1, tasks[i].nfiles1);
+        qld_mdf_free_files(tasks[i].files2, tasks[i].nfiles2);
+    }
+    if (ctx) qld_gpu_release(ctx);
+    printf("Finished in: %.3fs\n", now() - t0);
+    printf("Output saved in: %s\n", dirname(o.output));
+    free(tasks);
+    return 0;
+}

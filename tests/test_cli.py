@@ -1,0 +1,227 @@
+"""qLD-compute (qld_amd/lib/qLD-compute) against the reference correlator.
+
+The golden reports under tests/golden/cli/expected/ were written by the
+reference's own CPU build (tests/golden/make_cli_golden.py) on the MDF
+directories under tests/golden/cli/data/.
+
+CPU: the CLI loads each case's tables (-dump-tables, no GPU) and the oracle
+computes counts, r^2 and the report text from them; the text must equal the
+reference's byte for byte, which pins header discovery, chunk selection,
+window clamping and row loading.  input_report.txt must match as well.
+GPU: the CLI runs for real, on every engine, and its report files must equal
+the reference's byte for byte.
+"""
+import json
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import qld_amd
+from conftest import GOLDEN, ROOT, unpack_bits
+
+CLI_DIR = os.path.join(GOLDEN, "cli")
+CLI_BIN = os.path.join(ROOT, "qld_amd", "lib", "qLD-compute")
+MANIFEST = json.load(open(os.path.join(CLI_DIR, "cases.json")))
+CASES = sorted(MANIFEST["cases"])
+
+
+def _workdir(tmp_path, name):
+    os.symlink(os.path.join(CLI_DIR, "data"), tmp_path / "data")
+    os.makedirs(tmp_path / "out")
+    if name == "input_list":
+        shutil.copy(os.path.join(CLI_DIR, "list.csv"), tmp_path / "list.csv")
+    return tmp_path
+
+
+def _run(tmp_path, args, extra=(), timeout=120):
+    cmd = [CLI_BIN] + list(args) + ["-output", "out/report.txt", "-mdf"] + list(extra)
+    return subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True,
+                          timeout=timeout)
+
+
+def _read_tables(path):
+    buf = open(path, "rb").read()
+    assert buf[:4] == b"QLDT"
+    single, wset2, snp_size = struct.unpack_from("<3i", buf, 4)
+    off = 16
+
+    def table():
+        nonlocal off
+        rows, words = struct.unpack_from("<2i", buf, off)
+        off += 8
+        pos = np.frombuffer(buf, np.uint32, rows, off); off += 4 * rows
+        bc = np.frombuffer(buf, np.uint32, rows, off); off += 4 * rows
+        snp = np.frombuffer(buf, np.uint64, rows * words, off).reshape(rows, words)
+        off += 8 * rows * words
+        ids = []
+        for _ in range(rows):
+            e = buf.index(b"\0", off)
+            ids.append(buf[off:e].decode())
+            off = e + 1
+        return dict(pos=pos, bc=bc, snp=snp, id=ids, words=words)
+
+    A = table()
+    B = A if single else table()
+    assert off == len(buf)
+    return single, wset2, snp_size, A, B
+
+
+def _oracle_report(path, A, B, snp_size, wset2, r2limit):
+    """Counts (exact integer matrix product of the unpacked bits), the
+    reference's float r^2 and writeResults text, all from the oracle."""
+    m, n = len(A["pos"]), len(B["pos"])
+    if m and n:
+        UA = unpack_bits(A["snp"], A["words"] * 64)
+        UB = unpack_bits(B["snp"], B["words"] * 64)
+        counts = (UB @ UA.T).astype(np.uint32)
+        r2 = oracle.r2_reference(A["bc"], B["bc"], counts, snp_size)
+    else:
+        r2 = np.zeros((n, m), np.float32)
+    return oracle.write_results_text(path, A["pos"], A["id"], A["bc"], B["pos"], B["id"],
+                                     B["bc"], snp_size, r2, r2limit, wset2)
+
+
+def _r2limit(args):
+    return float(args[args.index("-r2limit") + 1]) if "-r2limit" in args else 0.2
+
+
+def _expected(name):
+    d = os.path.join(CLI_DIR, "expected", name)
+    return {f: open(os.path.join(d, f), "rb").read() for f in sorted(os.listdir(d))}
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_cli_tables_reproduce_reference_report(name, tmp_path):
+    args = MANIFEST["cases"][name]
+    wd = _workdir(tmp_path, name)
+    r = _run(wd, args, ["-dump-tables"])
+    assert r.returncode == 0, r.stderr
+    exp = _expected(name)
+    assert open(wd / "input_report.txt", "rb").read() == exp["input_report.txt"]
+    dumps = sorted(f for f in os.listdir(wd / "out") if f.endswith(".tables"))
+    reports = sorted(f for f in exp if f != "input_report.txt")
+    assert [d[:-len(".tables")] for d in dumps] == reports
+    for d, rep in zip(dumps, reports):
+        single, wset2, snp_size, A, B = _read_tables(wd / "out" / d)
+        text = _oracle_report(wd / "out" / (rep + ".oracle"), A, B, snp_size, wset2,
+                              _r2limit(args))
+        assert text.encode() == exp[rep], (name, rep)
+
+
+def test_cli_zero_pos_takes_chrom_lines_as_rows(tmp_path):
+    """With a window reaching POS 0 every chunk's #CHROM line becomes a row
+    (POS 0, ID "ID", BC 0, words 0), as the reference reads it."""
+    args = MANIFEST["cases"]["zero_pos"]
+    wd = _workdir(tmp_path, "zero_pos")
+    assert _run(wd, args, ["-dump-tables"]).returncode == 0
+    _, _, _, A, _ = _read_tables(wd / "out" / "report.txt.tables")
+    hdr = qld_amd.mdf_read_header(os.path.join(CLI_DIR, "data", "zp"))
+    chunks = -(-hdr.total_snips // hdr.snips_per_file)
+    fake = [i for i, s in enumerate(A["id"]) if s == "ID"]
+    assert len(fake) == chunks
+    assert all(A["pos"][i] == 0 and A["bc"][i] == 0 and not A["snp"][i].any() for i in fake)
+    assert len(A["pos"]) == hdr.total_snips + chunks
+
+
+@pytest.mark.parametrize("name", sorted(MANIFEST["errors"]))
+def test_cli_error_cases_exit_instead_of_hanging(name, tmp_path):
+    wd = _workdir(tmp_path, name)
+    r = _run(wd, MANIFEST["errors"][name], ["-dump-tables"], timeout=60)
+    assert r.returncode != 0
+    assert "ERROR" in r.stderr or "ploidy" in r.stderr
+
+
+def test_cli_rejects_vcf_input_and_unknown_flags(tmp_path):
+    wd = _workdir(tmp_path, "x")
+    r = subprocess.run([CLI_BIN, "-input", "data/chrA", "-ploidy", "haploid", "-output",
+                        "out/r.txt"], cwd=wd, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "-mdf" in r.stderr
+    r = _run(wd, ["-input", "data/chrA", "-ploidy", "haploid", "-bogus"])
+    assert r.returncode == 1 and "-bogus is not a valid" in r.stderr
+
+
+def test_mdf_writer_reproduces_fixture_files():
+    """Rows read back from every fixture chunk and written again with
+    qld_mdf_write give the same bytes (the MDF tool's layout)."""
+    for name in ("chrA", "chrB", "chrC", "dip"):
+        d = os.path.join(CLI_DIR, "data", name)
+        for fn in sorted(os.listdir(d)):
+            if not fn.endswith(".mdf"):
+                continue
+            path = os.path.join(d, fn)
+            words = int(open(path).read().split("\n", 3)[2].split("\t")[-1].split("_")[1]) + 1
+            t = qld_amd.mdf_read_table([path], 1, 2**31 - 1, words)
+            out = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"qld_mdf_{os.getpid()}.mdf")
+            qld_amd.mdf_write(out, name, t["snp"], t["pos"], t["id"], t["bc"])
+            same = open(out, "rb").read() == open(path, "rb").read()
+            os.unlink(out)
+            assert same, path
+
+
+def test_mdf_rows_match_bit_counts():
+    """Loaded words are the packed rows: popcount equals the BC column and the
+    padding bits past the sample count are zero."""
+    for name, n in (("chrA", 150), ("chrB", 150), ("dip", 80)):
+        d = os.path.join(CLI_DIR, "data", name)
+        files = sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".mdf"))
+        t = qld_amd.mdf_read_table(files, 1, 2**31 - 1, (n + 63) // 64)
+        bits = unpack_bits(t["snp"], t["snp"].shape[1] * 64)
+        assert (bits.sum(axis=1) == t["bc"]).all()
+        assert not bits[:, n:].any()
+
+
+def test_mdf_parallel_read_is_deterministic():
+    d = os.path.join(CLI_DIR, "data", "chrA")
+    h = qld_amd.mdf_read_header(d)
+    files = qld_amd.mdf_find_files(d, "chrA", 25, 96, h.pos_min, h.pos_max)
+    assert len(files) == 4
+    t1 = qld_amd.mdf_read_table(files, h.pos_min, h.pos_max, 3, nthreads=1)
+    t8 = qld_amd.mdf_read_table(files, h.pos_min, h.pos_max, 3, nthreads=8)
+    assert len(t1["pos"]) == 96
+    assert (t1["snp"] == t8["snp"]).all() and t1["id"] == t8["id"]
+    assert (t1["pos"] == t8["pos"]).all() and (t1["bc"] == t8["bc"]).all()
+
+
+def test_mdf_find_files_window_past_last_chunk_is_an_error():
+    """A window ending past the last chunk makes the reference rescan the
+    directory forever; here it is an error."""
+    d = os.path.join(CLI_DIR, "data", "chrA")
+    with pytest.raises(qld_amd.QldError, match="continues"):
+        qld_amd.mdf_find_files(d, "chrA", 25, 96, 1000, 10**6)
+
+
+def test_mdf_row_with_wrong_word_count_is_an_error(tmp_path):
+    src = os.path.join(CLI_DIR, "data", "chrB", "chrB_61_70_6181_6393.mdf")
+    if not os.path.exists(src):
+        src = sorted(os.path.join(CLI_DIR, "data", "chrB", f)
+                     for f in os.listdir(os.path.join(CLI_DIR, "data", "chrB"))
+                     if f.endswith(".mdf"))[-1]
+    text = open(src).read().split("\n")
+    text[4] = text[4].rsplit("\t", 1)[0]          # drop one word of the second row
+    bad = tmp_path / "bad.mdf"
+    bad.write_text("\n".join(text))
+    with pytest.raises(qld_amd.QldError, match="packed words"):
+        qld_amd.mdf_read_table([str(bad)], 1, 2**31 - 1, 3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["auto", "popcount", "mfma_fp4", "mfma_i8"])
+def test_cli_reports_equal_reference_on_gpu(engine, tmp_path):
+    for name in CASES:
+        wd = tmp_path / f"{name}_{engine}"
+        os.makedirs(wd)
+        _workdir(wd, name)
+        print(f"[cli] {name} engine={engine}", flush=True)
+        r = _run(wd, MANIFEST["cases"][name], ["-engine", engine])
+        assert r.returncode == 0, (name, r.stderr)
+        exp = _expected(name)
+        got = {f: open(wd / "out" / f, "rb").read() for f in sorted(os.listdir(wd / "out"))}
+        got["input_report.txt"] = open(wd / "input_report.txt", "rb").read()
+        assert sorted(got) == sorted(exp), name
+        for f in exp:
+            assert got[f] == exp[f], (name, f, engine)
