@@ -213,6 +213,65 @@ def rect_bench(args, steps):
             "roofline": roofline(engine, pairs, W, N, k_s)}
 
 
+def report_bench(args, A, bc, N):
+    """End to end through the drop-in report call: host tables in, report text
+    out (to /dev/null), r2limit 0.2, single region.  Device-side selection with
+    streamed hits (qld_correlate_report) vs the full r^2 matrix copied to the
+    host and scanned there (qld_gpu_correlate + qld_write_results, the
+    reference's correlate_gpu -> writeResults flow)."""
+    import ctypes as C
+    import qld_amd
+    m = A.shape[0]
+    W = A.shape[1]
+    pos = (np.arange(m, dtype=np.uint32) * 50 + 10000)
+    ids = [f"rs{i}" for i in range(m)]
+    idarr = (C.c_char_p * m)(*[s.encode() for s in ids])
+    idp = C.cast(idarr, C.c_void_p)
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    eng = qld_amd.Engine(engine="auto", r2_mode="reference")
+    host = qld_amd.host_lib()
+    lim = 0.2
+
+    def streamed():
+        fp = libc.fopen(b"/dev/null", b"w")
+        t0 = time.perf_counter()
+        rc = host.qld_correlate_report(eng._ctx, fp, A.ctypes.data, pos.ctypes.data, idp,
+                                       bc.ctypes.data, m, A.ctypes.data, pos.ctypes.data, idp,
+                                       bc.ctypes.data, m, W, N, 0, C.c_float(lim))
+        libc.fclose(fp)
+        assert rc == 0
+        return time.perf_counter() - t0
+
+    def full_matrix():
+        fp = libc.fopen(b"/dev/null", b"w")
+        t0 = time.perf_counter()
+        res, _ = eng.correlate(A, bc, snp_size=N, triangular=True)
+        host.qld_write_results_header(fp)
+        lines = host.qld_write_results(fp, pos.ctypes.data, idp, bc.ctypes.data, m,
+                                       pos.ctypes.data, idp, bc.ctypes.data, m, N,
+                                       res.ctypes.data, C.c_float(lim), 0)
+        libc.fclose(fp)
+        return time.perf_counter() - t0, lines
+
+    log("[bench] report: streamed device selection")
+    streamed()
+    ts = min(streamed() for _ in range(2))
+    log("[bench] report: full matrix to host")
+    tf, lines = full_matrix()
+    eng.close()
+    pairs = m * (m - 1) // 2
+    log(f"[bench] report {m} SNPs: streamed {ts:.3f} s, full matrix {tf:.3f} s, {lines} lines")
+    return {"workload": f"drop-in report, {m} SNPs x {N} samples, r2limit {lim}, text to "
+                        "/dev/null, host tables in",
+            "lines": int(lines), "pairs": pairs,
+            "streamed_s": round(ts, 3), "full_matrix_s": round(tf, 3),
+            "streamed_gpairs_s": round(pairs / ts / 1e9, 3),
+            "speedup": round(tf / ts, 2)}
+
+
 def serving_bench(args):
     """Poisson arrivals of LD queries (window size LogUniform(20, 5000), a
     uniformly chosen resident chromosome, uniform start) against three
@@ -298,6 +357,7 @@ def main():
     ap.add_argument("--no-windows", action="store_true")
     ap.add_argument("--no-rect", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-report", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size "
                          "kernel for calibrating HBM traffic counters)")
@@ -409,6 +469,8 @@ def main():
         out["rect_100k"] = rect_bench(args, max(3, args.steps // 2))
     if not args.no_serving:
         out["serving"] = serving_bench(args)
+    if not args.no_report:
+        out["report_end_to_end"] = report_bench(args, A, bc, N)
 
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle restatement of the BLIS path)")

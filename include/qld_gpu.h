@@ -262,6 +262,45 @@ int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_window *win
                          int nwin, float *out, int reps, int *best_cutoff,
                          float *best_ms);
 
+/* ---- report selection on the device ------------------------------------
+ *
+ * The reference prints pair (i, j) -- i a row of table B, j a column of
+ * table A, j > i when its second window is unset -- when r^2 >= r2limit and
+ * r^2 < 100 (writeResults, correlate_IO.c:403-410), after copying the whole
+ * r^2 matrix to the host.  These calls select the same pairs on the GPU. */
+
+/* Hits of rows [row0, row0 + rows) of table B: row r's hits are entries
+ * offsets[r] .. offsets[r+1]-1 of cols (column j of table A, ascending) and
+ * r2.  Pointers are valid during the call only.  Return 0 to continue. */
+typedef int (*qld_hits_fn)(void *user, int row0, int rows, const uint64_t *offsets,
+                           const uint32_t *cols, const float *r2);
+
+/* Host tables in, selected pairs out: r^2 of B (n rows) against A (m rows)
+ * is computed in blocks of about `block_pairs` cells (<= 0: 2^26) of B
+ * rows; each block is compacted on the device (`upper`: only j > i) and its
+ * hits are copied to pinned host memory and handed to `fn` on the calling
+ * thread, in row order, while the next block computes.  A single region
+ * passes the same table pointers as A and B with upper = 1 (only the upper
+ * triangle is computed).  Replaces the full-matrix D2H of correlate_gpu
+ * (gemm_gpu.c:849-861) for report output. */
+int qld_correlate_hits(qld_ctx *ctx,
+                       const uint64_t *A, const uint32_t *bcA, int m,
+                       const uint64_t *B, const uint32_t *bcB, int n,
+                       int words, int snp_size, int upper, float r2limit,
+                       int64_t block_pairs, qld_hits_fn fn, void *user);
+
+/* Device-resident variant for one block already in HBM: results
+ * dres[r * ldr + c] = r^2 of B row row0 + r and A column col0 + c, c < cols.
+ * Counts hits per row into dcounts[rows], their exclusive prefix into
+ * doffsets[rows + 1], sets *total, and -- when *total <= capacity -- writes
+ * the hits (column index, r^2) into dcols / dr2 in row-major order.
+ * QLD_ERR_BUDGET when capacity is too small (*total says how much is
+ * needed).  Synchronizes the context stream to read the total. */
+int qld_compact_dev(qld_ctx *ctx, const float *dres, int64_t ldr, int rows, int cols,
+                    int row0, int col0, int upper, float r2limit,
+                    uint32_t *dcounts, uint64_t *doffsets, uint32_t *dcols, float *dr2,
+                    uint64_t capacity, uint64_t *total);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,10 @@ GPU_SYMBOLS = [
     ("qld_get_tile_cutoff", C.c_int, [_vp]),
     ("qld_autotune_windows", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int,
                                        C.POINTER(C.c_int), _f32p]),
+    ("qld_correlate_hits", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_float, C.c_int64, _vp, _vp]),
+    ("qld_compact_dev", C.c_int, [_vp, _vp, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, C.c_float, _vp, _vp, _vp, _vp, C.c_uint64, _vp]),
 ]
 
 REQ_KEEP = 0
@@ -146,6 +150,9 @@ HOST_SYMBOLS = [
 ]
 
 PATH_MAX = 4096
+
+_HITS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_uint64),
+                       C.POINTER(C.c_uint32), C.POINTER(C.c_float))
 
 
 class MdfHeader(C.Structure):
@@ -405,6 +412,59 @@ class Engine:
             self._ctx, _ptr(A), _ptr(bcA), m, _ptr(B), _ptr(bcB), n, W,
             int(snp_size), 1 if triangular else 0, _ptr(res), _ptr(cnt)))
         return res, cnt
+
+    def correlate_hits(self, A, bcA, B=None, bcB=None, snp_size=None, upper=None,
+                       r2limit=0.2, block_pairs=0):
+        """Pairs the report prints, selected on the device (qld_correlate_hits):
+        returns (offsets[n + 1] uint64, cols uint32, r2 float32) over all
+        rows of B in order."""
+        A = np.ascontiguousarray(A, dtype=np.uint64)
+        bcA = np.ascontiguousarray(bcA, dtype=np.uint32)
+        if B is None:
+            B, bcB = A, bcA
+            if upper is None:
+                upper = True
+        B = np.ascontiguousarray(B, dtype=np.uint64)
+        bcB = np.ascontiguousarray(bcB, dtype=np.uint32)
+        m, W = A.shape
+        n = B.shape[0]
+        if snp_size is None:
+            snp_size = W * 64
+        counts = np.zeros(n, dtype=np.uint64)
+        parts = []
+
+        def cb(_user, row0, rows, off, cols, r2):
+            o = np.ctypeslib.as_array(off, (rows + 1,)).copy()
+            h = int(o[-1])
+            c = np.ctypeslib.as_array(cols, (h,)).copy() if h else np.zeros(0, np.uint32)
+            v = np.ctypeslib.as_array(r2, (h,)).copy() if h else np.zeros(0, np.float32)
+            counts[row0:row0 + rows] = np.diff(o)
+            parts.append((row0, c, v))
+            return 0
+
+        fn = _HITS_FN(cb)
+        self._check(hip_lib().qld_correlate_hits(
+            self._ctx, _ptr(A), _ptr(bcA), m, _ptr(B), _ptr(bcB), n, W, int(snp_size),
+            1 if upper else 0, C.c_float(r2limit), int(block_pairs),
+            C.cast(fn, C.c_void_p), None))
+        parts.sort(key=lambda t: t[0])
+        offsets = np.zeros(n + 1, dtype=np.uint64)
+        np.cumsum(counts, out=offsets[1:])
+        cols = np.concatenate([p[1] for p in parts]) if parts else np.zeros(0, np.uint32)
+        r2 = np.concatenate([p[2] for p in parts]) if parts else np.zeros(0, np.float32)
+        return offsets, cols, r2
+
+    def compact_dev(self, dres, ldr, rows, cols, row0, col0, upper, r2limit, dcounts,
+                    doffsets, dcols, dr2, capacity):
+        """qld_compact_dev on device buffers; returns the hit total."""
+        def p(x):
+            return None if x is None else (x.ptr if isinstance(x, DevBuffer) else int(x))
+        total = C.c_uint64()
+        self._check(hip_lib().qld_compact_dev(
+            self._ctx, p(dres), int(ldr), rows, cols, row0, col0, 1 if upper else 0,
+            C.c_float(r2limit), p(dcounts), p(doffsets), p(dcols), p(dr2), int(capacity),
+            C.byref(total)))
+        return total.value
 
     def alloc(self, nbytes: int) -> DevBuffer:
         return DevBuffer(self, nbytes)

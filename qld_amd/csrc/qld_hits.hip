@@ -1,0 +1,419 @@
+// Report selection on the device: r^2 blocks are compacted to the pairs the
+// report prints, and only those cross PCIe.
+//
+// The reference writes every pair of the result matrix through one loop on
+// the host (writeResults, correlate_IO.c:384-429): rows i of table B,
+// columns j of table A (j > i when the second window is not set), a line
+// when r^2 >= r2limit and r^2 < 100.  Here the matrix is produced block by
+// block of B rows in HBM; three small kernels select the same pairs in the
+// same order (count per row, exclusive scan, ordered write of (j, r^2)), the
+// hits of a block are copied to pinned host memory on a second stream, and
+// the caller's callback formats block b while the GPU computes block b+1.
+#include "qld_internal.h"
+#include "qld_ctx.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int HIT_WG = 256;
+
+__device__ __forceinline__ bool selected(float v, float lim)
+{
+    return v >= lim && v < 100.0f;      // NaN and the 123.456 marker fail
+}
+
+// First column of row r that the report may print.
+__device__ __forceinline__ int first_col(int r, int cols, int row0, int col0, int upper)
+{
+    if (!upper) return 0;
+    const long long c = (long long)row0 + r + 1 - col0;
+    return (int)(c < 0 ? 0 : (c > cols ? cols : c));
+}
+
+__global__ __launch_bounds__(HIT_WG) void hits_count_kernel(
+    const float *__restrict__ res, int64_t ldr, int cols, int row0, int col0, int upper,
+    float lim, uint32_t *__restrict__ counts)
+{
+    const int r = blockIdx.x;
+    const float *row = res + (int64_t)r * ldr;
+    uint32_t n = 0;
+    for (int c = first_col(r, cols, row0, col0, upper) + threadIdx.x; c < cols; c += HIT_WG)
+        n += selected(row[c], lim) ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_down(n, o, 64);
+    __shared__ uint32_t part[HIT_WG / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < HIT_WG / 64; w++) t += part[w];
+        counts[r] = t;
+    }
+}
+
+// Exclusive scan of `rows` counts into offsets[0..rows] (one workgroup).
+__global__ __launch_bounds__(1024) void hits_scan_kernel(const uint32_t *__restrict__ counts,
+                                                         int rows, uint64_t *__restrict__ offsets)
+{
+    __shared__ unsigned long long wsum[16];
+    __shared__ unsigned long long carry_s;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int base = 0; base < rows; base += 1024) {
+        const int i = base + threadIdx.x;
+        const unsigned long long v = i < rows ? counts[i] : 0ull;
+        unsigned long long x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            unsigned long long y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        unsigned long long pre = carry_s;
+        for (int w = 0; w < wv; w++) pre += wsum[w];
+        if (i < rows) offsets[i] = pre + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry_s = pre + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) offsets[rows] = carry_s;
+}
+
+// Ordered write: the row is walked in chunks of 4 x 256 columns; column
+// c = base + e*256 + t belongs to thread t, so lane order, wave order and e
+// order together are column order, and ballot prefix counts place each hit.
+__global__ __launch_bounds__(HIT_WG) void hits_write_kernel(
+    const float *__restrict__ res, int64_t ldr, int cols, int row0, int col0, int upper,
+    float lim, const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out_cols,
+    float *__restrict__ out_r2)
+{
+    constexpr int NW = HIT_WG / 64;
+    const int r = blockIdx.x;
+    const float *row = res + (int64_t)r * ldr;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __shared__ uint32_t tot[2][4][NW];
+    uint64_t pos = offsets[r];
+    int buf = 0;
+    for (int base = first_col(r, cols, row0, col0, upper); base < cols; base += 4 * HIT_WG) {
+        float v[4];
+        bool p[4];
+        uint32_t pre[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int c = base + e * HIT_WG + threadIdx.x;
+            v[e] = c < cols ? row[c] : 0.0f;
+            p[e] = c < cols && selected(v[e], lim);
+            const unsigned long long mask = __ballot(p[e]);
+            pre[e] = (uint32_t)__popcll(mask & below);
+            if (lane == 0) tot[buf][e][wv] = (uint32_t)__popcll(mask);
+        }
+        __syncthreads();
+        uint32_t run = 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            uint32_t before = run;
+            for (int w = 0; w < wv; w++) before += tot[buf][e][w];
+            if (p[e]) {
+                const uint64_t o = pos + before + pre[e];
+                out_cols[o] = (uint32_t)(col0 + base + e * HIT_WG + threadIdx.x);
+                out_r2[o] = v[e];
+            }
+            for (int w = 0; w < NW; w++) run += tot[buf][e][w];
+        }
+        pos += run;
+        buf ^= 1;       // the next chunk's totals go to the other buffer
+    }
+}
+
+
+}  // namespace
+
+// Enqueue count + scan for a block on `s`.
+static hipError_t enqueue_count_scan(const float *res, int64_t ldr, int rows, int cols, int row0,
+                                     int col0, int upper, float lim, uint32_t *counts,
+                                     uint64_t *offsets, hipStream_t s)
+{
+    // Rows with no columns still get their zero count from the kernel.
+    if (rows > 0)
+        hipLaunchKernelGGL(hits_count_kernel, dim3(rows), dim3(HIT_WG), 0, s, res, ldr, cols,
+                           row0, col0, upper, lim, counts);
+    hipLaunchKernelGGL(hits_scan_kernel, dim3(1), dim3(1024), 0, s, counts, rows, offsets);
+    return hipGetLastError();
+}
+
+extern "C" int qld_compact_dev(qld_ctx *ctx, const float *dres, int64_t ldr, int rows, int cols,
+                               int row0, int col0, int upper, float r2limit,
+                               uint32_t *dcounts, uint64_t *doffsets, uint32_t *dcols,
+                               float *dr2, uint64_t capacity, uint64_t *total)
+{
+    if (!ctx || !total || rows < 0 || cols < 0 || (rows && (!dres || !dcounts || !doffsets)) ||
+        ldr < cols) {
+        if (ctx) qld_set_error(ctx, "qld_compact_dev: bad arguments");
+        return QLD_ERR_INVALID;
+    }
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    hipError_t e = enqueue_count_scan(dres, ldr, rows, cols, row0, col0, upper, r2limit,
+                                      dcounts, doffsets, s);
+    if (e != hipSuccess) return qld_hip_fail(ctx, e, "hits count/scan");
+    uint64_t t = 0;
+    if ((e = hipMemcpyAsync(&t, doffsets + rows, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+        return qld_hip_fail(ctx, e, "hits total");
+    *total = t;
+    if (t > capacity) {
+        qld_set_error(ctx, "qld_compact_dev: %llu hits exceed capacity %llu",
+                      (unsigned long long)t, (unsigned long long)capacity);
+        return QLD_ERR_BUDGET;
+    }
+    if (t && (!dcols || !dr2)) {
+        qld_set_error(ctx, "qld_compact_dev: NULL hit buffers");
+        return QLD_ERR_INVALID;
+    }
+    if (t) {
+        hipLaunchKernelGGL(hits_write_kernel, dim3(rows), dim3(HIT_WG), 0, s, dres, ldr, cols,
+                           row0, col0, upper, r2limit, doffsets, dcols, dr2);
+        if ((e = hipGetLastError()) != hipSuccess) return qld_hip_fail(ctx, e, "hits write");
+    }
+    return QLD_OK;
+}
+
+namespace {
+
+struct Slot {
+    uint64_t *d_off = nullptr;      // rows + 1 offsets
+    uint32_t *d_cols = nullptr;
+    float *d_r2 = nullptr;
+    uint64_t cap = 0;
+    uint64_t *h_off = nullptr;      // pinned
+    uint32_t *h_cols = nullptr;
+    float *h_r2 = nullptr;
+    uint64_t h_cap = 0;
+    hipEvent_t done = nullptr;      // hits of this slot are in host memory
+    int row0 = 0, rows = 0;
+    bool pending = false;
+};
+
+struct HitsRun {
+    qld_ctx *ctx;
+    hipStream_t copy = nullptr;
+    hipEvent_t ev_total = nullptr, ev_written = nullptr;
+    std::vector<std::pair<void *, size_t>> dev;
+    std::vector<void *> pinned;
+    Slot slot[2];
+    uint64_t *h_total = nullptr;
+
+    int dalloc(void **p, size_t bytes)
+    {
+        int rc = qld_ctx_malloc(ctx, bytes ? bytes : 8, p);
+        if (rc == QLD_OK) dev.push_back({*p, bytes ? bytes : 8});
+        return rc;
+    }
+    void dfree(void *p)
+    {
+        for (size_t k = 0; k < dev.size(); k++)
+            if (dev[k].first == p) {
+                qld_ctx_mfree(ctx, p, dev[k].second);
+                dev.erase(dev.begin() + (long)k);
+                return;
+            }
+    }
+    int halloc(void **p, size_t bytes)
+    {
+        hipError_t e = hipHostMalloc(p, bytes ? bytes : 8, hipHostMallocDefault);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "pinned host buffer");
+        pinned.push_back(*p);
+        return QLD_OK;
+    }
+    void hfree(void *p)
+    {
+        auto it = std::find(pinned.begin(), pinned.end(), p);
+        if (it != pinned.end()) {
+            hipHostFree(p);
+            pinned.erase(it);
+        }
+    }
+    ~HitsRun()
+    {
+        if (copy) hipStreamSynchronize(copy);
+        hipStreamSynchronize(ctx->stream);
+        for (auto &d : dev) qld_ctx_mfree(ctx, d.first, d.second);
+        for (void *p : pinned) hipHostFree(p);
+        for (auto &s : slot)
+            if (s.done) hipEventDestroy(s.done);
+        if (ev_total) hipEventDestroy(ev_total);
+        if (ev_written) hipEventDestroy(ev_written);
+        if (copy) hipStreamDestroy(copy);
+    }
+    // Device and pinned hit capacity of a slot for `hits` entries (the slot's
+    // previous copy has completed: its callback already ran).
+    int reserve(Slot &s, uint64_t hits)
+    {
+        int rc;
+        if (hits > s.cap) {
+            uint64_t cap = std::max<uint64_t>(hits + hits / 4, 1u << 20);
+            dfree(s.d_cols);
+            dfree(s.d_r2);
+            s.d_cols = nullptr;
+            s.d_r2 = nullptr;
+            s.cap = 0;
+            if ((rc = dalloc((void **)&s.d_cols, cap * 4)) ||
+                (rc = dalloc((void **)&s.d_r2, cap * 4)))
+                return rc;
+            s.cap = cap;
+        }
+        if (hits > s.h_cap) {
+            uint64_t cap = std::max<uint64_t>(hits + hits / 4, 1u << 20);
+            hfree(s.h_cols);
+            hfree(s.h_r2);
+            s.h_cols = nullptr;
+            s.h_r2 = nullptr;
+            s.h_cap = 0;
+            if ((rc = halloc((void **)&s.h_cols, cap * 4)) ||
+                (rc = halloc((void **)&s.h_r2, cap * 4)))
+                return rc;
+            s.h_cap = cap;
+        }
+        return QLD_OK;
+    }
+};
+
+}  // namespace
+
+int qld_check_job(qld_ctx *ctx, const LdJob &j);
+
+extern "C" int qld_correlate_hits(qld_ctx *ctx,
+                                  const uint64_t *A, const uint32_t *bcA, int m,
+                                  const uint64_t *B, const uint32_t *bcB, int n,
+                                  int words, int snp_size, int upper, float r2limit,
+                                  int64_t block_pairs, qld_hits_fn fn, void *user)
+{
+    if (!ctx || !fn) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    LdJob probe{A, bcA, m, B, bcB, n, words, snp_size, 0, nullptr, m, nullptr, m};
+    int rc = qld_check_job(ctx, probe);
+    if (rc) return rc;
+    if (m == 0 || n == 0) return QLD_OK;
+    const bool same = A == B && bcA == bcB && m == n;
+    HitsRun run{ctx};
+    hipError_t e;
+    hipStream_t s = ctx->stream;
+
+    // Tables on the device.
+    uint64_t *dA, *dB;
+    uint32_t *dbcA, *dbcB;
+    const size_t ta = (size_t)m * words * 8, tb = (size_t)n * words * 8;
+    if ((rc = run.dalloc((void **)&dA, ta)) || (rc = run.dalloc((void **)&dbcA, (size_t)m * 4)))
+        return rc;
+    if (same) {
+        dB = dA;
+        dbcB = dbcA;
+    } else if ((rc = run.dalloc((void **)&dB, tb)) ||
+               (rc = run.dalloc((void **)&dbcB, (size_t)n * 4))) {
+        return rc;
+    }
+    if ((e = hipMemcpyAsync(dA, A, ta, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(dbcA, bcA, (size_t)m * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return qld_hip_fail(ctx, e, "H2D");
+    if (!same && ((e = hipMemcpyAsync(dB, B, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+                  (e = hipMemcpyAsync(dbcB, bcB, (size_t)n * 4, hipMemcpyHostToDevice, s)) !=
+                      hipSuccess))
+        return qld_hip_fail(ctx, e, "H2D");
+
+    // Blocks of B rows: about block_pairs r^2 cells each, a multiple of 256.
+    if (block_pairs <= 0) block_pairs = (int64_t)1 << 26;
+    int64_t R = block_pairs / std::max(m, 1);
+    R = std::max<int64_t>(256, (R + 255) / 256 * 256);
+    R = std::min<int64_t>(R, n);
+    float *dres;
+    uint32_t *dcounts;
+    if ((rc = run.dalloc((void **)&dres, (size_t)R * m * 4)) ||
+        (rc = run.dalloc((void **)&dcounts, (size_t)R * 4)))
+        return rc;
+    for (auto &sl : run.slot) {
+        if ((rc = run.dalloc((void **)&sl.d_off, (size_t)(R + 1) * 8)) ||
+            (rc = run.halloc((void **)&sl.h_off, (size_t)(R + 1) * 8)))
+            return rc;
+        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
+            return qld_hip_fail(ctx, e, "event");
+    }
+    if ((rc = run.halloc((void **)&run.h_total, 8))) return rc;
+    if ((e = hipStreamCreateWithFlags(&run.copy, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&run.ev_total, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&run.ev_written, hipEventDisableTiming)) != hipSuccess)
+        return qld_hip_fail(ctx, e, "stream/event");
+
+    auto deliver = [&](Slot &sl) -> int {
+        if ((e = hipEventSynchronize(sl.done)) != hipSuccess) return qld_hip_fail(ctx, e, "D2H");
+        sl.pending = false;
+        int u = fn(user, sl.row0, sl.rows, sl.h_off, sl.h_cols, sl.h_r2);
+        if (u) {
+            qld_set_error(ctx, "hits callback returned %d", u);
+            return QLD_ERR_INVALID;
+        }
+        return QLD_OK;
+    };
+
+    int prev = -1;
+    for (int64_t i0 = 0, b = 0; i0 < n; i0 += R, b++) {
+        const int rows = (int)std::min<int64_t>(R, n - i0);
+        const int col0 = upper ? (int)std::min<int64_t>(i0, m) : 0;
+        const int cols = m - col0;
+        const int64_t ldr = std::max(cols, 1);
+        Slot &sl = run.slot[b & 1];
+        // r^2 of block rows [i0, i0 + rows) x columns [col0, m).
+        if (cols > 0) {
+            if (upper && same) {
+                LdJob diag{dA + (size_t)i0 * words, dbcA + i0, rows, dA + (size_t)i0 * words,
+                           dbcA + i0, rows, words, snp_size, 1, dres, ldr, nullptr, rows};
+                if ((rc = qld_run_job(ctx, diag))) return rc;
+                if (i0 + rows < m) {
+                    const int rest = (int)(m - i0 - rows);
+                    LdJob rect{dA + (size_t)(i0 + rows) * words, dbcA + i0 + rows, rest,
+                               dA + (size_t)i0 * words, dbcA + i0, rows, words, snp_size, 0,
+                               dres + rows, ldr, nullptr, rest};
+                    if ((rc = qld_run_job(ctx, rect))) return rc;
+                }
+            } else {
+                LdJob job{dA + (size_t)col0 * words, dbcA + col0, cols, dB + (size_t)i0 * words,
+                          dbcB + i0, rows, words, snp_size, 0, dres, ldr, nullptr, cols};
+                if ((rc = qld_run_job(ctx, job))) return rc;
+            }
+        }
+        if ((e = enqueue_count_scan(dres, ldr, rows, cols, (int)i0, col0, upper, r2limit,
+                                    dcounts, sl.d_off, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(run.h_total, sl.d_off + rows, 8, hipMemcpyDeviceToHost, s)) !=
+                hipSuccess ||
+            (e = hipEventRecord(run.ev_total, s)) != hipSuccess)
+            return qld_hip_fail(ctx, e, "hits count");
+        // The previous block is formatted while this one computes.
+        if (prev >= 0 && (rc = deliver(run.slot[prev]))) return rc;
+        if ((e = hipEventSynchronize(run.ev_total)) != hipSuccess)
+            return qld_hip_fail(ctx, e, "hits total");
+        const uint64_t hits = *run.h_total;
+        if ((rc = run.reserve(sl, hits))) return rc;
+        if (hits) {
+            hipLaunchKernelGGL(hits_write_kernel, dim3(rows), dim3(HIT_WG), 0, s, dres, ldr, cols,
+                               (int)i0, col0, upper, r2limit, sl.d_off, sl.d_cols, sl.d_r2);
+            if ((e = hipGetLastError()) != hipSuccess) return qld_hip_fail(ctx, e, "hits write");
+        }
+        if ((e = hipEventRecord(run.ev_written, s)) != hipSuccess ||
+            (e = hipStreamWaitEvent(run.copy, run.ev_written, 0)) != hipSuccess ||
+            (e = hipMemcpyAsync(sl.h_off, sl.d_off, (size_t)(rows + 1) * 8,
+                                hipMemcpyDeviceToHost, run.copy)) != hipSuccess ||
+            (hits && ((e = hipMemcpyAsync(sl.h_cols, sl.d_cols, hits * 4, hipMemcpyDeviceToHost,
+                                          run.copy)) != hipSuccess ||
+                      (e = hipMemcpyAsync(sl.h_r2, sl.d_r2, hits * 4, hipMemcpyDeviceToHost,
+                                          run.copy)) != hipSuccess)) ||
+            (e = hipEventRecord(sl.done, run.copy)) != hipSuccess)
+            return qld_hip_fail(ctx, e, "hits D2H");
+        sl.row0 = (int)i0;
+        sl.rows = rows;
+        sl.pending = true;
+        prev = (int)(b & 1);
+    }
+    if (prev >= 0 && (rc = deliver(run.slot[prev]))) return rc;
+    return QLD_OK;
+}

@@ -311,6 +311,8 @@ static int check_job(qld_ctx *ctx, const LdJob &j)
     return QLD_OK;
 }
 
+int qld_check_job(qld_ctx *ctx, const LdJob &j) { return check_job(ctx, j); }
+
 // AUTO policy from measurements on MI355X (30,000 SNPs x 16,622 samples:
 // popcount 17.1 ms, int8 MFMA 11.2 ms, fp4 MFMA 5.0 ms per triangle): the fp4
 // engine wins once a launch has enough 256 x 256 tiles to keep the matrix

@@ -8,6 +8,7 @@
  */
 #include "qld_host.h"
 
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -146,6 +147,107 @@ long qld_write_results(FILE *fp,
     return lines;
 }
 
+/* ---- the drop-in correlate call ------------------------------------------ */
+
+/* Formats the hits of row blocks delivered by qld_correlate_hits: rows are cut
+ * into segments of about SEG_HITS lines, a group of segments is formatted in
+ * parallel into per-thread buffers and the buffers are written in order. */
+enum { SEG_HITS = 1 << 18 };
+
+typedef struct {
+    FILE *fp;
+    const uint32_t *posA, *bcA, *posB, *bcB;
+    char **idA, **idB;
+    int snp_size;
+    char (*fa)[32];
+    size_t line_max;
+    int nthreads;
+    char **buf;
+    size_t *cap, *len;
+    long lines;
+    int err;
+} hits_writer;
+
+static size_t max_id_len(char **ids, int n)
+{
+    size_t mx = 0;
+    for (int i = 0; ids && i < n; i++) {
+        size_t l = strlen(ids[i]);
+        if (l > mx) mx = l;
+    }
+    return mx;
+}
+
+static char *format_row_hits(char *p, const hits_writer *w, int i, uint64_t h0, uint64_t h1,
+                             const uint32_t *cols, const float *r2)
+{
+    if (h0 == h1) return p;
+    char fb[32];
+    qld_format_float(fb, 1.0f * (float)w->bcB[i] / (float)w->snp_size, 9);
+    const char *ib = w->idB ? w->idB[i] : "";
+    for (uint64_t h = h0; h < h1; h++) {
+        const uint32_t j = cols[h];
+        const char *ia = w->idA ? w->idA[j] : "";
+        p = put_u32(p, w->posB[i]);
+        *p++ = '\t';
+        p = put_u32(p, w->posA[j]);
+        *p++ = '\t';
+        p = put_str(p, ib);
+        *p++ = '\t';
+        p = put_str(p, ia);
+        *p++ = '\t';
+        p = put_str(p, fb);
+        *p++ = '\t';
+        p = put_str(p, w->fa[j]);
+        *p++ = '\t';
+        p += qld_format_float(p, r2[h], 9);
+        *p++ = '\n';
+    }
+    return p;
+}
+
+static int write_hits(void *user, int row0, int rows, const uint64_t *off,
+                      const uint32_t *cols, const float *r2)
+{
+    hits_writer *w = (hits_writer *)user;
+    if (w->err || off[rows] == 0) return w->err;
+    int r = 0;
+    while (r < rows) {
+        /* the next group: up to nthreads segments of about SEG_HITS hits */
+        int seg_lo[64], seg_hi[64], nseg = 0;
+        while (r < rows && nseg < w->nthreads) {
+            int r1 = r;
+            while (r1 < rows && (r1 == r || off[r1 + 1] - off[r] <= SEG_HITS)) r1++;
+            seg_lo[nseg] = r;
+            seg_hi[nseg] = r1;
+            nseg++;
+            r = r1;
+        }
+        for (int k = 0; k < nseg; k++) {
+            size_t need = (size_t)(off[seg_hi[k]] - off[seg_lo[k]]) * w->line_max + 1;
+            if (need > w->cap[k]) {
+                free(w->buf[k]);
+                w->buf[k] = (char *)malloc(need);
+                w->cap[k] = w->buf[k] ? need : 0;
+                if (!w->buf[k]) return w->err = -1;
+            }
+        }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(w->nthreads)
+        for (int k = 0; k < nseg; k++) {
+            char *p = w->buf[k];
+            for (int rr = seg_lo[k]; rr < seg_hi[k]; rr++)
+                p = format_row_hits(p, w, row0 + rr, off[rr], off[rr + 1], cols, r2);
+            w->len[k] = (size_t)(p - w->buf[k]);
+        }
+        for (int k = 0; k < nseg; k++) {
+            if (w->len[k] && fwrite(w->buf[k], 1, w->len[k], w->fp) != w->len[k])
+                return w->err = -1;
+            w->lines += (long)(off[seg_hi[k]] - off[seg_lo[k]]);
+        }
+    }
+    return 0;
+}
+
 int qld_correlate_report(qld_ctx *ctx, FILE *fp,
                          const uint64_t *tableA, const uint32_t *posA, char **idA,
                          const uint32_t *bcA, int m,
@@ -154,21 +256,44 @@ int qld_correlate_report(qld_ctx *ctx, FILE *fp,
                          int compressed_snp_size, int snp_size, int posWset2,
                          float r2limit)
 {
-    if (!ctx || !fp || m < 0 || n < 0) return QLD_ERR_INVALID;
-    size_t cells = (size_t)m * (size_t)n;
-    float *results = (float *)malloc((cells ? cells : 1) * sizeof(float));
-    if (!results) return QLD_ERR_NOMEM;
-    const int tri = (posWset2 == 0) && tableA == tableB && m == n;
+    if (!ctx || !fp || m < 0 || n < 0 || (m && !posA) || (n && !posB) || snp_size <= 0)
+        return QLD_ERR_INVALID;
+    qld_write_results_header(fp);
+    if (m == 0 || n == 0) return QLD_OK;
+    hits_writer w;
+    memset(&w, 0, sizeof w);
+    w.fp = fp;
+    w.posA = posA;
+    w.bcA = bcA;
+    w.posB = posB;
+    w.bcB = bcB;
+    w.idA = idA;
+    w.idB = idB;
+    w.snp_size = snp_size;
+    w.line_max = LINE_MAX_BYTES + max_id_len(idA, m) + max_id_len(idB, n);
+    w.nthreads = omp_get_max_threads();
+    if (w.nthreads > 64) w.nthreads = 64;
+    if (w.nthreads < 1) w.nthreads = 1;
+    w.fa = (char (*)[32])malloc((size_t)m * 32);
+    w.buf = (char **)calloc((size_t)w.nthreads, sizeof(char *));
+    w.cap = (size_t *)calloc((size_t)w.nthreads, sizeof(size_t));
+    w.len = (size_t *)calloc((size_t)w.nthreads, sizeof(size_t));
     int rc = QLD_OK;
-    if (cells)
-        rc = qld_gpu_correlate(ctx, tableA, bcA, m, tableB, bcB, n,
-                               compressed_snp_size, snp_size, tri, results, NULL);
-    if (rc == QLD_OK) {
-        qld_write_results_header(fp);
-        if (qld_write_results(fp, posA, idA, bcA, m, posB, idB, bcB, n, snp_size,
-                              results, r2limit, posWset2) < 0)
-            rc = QLD_ERR_NOMEM;
+    if (!w.fa || !w.buf || !w.cap || !w.len) {
+        rc = QLD_ERR_NOMEM;
+    } else {
+        /* frequency text of every table-A column, reused by every row */
+#pragma omp parallel for schedule(static) num_threads(w.nthreads)
+        for (int j = 0; j < m; j++)
+            qld_format_float(w.fa[j], 1.0f * (float)bcA[j] / (float)snp_size, 9);
+        rc = qld_correlate_hits(ctx, tableA, bcA, m, tableB, bcB, n, compressed_snp_size,
+                                snp_size, posWset2 == 0, r2limit, 0, write_hits, &w);
+        if (rc != QLD_OK && w.err) rc = QLD_ERR_NOMEM;   /* out of memory or a failed write */
     }
-    free(results);
+    for (int k = 0; w.buf && k < w.nthreads; k++) free(w.buf[k]);
+    free(w.buf);
+    free(w.cap);
+    free(w.len);
+    free(w.fa);
     return rc;
 }
