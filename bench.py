@@ -256,6 +256,8 @@ def report_bench(args, A, bc, N):
         libc.fclose(fp)
         return time.perf_counter() - t0, lines
 
+    cli = cli_bench(A, bc, N, pos, ids)
+
     log("[bench] report: streamed device selection")
     streamed()
     ts = min(streamed() for _ in range(2))
@@ -269,7 +271,115 @@ def report_bench(args, A, bc, N):
             "lines": int(lines), "pairs": pairs,
             "streamed_s": round(ts, 3), "full_matrix_s": round(tf, 3),
             "streamed_gpairs_s": round(pairs / ts / 1e9, 3),
-            "speedup": round(tf / ts, 2)}
+            "speedup": round(tf / ts, 2), "cli": cli}
+
+
+def cli_bench(A, bc, N, pos, ids, chunk=5000):
+    """qLD-compute on an MDF directory of the same table (chunks of 5,000
+    SNPs, written by qld_mdf_write): wall time of the whole command, report
+    to /dev/null, and the MDF ingest alone (-dump-tables)."""
+    import gzip
+    import shutil
+    import subprocess
+    import tempfile
+    import qld_amd
+    m = A.shape[0]
+    d = tempfile.mkdtemp(prefix="qld_mdf_")
+    try:
+        mdf = os.path.join(d, "chr1")
+        os.makedirs(mdf)
+        hdr = ("##fileformat=VCFv4.2\n#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t"
+               + "\t".join(f"S{i}" for i in range(N)) + "\n"
+               + f"chr1 {chunk} {N} {m} {int(pos[0])} {int(pos[-1])}\n")
+        with gzip.open(os.path.join(mdf, "chr1_header.vcf.gz"), "wb") as f:
+            f.write(hdr.encode())
+        for a in range(0, m, chunk):
+            b = min(a + chunk, m)
+            qld_amd.mdf_write(os.path.join(mdf, f"chr1_{a + 1}_{b}_{int(pos[a])}_{int(pos[b - 1])}.mdf"),
+                              "chr1", A[a:b], pos[a:b], ids[a:b], bc[a:b])
+        mb = sum(os.path.getsize(os.path.join(mdf, f)) for f in os.listdir(mdf)) / 1e6
+        exe = os.path.join(ROOT, "qld_amd", "lib", "qLD-compute")
+        base = [exe, "-input", mdf, "-ploidy", "haploid", "-mdf", "-gpu", "-r2limit", "0.2"]
+        t0 = time.perf_counter()
+        r = subprocess.run(base + ["-output", os.path.join(d, "t.txt"), "-dump-tables"],
+                           cwd=d, capture_output=True, text=True, timeout=600)
+        t_ingest = time.perf_counter() - t0
+        assert r.returncode == 0, r.stderr
+        t0 = time.perf_counter()
+        r = subprocess.run(base + ["-output", "/dev/null"], cwd=d, capture_output=True,
+                           text=True, timeout=600)
+        t_all = time.perf_counter() - t0
+        assert r.returncode == 0, r.stderr
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"[bench] qLD-compute {m} SNPs: {t_all:.3f} s end to end, ingest {t_ingest:.3f} s "
+        f"({mb:.0f} MB MDF)")
+    return {"mdf_mb": round(mb, 1), "wall_s": round(t_all, 3),
+            "ingest_and_dump_s": round(t_ingest, 3)}
+
+
+def list_bench(args):
+    """qLD-compute -inputList: 1,024 single-region windows (sizes LogUniform(50,
+    2000) SNPs) over one 100,000-SNP x 16,622-sample MDF chromosome, batched
+    into one persistent plan vs run task by task (-plain-tasks); every report
+    to its own file at r2limit 0.2."""
+    import gzip
+    import shutil
+    import subprocess
+    import tempfile
+    import qld_amd
+    N, m, nwin = 16622, args.list_snps, args.list_windows
+    A, bc = qld_amd.synth_blocks(args.seed + 7, m, N)
+    pos = (np.arange(m, dtype=np.uint32) * 50 + 10000)
+    ids = [f"rs{i}" for i in range(m)]
+    rng = np.random.default_rng(args.seed + 8)
+    sizes = np.exp(rng.uniform(np.log(50), np.log(2000), nwin)).round().astype(np.int64)
+    starts = rng.integers(0, m - sizes + 1)
+    pairs = int((sizes * (sizes - 1) // 2).sum())
+    d = tempfile.mkdtemp(prefix="qld_list_")
+    try:
+        mdf = os.path.join(d, "chr1")
+        os.makedirs(mdf)
+        chunk = 5000
+        hdr = ("##fileformat=VCFv4.2\n#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t"
+               + "\t".join(f"S{i}" for i in range(N)) + "\n"
+               + f"chr1 {chunk} {N} {m} {int(pos[0])} {int(pos[-1])}\n")
+        with gzip.open(os.path.join(mdf, "chr1_header.vcf.gz"), "wb") as f:
+            f.write(hdr.encode())
+        for a in range(0, m, chunk):
+            b = min(a + chunk, m)
+            qld_amd.mdf_write(os.path.join(mdf, f"chr1_{a + 1}_{b}_{int(pos[a])}_{int(pos[b - 1])}.mdf"),
+                              "chr1", A[a:b], pos[a:b], ids[a:b], bc[a:b])
+        with open(os.path.join(d, "list.csv"), "w") as f:
+            for s0, sz in zip(starts, sizes):
+                lo, hi = int(pos[s0]), int(pos[s0 + sz - 1])
+                f.write(f"{mdf},{lo},{hi},{mdf},{lo},{hi}\n")
+        exe = os.path.join(ROOT, "qld_amd", "lib", "qLD-compute")
+        out = {}
+        for name, extra in (("batched", []), ("plain_tasks", ["-plain-tasks"])):
+            od = os.path.join(d, name)
+            os.makedirs(od)
+            log(f"[bench] inputList {nwin} windows: {name}")
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "-inputList", os.path.join(d, "list.csv"), "-ploidy",
+                                "haploid", "-mdf", "-gpu", "-r2limit", "0.2", "-output",
+                                os.path.join(od, "w.txt")] + extra, cwd=d,
+                               capture_output=True, text=True, timeout=900)
+            out[name] = round(time.perf_counter() - t0, 3)
+            assert r.returncode == 0, r.stderr[-2000:]
+        same = all(open(os.path.join(d, "batched", f), "rb").read()
+                   == open(os.path.join(d, "plain_tasks", f), "rb").read()
+                   for f in os.listdir(os.path.join(d, "batched")))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"[bench] inputList: batched {out['batched']} s, task by task {out['plain_tasks']} s, "
+        f"reports identical: {same}")
+    return {"workload": f"qLD-compute -inputList, {nwin} windows LogUniform(50, 2000) SNPs over "
+                        f"{m} SNPs x {N} samples (MDF, chunks of 5,000), r2limit 0.2, one report "
+                        "file per window",
+            "pairs": pairs, "batched_s": out["batched"], "plain_tasks_s": out["plain_tasks"],
+            "speedup": round(out["plain_tasks"] / out["batched"], 2),
+            "reports_identical": bool(same)}
 
 
 def serving_bench(args):
@@ -358,6 +468,9 @@ def main():
     ap.add_argument("--no-rect", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-report", action="store_true")
+    ap.add_argument("--no-list", action="store_true")
+    ap.add_argument("--list-snps", type=int, default=100000)
+    ap.add_argument("--list-windows", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size "
                          "kernel for calibrating HBM traffic counters)")
@@ -471,6 +584,8 @@ def main():
         out["serving"] = serving_bench(args)
     if not args.no_report:
         out["report_end_to_end"] = report_bench(args, A, bc, N)
+    if not args.no_list:
+        out["input_list"] = list_bench(args)
 
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle restatement of the BLIS path)")

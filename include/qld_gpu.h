@@ -193,6 +193,24 @@ typedef struct qld_plan qld_plan;
  * max(out_offset + size*size) floats. */
 int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
                      int nwin, float *out, int mode, qld_plan **plan);
+/* A general unit of a plan, for the reference's two-window and two-input
+ * tasks (load_balancer.c:122-357, correlate_function.c:239-290): rows i of
+ * table rows [b_start, b_start + b_size) against columns j of
+ * [a_start, a_start + a_size) of the same table; r^2 of (i, j) is written to
+ * out[out_offset + i * a_size + j].  tri = 1 is a single region (a == b):
+ * only j > i is defined. */
+typedef struct qld_pair_window {
+    int32_t a_start, a_size;
+    int32_t b_start, b_size;
+    int64_t out_offset;
+    int32_t tri;
+    int32_t reserved;
+} qld_pair_window;
+
+/* As qld_plan_windows, for pair windows. */
+int qld_plan_pairs(qld_ctx *ctx, const qld_table *t, const qld_pair_window *wins,
+                   int nwin, float *out, int mode, qld_plan **plan);
+
 /* Queue one execution of the plan on the context's stream (asynchronous). */
 int qld_plan_run(qld_ctx *ctx, qld_plan *plan);
 /* Replay through the captured HIP graph (nonzero) or by direct launches. */

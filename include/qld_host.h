@@ -43,8 +43,10 @@ long qld_write_results(FILE *fp,
  * reference): compute r^2 on the GPU for table A (m rows) against table B
  * (n rows; same pointer and posWset2 == 0 for a single region) and write
  * header + report to fp.  Uses uint64 rows directly (the MDF word), not the
- * reference's uint32 split.  Returns QLD_OK or a negative status; never
- * exits. */
+ * reference's uint32 split.  The pairs to print are selected on the GPU
+ * block by block (qld_correlate_hits); only they cross PCIe, and the text of
+ * one block is formatted while the next computes.  Returns QLD_OK or a
+ * negative status; never exits. */
 int qld_correlate_report(qld_ctx *ctx, FILE *fp,
                          const uint64_t *tableA, const uint32_t *posA, char **idA,
                          const uint32_t *bcA, int m,
@@ -52,6 +54,17 @@ int qld_correlate_report(qld_ctx *ctx, FILE *fp,
                          const uint32_t *bcB, int n,
                          int compressed_snp_size, int snp_size, int posWset2,
                          float r2limit);
+
+/* Report lines of selected pairs (the hits qld_correlate_hits delivers or
+ * qld_compact_dev writes): rows i = row0 .. row0+rows-1 of table B, row
+ * i's hits at entries offsets[i-row0] .. offsets[i-row0+1]-1 of cols
+ * (table-A column) and r2.  Text as writeResults (correlate_IO.c:411-426),
+ * formatted on all OpenMP threads.  Returns lines written or -1. */
+long qld_write_hits(FILE *fp,
+                    const uint32_t *posA, char **idA, const uint32_t *bcA, int m,
+                    const uint32_t *posB, char **idB, const uint32_t *bcB, int n,
+                    int snp_size, int row0, int rows, const uint64_t *offsets,
+                    const uint32_t *cols, const float *r2);
 
 /* ---- synthetic haplotype data ---------------------------------------- */
 
@@ -136,6 +149,18 @@ typedef struct qld_snp_table {
 int qld_mdf_read_table(char *const *files, int nfiles, int pos_wmin, int pos_wmax,
                        int pos_wset, int words, int nthreads, qld_snp_table *t);
 void qld_snp_table_free(qld_snp_table *t);
+
+/* A reader that keeps every chunk file it has read: the text, the line
+ * index (each data line's POS) and the rows parsed so far, so that many
+ * windows over the same chunks (the reference's -inputList tasks) read and
+ * parse each file and row once.  qld_mdf_reader_table has the semantics of
+ * qld_mdf_read_table.  Not thread-safe: one caller at a time. */
+typedef struct qld_mdf_reader qld_mdf_reader;
+qld_mdf_reader *qld_mdf_reader_create(void);
+int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int pos_wmin,
+                         int pos_wmax, int pos_wset, int words, int nthreads,
+                         qld_snp_table *t);
+void qld_mdf_reader_free(qld_mdf_reader *r);
 
 /* Write one MDF chunk file exactly as the MDF tool does
  * (process_data.c:447-460): `##fileformat=MDFv0.1`, `##SNPs:<rows>`, the

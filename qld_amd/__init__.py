@@ -80,6 +80,7 @@ GPU_SYMBOLS = [
     ("qld_table_rows", C.c_int, [_vp]),
     ("qld_table_bytes", C.c_size_t, [_vp]),
     ("qld_plan_windows", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, C.POINTER(_vp)]),
+    ("qld_plan_pairs", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, C.POINTER(_vp)]),
     ("qld_plan_run", C.c_int, [_vp, _vp]),
     ("qld_plan_use_graph", C.c_int, [_vp, C.c_int]),
     ("qld_plan_stats", C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -117,6 +118,9 @@ class ServerCfg(C.Structure):
 
 
 WINDOW_DTYPE = np.dtype([("start", "<i4"), ("size", "<i4"), ("out_offset", "<i8")])
+PAIR_WINDOW_DTYPE = np.dtype([("a_start", "<i4"), ("a_size", "<i4"), ("b_start", "<i4"),
+                              ("b_size", "<i4"), ("out_offset", "<i8"), ("tri", "<i4"),
+                              ("reserved", "<i4")])
 BATCH_PLAIN = 0
 BATCH_PERSISTENT = 1
 
@@ -128,6 +132,8 @@ HOST_SYMBOLS = [
     ("qld_correlate_report", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                                        _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_float]),
+    ("qld_write_hits", C.c_long, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, C.c_int,
+                                  C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     ("qld_words", C.c_int, [C.c_int]),
     ("qld_synth_blocks", None, [C.c_uint64, C.c_int, C.c_int, C.c_double,
                                 C.c_double, C.c_double, C.c_double, _vp, _vp,
@@ -145,6 +151,10 @@ HOST_SYMBOLS = [
     ("qld_mdf_read_table", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_int, _vp]),
     ("qld_snp_table_free", None, [_vp]),
+    ("qld_mdf_reader_create", _vp, []),
+    ("qld_mdf_reader_table", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, _vp]),
+    ("qld_mdf_reader_free", None, [_vp]),
     ("qld_mdf_write", C.c_int, [C.c_char_p, C.c_char_p, _vp, _vp, _vp, _vp, C.c_int,
                                 C.c_int]),
 ]
@@ -269,15 +279,41 @@ def mdf_find_files(directory: str, hid: str, snips_per_file: int, total_snips: i
     return out
 
 
+class MdfReader:
+    """qld_mdf_reader: chunk files read and rows parsed once across calls."""
+
+    def __init__(self):
+        self._r = host_lib().qld_mdf_reader_create()
+
+    def table(self, files, pos_wmin, pos_wmax, words, pos_wset=1, nthreads=0) -> dict:
+        return mdf_read_table(files, pos_wmin, pos_wmax, words, pos_wset, nthreads, reader=self)
+
+    def close(self):
+        if self._r:
+            host_lib().qld_mdf_reader_free(self._r)
+            self._r = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def mdf_read_table(files: list[str], pos_wmin: int, pos_wmax: int, words: int,
-                   pos_wset: int = 1, nthreads: int = 0) -> dict:
+                   pos_wset: int = 1, nthreads: int = 0, reader: "MdfReader | None" = None) -> dict:
     """Rows of `files` in [pos_wmin, pos_wmax] -> dict(snp, pos, bc, id)."""
     arr = (C.c_char_p * max(1, len(files)))(*[f.encode() for f in files])
     t = _SnpTable()
-    _host_check(host_lib().qld_mdf_read_table(C.cast(arr, C.c_void_p), len(files), pos_wmin,
-                                              pos_wmax, pos_wset, words,
-                                              nthreads or (os.cpu_count() or 1),
-                                              C.byref(t)), "reading MDF rows")
+    nt = nthreads or (os.cpu_count() or 1)
+    if reader is None:
+        rc = host_lib().qld_mdf_read_table(C.cast(arr, C.c_void_p), len(files), pos_wmin,
+                                           pos_wmax, pos_wset, words, nt, C.byref(t))
+    else:
+        rc = host_lib().qld_mdf_reader_table(reader._r, C.cast(arr, C.c_void_p), len(files),
+                                             pos_wmin, pos_wmax, pos_wset, words, nt,
+                                             C.byref(t))
+    _host_check(rc, "reading MDF rows")
     try:
         r, w = t.rows, t.words
         snp = np.ctypeslib.as_array(C.cast(t.snp, C.POINTER(C.c_uint64)), (max(r, 1) * w,))
@@ -499,6 +535,8 @@ class Engine:
 
     def plan_windows(self, table: "Table", windows: np.ndarray, out,
                      mode=BATCH_PERSISTENT) -> "Plan":
+        """windows: WINDOW_DTYPE (triangles, qld_plan_windows) or
+        PAIR_WINDOW_DTYPE (triangles and rectangles, qld_plan_pairs)."""
         return Plan(self, table, windows, out, mode)
 
     def set_fp4_operands(self, packed: bool):
@@ -557,13 +595,14 @@ class Plan:
     """A prepared window set (qld_plan_windows)."""
 
     def __init__(self, eng: Engine, table: Table, windows: np.ndarray, out, mode):
-        w = np.ascontiguousarray(windows, dtype=WINDOW_DTYPE)
+        pairs = windows.dtype == PAIR_WINDOW_DTYPE
+        w = np.ascontiguousarray(windows, dtype=PAIR_WINDOW_DTYPE if pairs else WINDOW_DTYPE)
         self.eng = eng
         self.windows = w
         p = C.c_void_p()
         optr = out.ptr if isinstance(out, DevBuffer) else int(out)
-        eng._check(hip_lib().qld_plan_windows(eng._ctx, table._t, _ptr(w), len(w), optr,
-                                              int(mode), C.byref(p)))
+        fn = hip_lib().qld_plan_pairs if pairs else hip_lib().qld_plan_windows
+        eng._check(fn(eng._ctx, table._t, _ptr(w), len(w), optr, int(mode), C.byref(p)))
         self._p = p
 
     def run(self):

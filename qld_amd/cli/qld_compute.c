@@ -34,7 +34,7 @@ typedef struct {
     char list[QLD_PATH_MAX];
     int in2set, wset1, wset2, listset, outset, pathset;
     int wmin1, wmax1, wmin2, wmax2;
-    int ploidy, threads, mdf, gpu, device, dump;
+    int ploidy, threads, mdf, gpu, device, dump, plain_tasks;
     float r2limit;
     int engine, r2mode;
 } options;
@@ -81,7 +81,8 @@ static void usage(void)
            "                      r2, default) | exact (within 1 ulp of FP64)\n"
            "  -device D           GPU ordinal (default 0)\n"
            "  -dump-tables        write the tables each task would correlate to\n"
-           "                      <output>.tables instead of computing (no GPU)\n");
+           "                      <output>.tables instead of computing (no GPU)\n"
+           "  -plain-tasks        run -inputList tasks one by one (no batching)\n");
 }
 
 static int need_arg(int i, int argc) { return i != argc - 1; }
@@ -184,6 +185,7 @@ static void parse_args(int argc, char **argv, options *o)
         }
         if (!strcmp(a, "-mdf")) { o->mdf = 1; continue; }
         if (!strcmp(a, "-dump-tables")) { o->dump = 1; continue; }
+        if (!strcmp(a, "-plain-tasks")) { o->plain_tasks = 1; continue; }
         if (!strcmp(a, "-gpu")) { o->gpu = 1; continue; }
         if (!strcmp(a, "-blis") || !strcmp(a, "-sorted") || !strcmp(a, "-competing") ||
             !strcmp(a, "-rTable"))
@@ -361,55 +363,256 @@ static void dump_tables(const task *t, int single, const qld_snp_table *A,
     if (fclose(f) != 0) die("\n ERROR: write error on %s\n\n", path);
 }
 
-/* Table loading and the correlate call of correlate_function.c:60-290. */
+/* Every task reads through one reader: chunk files shared by tasks are read
+ * and parsed once. */
+static qld_mdf_reader *g_reader;
+
+/* The tables of one task, loaded as correlate_function.c:60-240 does. */
+typedef struct {
+    qld_snp_table A, B;
+    int single;     /* one region: B is A, only j > i */
+    int upper;      /* the report prints j > i only (second window unset) */
+} loaded;
+
+static void load_task(const options *o, const task *t, loaded *L)
+{
+    memset(L, 0, sizeof *L);
+    int in2set = t->in2set && strcmp(t->input, t->input2) != 0;
+    const int words1 = (t->h1.samples * o->ploidy + 63) / 64;
+    const int words2 = (t->h2.samples * o->ploidy + 63) / 64;
+    check(qld_mdf_reader_table(g_reader, t->files1, t->nfiles1, t->wmin1, t->wmax1, t->wset1,
+                               words1, o->threads, &L->A), "reading the first region");
+    L->single = !t->wset2 && !in2set;
+    L->upper = !t->wset2;
+    if (in2set)
+        check(qld_mdf_reader_table(g_reader, t->files2, t->nfiles2, t->wmin2, t->wmax2,
+                                   t->wset2, words2, o->threads, &L->B),
+              "reading the second region");
+    else if (t->wset2)
+        check(qld_mdf_reader_table(g_reader, t->files2, t->nfiles2, t->wmin2, t->wmax2,
+                                   t->wset2, words1, o->threads, &L->B),
+              "reading the second region");
+    if (!L->single && L->B.words != L->A.words) {
+        fprintf(stderr, "\n ERROR: the two regions pack %d and %d words per SNP\n\n",
+                L->A.words, L->B.words);
+        exit(1);
+    }
+}
+
+static const qld_snp_table *table_b(const loaded *L) { return L->single ? &L->A : &L->B; }
+
+static void free_loaded(loaded *L)
+{
+    qld_snp_table_free(&L->A);
+    qld_snp_table_free(&L->B);
+}
+
+static void gpu_check(qld_ctx *ctx, int rc, const char *what)
+{
+    if (rc != QLD_OK) {
+        fprintf(stderr, "\n ERROR: %s failed (%d): %s\n\n", what, rc, qld_gpu_last_error(ctx));
+        exit(1);
+    }
+}
+
+/* One task through the drop-in correlate call (correlate_function.c:239-290). */
 static void run_task(qld_ctx *ctx, const options *o, task *t)
 {
     FILE *fp = o->dump ? NULL : fopen(t->output, "wb");
     if (!o->dump && !fp) die("\n ERROR: cannot create output file %s\n\n", t->output);
-    int in2set = t->in2set && strcmp(t->input, t->input2) != 0;
-    const int words1 = (t->h1.samples * o->ploidy + 63) / 64;
-    const int words2 = (t->h2.samples * o->ploidy + 63) / 64;
-    qld_snp_table A, B;
-    memset(&B, 0, sizeof B);
-    check(qld_mdf_read_table(t->files1, t->nfiles1, t->wmin1, t->wmax1, t->wset1, words1,
-                             o->threads, &A), "reading the first region");
-    const int single = !t->wset2 && !in2set;
-    if (in2set)
-        check(qld_mdf_read_table(t->files2, t->nfiles2, t->wmin2, t->wmax2, t->wset2, words2,
-                                 o->threads, &B), "reading the second region");
-    else if (t->wset2)
-        check(qld_mdf_read_table(t->files2, t->nfiles2, t->wmin2, t->wmax2, t->wset2, words1,
-                                 o->threads, &B), "reading the second region");
-    const qld_snp_table *b = single ? &A : &B;
-    if (!single && b->words != A.words) {
-        fprintf(stderr, "\n ERROR: the two regions pack %d and %d words per SNP\n\n",
-                A.words, b->words);
-        exit(1);
-    }
+    loaded L;
+    load_task(o, t, &L);
+    const qld_snp_table *b = table_b(&L);
     if (o->dump) {
-        dump_tables(t, single, &A, b);
-        qld_snp_table_free(&A);
-        qld_snp_table_free(&B);
+        dump_tables(t, L.single, &L.A, b);
+        free_loaded(&L);
         return;
     }
-    int rc = qld_correlate_report(ctx, fp, A.snp, A.pos, A.id, A.bc, A.rows,
-                                  b->snp, b->pos, b->id, b->bc, b->rows, A.words,
-                                  t->h1.snip_size, t->wset2, o->r2limit);
-    if (rc != QLD_OK) {
-        fprintf(stderr, "\n ERROR: GPU correlate failed (%d): %s\n\n", rc,
-                qld_gpu_last_error(ctx));
-        exit(1);
-    }
+    gpu_check(ctx, qld_correlate_report(ctx, fp, L.A.snp, L.A.pos, L.A.id, L.A.bc, L.A.rows,
+                                        b->snp, b->pos, b->id, b->bc, b->rows, L.A.words,
+                                        t->h1.snip_size, t->wset2, o->r2limit),
+              "GPU correlate");
     if (fclose(fp) != 0) die("\n ERROR: write error on %s\n\n", t->output);
-    printf("%s: %d x %d SNPs\n", t->output, A.rows, b->rows);
-    qld_snp_table_free(&A);
-    qld_snp_table_free(&B);
+    printf("%s: %d x %d SNPs\n", t->output, L.A.rows, b->rows);
+    free_loaded(&L);
+}
+
+/* ---- -inputList tasks as one batched launch ------------------------------
+ * The reference runs every list line as an independent correlate() call
+ * (input_list.c:205-323 queues them; correlate_function.c runs one GEMM per
+ * task).  Here the tables of a batch of tasks are stacked into one resident
+ * table, every task becomes one pair window of a persistent plan (one kernel
+ * launch per tile shape for the whole batch), and each task's r^2 block is
+ * compacted on the device before its report is written. */
+
+enum { BATCH_CELLS = 1 << 29 };     /* r^2 cells of one batch (2 GiB) */
+
+static void *xmalloc(size_t n)
+{
+    void *p = malloc(n ? n : 1);
+    if (!p) die("ERROR: out of host memory%s\n", "");
+    return p;
+}
+
+static int64_t task_cells(const loaded *L)
+{
+    return (int64_t)L->A.rows * table_b(L)->rows;
+}
+
+static void run_batch(qld_ctx *ctx, const options *o, task *const *bt, loaded *L, int nb)
+{
+    const int words = L[0].A.words, snp_size = bt[0]->h1.snip_size;
+    int64_t rows = 0, cells = 0, max_rows_b = 0, max_cells = 0;
+    for (int k = 0; k < nb; k++) {
+        rows += L[k].A.rows + (L[k].single ? 0 : L[k].B.rows);
+        cells += task_cells(&L[k]);
+        if (table_b(&L[k])->rows > max_rows_b) max_rows_b = table_b(&L[k])->rows;
+        if (task_cells(&L[k]) > max_cells) max_cells = task_cells(&L[k]);
+    }
+    qld_table *tab = NULL;
+    qld_plan *plan = NULL;
+    void *dout = NULL, *dcnt = NULL, *doff = NULL, *dcols = NULL, *dr2 = NULL;
+    qld_pair_window *w = (qld_pair_window *)xmalloc((size_t)nb * sizeof *w);
+    if (rows > 0) {
+        uint64_t *snp = (uint64_t *)xmalloc((size_t)rows * words * 8);
+        uint32_t *bc = (uint32_t *)xmalloc((size_t)rows * 4);
+        int64_t r = 0, off = 0;
+        for (int k = 0; k < nb; k++) {
+            const qld_snp_table *A = &L[k].A, *B = table_b(&L[k]);
+            memset(&w[k], 0, sizeof w[k]);
+            memcpy(snp + r * words, A->snp, (size_t)A->rows * words * 8);
+            memcpy(bc + r, A->bc, (size_t)A->rows * 4);
+            w[k].a_start = (int32_t)r;
+            w[k].a_size = A->rows;
+            r += A->rows;
+            if (L[k].single) {
+                w[k].b_start = w[k].a_start;
+                w[k].tri = 1;
+            } else {
+                memcpy(snp + r * words, B->snp, (size_t)B->rows * words * 8);
+                memcpy(bc + r, B->bc, (size_t)B->rows * 4);
+                w[k].b_start = (int32_t)r;
+                r += B->rows;
+            }
+            w[k].b_size = B->rows;
+            w[k].out_offset = off;
+            off += task_cells(&L[k]);
+        }
+        gpu_check(ctx, qld_table_create(ctx, snp, bc, (int)rows, words, snp_size, &tab),
+                  "table upload");
+        free(snp);
+        free(bc);
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(cells ? cells : 1) * 4, &dout), "allocation");
+        gpu_check(ctx, qld_plan_pairs(ctx, tab, w, nb, (float *)dout, QLD_BATCH_PERSISTENT,
+                                      &plan), "batch plan");
+        gpu_check(ctx, qld_plan_run(ctx, plan), "batch run");
+        gpu_check(ctx, qld_sync(ctx), "batch run");
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_rows_b + 1) * 4, &dcnt), "allocation");
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_rows_b + 1) * 8, &doff), "allocation");
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_cells ? max_cells : 1) * 4, &dcols),
+                  "allocation");
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_cells ? max_cells : 1) * 4, &dr2),
+                  "allocation");
+    }
+    uint64_t *hoff = (uint64_t *)xmalloc((size_t)(max_rows_b + 1) * 8);
+    uint64_t hcap = 0;
+    uint32_t *hcols = NULL;
+    float *hr2 = NULL;
+    for (int k = 0; k < nb; k++) {
+        const task *t = bt[k];
+        const qld_snp_table *A = &L[k].A, *B = table_b(&L[k]);
+        FILE *fp = fopen(t->output, "wb");
+        if (!fp) die("\n ERROR: cannot create output file %s\n\n", t->output);
+        qld_write_results_header(fp);
+        if (task_cells(&L[k]) > 0) {
+            uint64_t total = 0;
+            gpu_check(ctx, qld_compact_dev(ctx, (float *)dout + w[k].out_offset, A->rows, B->rows,
+                                           A->rows, 0, 0, L[k].upper, o->r2limit,
+                                           (uint32_t *)dcnt, (uint64_t *)doff, (uint32_t *)dcols,
+                                           (float *)dr2, (uint64_t)max_cells, &total),
+                      "report selection");
+            if (total > hcap) {
+                free(hcols);
+                free(hr2);
+                hcap = total + total / 4;
+                hcols = (uint32_t *)xmalloc(hcap * 4);
+                hr2 = (float *)xmalloc(hcap * 4);
+            }
+            gpu_check(ctx, qld_d2h(ctx, hoff, doff, (size_t)(B->rows + 1) * 8), "D2H");
+            if (total) {
+                gpu_check(ctx, qld_d2h(ctx, hcols, dcols, total * 4), "D2H");
+                gpu_check(ctx, qld_d2h(ctx, hr2, dr2, total * 4), "D2H");
+                if (qld_write_hits(fp, A->pos, A->id, A->bc, A->rows, B->pos, B->id, B->bc,
+                                   B->rows, snp_size, 0, B->rows, hoff, hcols, hr2) < 0)
+                    die("\n ERROR: write error on %s\n\n", t->output);
+            }
+        }
+        if (fclose(fp) != 0) die("\n ERROR: write error on %s\n\n", t->output);
+    }
+    free(hoff);
+    free(hcols);
+    free(hr2);
+    free(w);
+    if (plan) qld_plan_free(ctx, plan);
+    if (tab) qld_table_free(ctx, tab);
+    void *bufs[5] = {dout, dcnt, doff, dcols, dr2};
+    for (int k = 0; k < 5; k++)
+        if (bufs[k]) qld_dev_free(ctx, bufs[k]);
+}
+
+/* Tasks in list order; consecutive tasks with the same packing go into one
+ * batch until BATCH_CELLS; a task above a quarter of that runs alone through
+ * the streamed drop-in call. */
+static void run_list(qld_ctx *ctx, const options *o, task *tasks, int ntask)
+{
+    task **bt = (task **)xmalloc((size_t)ntask * sizeof(task *));
+    loaded *L = (loaded *)xmalloc((size_t)ntask * sizeof(loaded));
+    int nb = 0;
+    int64_t cells = 0;
+    double t0 = now();
+    for (int i = 0; i <= ntask; i++) {
+        loaded cur;
+        int flush = i == ntask;
+        if (!flush) {
+            load_task(o, &tasks[i], &cur);
+            if (task_cells(&cur) > BATCH_CELLS / 4) {
+                free_loaded(&cur);
+                if (nb) {
+                    run_batch(ctx, o, bt, L, nb);
+                    for (int k = 0; k < nb; k++) free_loaded(&L[k]);
+                    nb = 0;
+                    cells = 0;
+                }
+                run_task(ctx, o, &tasks[i]);
+                continue;
+            }
+            flush = nb && (cur.A.words != L[0].A.words ||
+                           tasks[i].h1.snip_size != bt[0]->h1.snip_size ||
+                           cells + task_cells(&cur) > BATCH_CELLS);
+        }
+        if (flush && nb) {
+            run_batch(ctx, o, bt, L, nb);
+            for (int k = 0; k < nb; k++) free_loaded(&L[k]);
+            nb = 0;
+            cells = 0;
+        }
+        if (i < ntask) {
+            bt[nb] = &tasks[i];
+            L[nb++] = cur;
+            cells += task_cells(&cur);
+        }
+    }
+    printf("%d tasks in %.3fs\n", ntask, now() - t0);
+    free(bt);
+    free(L);
 }
 
 int main(int argc, char **argv)
 {
     options o;
     parse_args(argc, argv, &o);
+    g_reader = qld_mdf_reader_create();
+    if (!g_reader) die("ERROR: out of host memory%s\n", "");
     if (o.gpu) printf("GPU kernel active\n");
     printf("MDF input active\n");
 
@@ -447,8 +650,12 @@ int main(int argc, char **argv)
         return 1;
     }
     double t0 = now();
+    if (o.listset && !o.dump && !o.plain_tasks) {
+        run_list(ctx, &o, tasks, ntask);
+    } else {
+        for (int i = 0; i < ntask; i++) run_task(ctx, &o, &tasks[i]);
+    }
     for (int i = 0; i < ntask; i++) {
-        run_task(ctx, &o, &tasks[i]);
         qld_mdf_free_files(tasks[i].files1, tasks[i].nfiles1);
         qld_mdf_free_files(tasks[i].files2, tasks[i].nfiles2);
     }
@@ -456,5 +663,6 @@ int main(int argc, char **argv)
     printf("Finished in: %.3fs\n", now() - t0);
     printf("Output saved in: %s\n", dirname(o.output));
     free(tasks);
+    qld_mdf_reader_free(g_reader);
     return 0;
 }

@@ -201,21 +201,43 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
     return e;
 }
 
-extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
-                                int nwin, float *out, int mode, qld_plan **plan)
+// One unit of a plan: rows i of [b_start, b_start + b_size) against columns
+// j of [a_start, a_start + a_size) of the same table; r^2 of (i, j) at
+// out[out_offset + i * a_size + j]; tri: a single region, j > i only.
+struct PairWin {
+    int a_start, a_size, b_start, b_size;
+    int64_t out_offset;
+    int tri;
+};
+
+static int64_t pair_count(const PairWin &w)
 {
-    if (!ctx || !t || !plan || nwin < 0 || (nwin && (!wins || !out)) ||
-        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT)) {
-        if (ctx) qld_set_error(ctx, "qld_plan_windows: bad arguments");
-        return QLD_ERR_INVALID;
-    }
-    *plan = nullptr;
+    return w.tri ? (int64_t)w.a_size * (w.a_size - 1) / 2 : (int64_t)w.a_size * w.b_size;
+}
+
+static bool pair_empty(const PairWin &w)
+{
+    return w.tri ? w.a_size < 2 : (w.a_size == 0 || w.b_size == 0);
+}
+
+static int pair_tile(const qld_ctx *ctx, const PairWin &w)
+{
+    return qld_window_tile(ctx, std::min(w.a_size, w.b_size));
+}
+
+static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWin> &wins,
+                      float *out, int mode, qld_plan **plan)
+{
+    const int nwin = (int)wins.size();
     for (int k = 0; k < nwin; ++k) {
-        const qld_window &w = wins[k];
-        if (w.start < 0 || w.size < 0 || (int64_t)w.start + w.size > t->rows ||
-            w.out_offset < 0 || w.size > 65535 * QLD_TILE_SMALL) {
-            qld_set_error(ctx, "window %d (start %d size %d) outside table of %d rows",
-                          k, w.start, w.size, t->rows);
+        const PairWin &w = wins[k];
+        const int64_t lim = 65535LL * QLD_TILE_SMALL;
+        if (w.a_start < 0 || w.a_size < 0 || w.b_start < 0 || w.b_size < 0 ||
+            (int64_t)w.a_start + w.a_size > t->rows || (int64_t)w.b_start + w.b_size > t->rows ||
+            w.out_offset < 0 || w.a_size > lim || w.b_size > lim ||
+            (w.tri && (w.a_start != w.b_start || w.a_size != w.b_size))) {
+            qld_set_error(ctx, "window %d (A %d+%d, B %d+%d, tri %d) invalid for a table of "
+                          "%d rows", k, w.a_start, w.a_size, w.b_start, w.b_size, w.tri, t->rows);
             return QLD_ERR_INVALID;
         }
     }
@@ -230,15 +252,18 @@ extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_wind
     // work list covers one stretch of the chromosome.
     std::vector<int> order(nwin);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return wins[a].start < wins[b].start; });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return wins[a].b_start != wins[b].b_start ? wins[a].b_start < wins[b].b_start
+                                                  : wins[a].a_start < wins[b].a_start;
+    });
 
-    auto desc_of = [&](const qld_window &w) {
+    auto desc_of = [&](const PairWin &w) {
         WorkDesc d;
-        const uint8_t *X = t->X + (int64_t)w.start * t->row_bytes;
-        const uint32_t *bc = t->bc + w.start;
-        fill_work_desc(d, X, X, bc, bc, w.size, w.size, t->row_bytes, 1, t->snp_size,
-                       out + w.out_offset, w.size, nullptr, w.size);
+        const uint8_t *XI = t->X + (int64_t)w.b_start * t->row_bytes;
+        const uint8_t *XJ = t->X + (int64_t)w.a_start * t->row_bytes;
+        fill_work_desc(d, XI, XJ, t->bc + w.b_start, t->bc + w.a_start, w.b_size, w.a_size,
+                       t->row_bytes, w.tri, t->snp_size, out + w.out_offset, w.a_size, nullptr,
+                       w.a_size);
         return d;
     };
 
@@ -248,14 +273,14 @@ extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_wind
             std::vector<WorkDesc> descs;
             std::vector<WorkItem> items;
             for (int k : order) {
-                const qld_window &w = wins[k];
-                if (w.size < 2 || qld_window_tile(ctx, w.size) != tile) continue;
+                const PairWin &w = wins[k];
+                if (pair_empty(w) || pair_tile(ctx, w) != tile) continue;
                 size_t cnt = items.size();
-                items.resize(cnt + job_tile_count(w.size, w.size, 1, tile));
-                append_job_tiles(items.data(), &cnt, (uint32_t)descs.size(), w.size, w.size,
-                                 1, tile);
+                items.resize(cnt + job_tile_count(w.b_size, w.a_size, w.tri, tile));
+                append_job_tiles(items.data(), &cnt, (uint32_t)descs.size(), w.b_size, w.a_size,
+                                 w.tri, tile);
                 descs.push_back(desc_of(w));
-                p->pairs += (int64_t)w.size * (w.size - 1) / 2;
+                p->pairs += pair_count(w);
             }
             if (items.empty()) continue;
             PlanClass c;
@@ -278,19 +303,19 @@ extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_wind
     } else {
         std::vector<WorkItem> items;
         for (int k : order) {
-            const qld_window &w = wins[k];
-            if (w.size < 2) continue;
-            const int tile = qld_window_tile(ctx, w.size);
+            const PairWin &w = wins[k];
+            if (pair_empty(w)) continue;
+            const int tile = pair_tile(ctx, w);
             PlainLaunch l;
             l.tile = tile;
             l.item_off = (int64_t)items.size();
             size_t cnt = items.size();
-            items.resize(cnt + job_tile_count(w.size, w.size, 1, tile));
-            append_job_tiles(items.data(), &cnt, 0, w.size, w.size, 1, tile);
+            items.resize(cnt + job_tile_count(w.b_size, w.a_size, w.tri, tile));
+            append_job_tiles(items.data(), &cnt, 0, w.b_size, w.a_size, w.tri, tile);
             l.item_cnt = (int64_t)cnt - l.item_off;
             l.desc = desc_of(w);
             p->plain.push_back(l);
-            p->pairs += (int64_t)w.size * (w.size - 1) / 2;
+            p->pairs += pair_count(w);
         }
         p->tiles = (int64_t)items.size();
         p->launches = (int)p->plain.size();
@@ -319,6 +344,38 @@ extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_wind
     }
     *plan = p;
     return QLD_OK;
+}
+
+extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
+                                int nwin, float *out, int mode, qld_plan **plan)
+{
+    if (!ctx || !t || !plan || nwin < 0 || (nwin && (!wins || !out)) ||
+        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT)) {
+        if (ctx) qld_set_error(ctx, "qld_plan_windows: bad arguments");
+        return QLD_ERR_INVALID;
+    }
+    *plan = nullptr;
+    std::vector<PairWin> pw((size_t)nwin);
+    for (int k = 0; k < nwin; ++k)
+        pw[k] = PairWin{wins[k].start, wins[k].size, wins[k].start, wins[k].size,
+                        wins[k].out_offset, 1};
+    return build_plan(ctx, t, pw, out, mode, plan);
+}
+
+extern "C" int qld_plan_pairs(qld_ctx *ctx, const qld_table *t, const qld_pair_window *wins,
+                              int nwin, float *out, int mode, qld_plan **plan)
+{
+    if (!ctx || !t || !plan || nwin < 0 || (nwin && (!wins || !out)) ||
+        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT)) {
+        if (ctx) qld_set_error(ctx, "qld_plan_pairs: bad arguments");
+        return QLD_ERR_INVALID;
+    }
+    *plan = nullptr;
+    std::vector<PairWin> pw((size_t)nwin);
+    for (int k = 0; k < nwin; ++k)
+        pw[k] = PairWin{wins[k].a_start, wins[k].a_size, wins[k].b_start, wins[k].b_size,
+                        wins[k].out_offset, wins[k].tri ? 1 : 0};
+    return build_plan(ctx, t, pw, out, mode, plan);
 }
 
 extern "C" int qld_plan_run(qld_ctx *ctx, qld_plan *p)

@@ -311,170 +311,338 @@ void qld_snp_table_free(qld_snp_table *t)
     memset(t, 0, sizeof *t);
 }
 
+/* One chunk file in a reader: its text (lines NUL-terminated in place), the
+ * reference's view of each data line (the lines after the first two), and
+ * the rows parsed so far.  kind: 0 = short line (the loop sees POS 0),
+ * 1 = a line with POS, 2 = a counted line without a POS field. */
 typedef struct {
+    char *path;
     char *buf;
     size_t len;
-    char **rows;   /* selected row lines (NUL-terminated in buf) */
-    int nrows, cap;
-    int rc;
-} file_scan;
+    int rc;                /* load status */
+    int has_data;          /* a second header line exists: data lines follow */
+    int nlines;
+    char **line;
+    int *pos;
+    unsigned char *kind;
+    /* parsed rows, per data line */
+    int words;             /* words per row of the parse store (0: none yet) */
+    uint64_t *snp;         /* nlines x words */
+    uint32_t *bc, *ppos;
+    char **id;
+    unsigned short *idlen;
+    unsigned char *parsed; /* 0 no, 1 ok, 2 too few fields, 3 wrong word count */
+} mdf_file;
 
-/* Pass 1 for one file: the reference's line loop (correlate_IOx64.c:371-444),
- * recording the lines whose POS lies in [wmin, wmax]. */
-static int scan_file(const char *path, file_scan *fs, int wmin, int wmax, int wset)
+struct qld_mdf_reader {
+    mdf_file *f;
+    int n, cap;
+};
+
+qld_mdf_reader *qld_mdf_reader_create(void)
 {
-    int rc = slurp(path, &fs->buf, &fs->len);
+    return (qld_mdf_reader *)calloc(1, sizeof(qld_mdf_reader));
+}
+
+static void file_free_rows(mdf_file *f)
+{
+    free(f->snp);
+    free(f->bc);
+    free(f->ppos);
+    free(f->id);
+    free(f->idlen);
+    free(f->parsed);
+    f->snp = NULL;
+    f->bc = f->ppos = NULL;
+    f->id = NULL;
+    f->idlen = NULL;
+    f->parsed = NULL;
+    f->words = 0;
+}
+
+void qld_mdf_reader_free(qld_mdf_reader *r)
+{
+    if (!r) return;
+    for (int i = 0; i < r->n; i++) {
+        mdf_file *f = &r->f[i];
+        free(f->path);
+        free(f->buf);
+        free(f->line);
+        free(f->pos);
+        free(f->kind);
+        file_free_rows(f);
+    }
+    free(r->f);
+    free(r);
+}
+
+/* Read and index one file: the reference's reader sees line 1 (must count,
+ * else "file empty"), line 2 (if it does not count, no data line is read),
+ * then the data lines (correlate_IOx64.c:371-444). */
+static int load_file(mdf_file *f)
+{
+    int rc = slurp(f->path, &f->buf, &f->len);
     if (rc) return rc;
-    cursor c = {fs->buf, fs->buf + fs->len};
+    cursor c = {f->buf, f->buf + f->len};
     char *line;
     int eof;
-    if (!next_line(&c, &line, &eof))
-        return fail(QLD_ERR_INVALID, "file %s empty", path);
-    if (!next_line(&c, &line, &eof))
-        return QLD_OK;          /* no second line: the file contributes nothing */
-    int pos;
-    do {
-        if (next_line(&c, &line, &eof)) {
+    if (!next_line(&c, &line, &eof)) return fail(QLD_ERR_INVALID, "file %s empty", f->path);
+    if (eof || !next_line(&c, &line, &eof)) return QLD_OK;
+    f->has_data = 1;
+    int cap = 0;
+    while (!eof) {
+        int st = next_line(&c, &line, &eof);
+        if (!line) break;       /* the read after a final newline */
+        if (f->nlines == cap) {
+            cap = cap ? cap * 2 : 4096;
+            char **nl = (char **)realloc(f->line, (size_t)cap * sizeof(char *));
+            int *np = (int *)realloc(f->pos, (size_t)cap * sizeof(int));
+            unsigned char *nk = (unsigned char *)realloc(f->kind, (size_t)cap);
+            if (nl) f->line = nl;
+            if (np) f->pos = np;
+            if (nk) f->kind = nk;
+            if (!nl || !np || !nk) return fail(QLD_ERR_NOMEM, "out of host memory");
+        }
+        int k = f->nlines++;
+        f->line[k] = line;
+        f->pos[k] = 0;
+        f->kind[k] = 0;
+        if (st) {
             char *p = line;
             size_t wl;
             char *w = next_word(&p, &wl) ? next_word(&p, &wl) : NULL;
-            if (!w)
-                return fail(QLD_ERR_INVALID, "%s: line without CHROM and POS fields", path);
-            pos = atoi_word(w);
-            if (pos >= wmin && pos <= wmax) {
-                if (fs->nrows == fs->cap) {
-                    int nc = fs->cap ? fs->cap * 2 : 1024;
-                    char **nr = (char **)realloc(fs->rows, (size_t)nc * sizeof(char *));
-                    if (!nr) return fail(QLD_ERR_NOMEM, "out of host memory");
-                    fs->rows = nr;
-                    fs->cap = nc;
-                }
-                fs->rows[fs->nrows++] = line;
-            }
-        } else {
-            pos = 0;
+            f->kind[k] = w ? 1 : 2;
+            if (w) f->pos[k] = atoi_word(w);
         }
-    } while (!eof && wset && pos < wmax);
+    }
     return QLD_OK;
 }
 
-/* Pass 2 for one row line: POS, ID, BC, then exactly `words` decimal words. */
-static int parse_row(char *line, int words, uint64_t *snp, uint32_t *pos,
-                     uint32_t *bc, char **id, size_t *idlen)
+static mdf_file *reader_file(qld_mdf_reader *r, const char *path)
 {
-    char *p = line;
-    size_t wl;
-    *pos = 0;
-    char *w = next_word(&p, &wl);                 /* CHROM */
-    char *wp = next_word(&p, &wl);                /* POS */
-    if (!w || !wp) return -1;
-    *pos = (uint32_t)atoi_word(wp);
-    char *wid = next_word(&p, &wl);               /* ID */
-    if (!wid) return -1;
-    *id = wid;
-    *idlen = wl;
-    char *wbc = next_word(&p, &wl);               /* BC */
-    if (!wbc) return -1;
-    *bc = (uint32_t)atoi_word(wbc);
-    int k = 0;
-    while ((w = next_word(&p, &wl)) != NULL) {
-        if (k == words) return -2;
-        snp[k++] = u64_word(w, wl);
+    for (int i = 0; i < r->n; i++)
+        if (!strcmp(r->f[i].path, path)) return &r->f[i];
+    if (r->n == r->cap) {
+        int nc = r->cap ? r->cap * 2 : 64;
+        mdf_file *nf = (mdf_file *)realloc(r->f, (size_t)nc * sizeof(mdf_file));
+        if (!nf) return NULL;
+        r->f = nf;
+        r->cap = nc;
     }
-    return k == words ? 0 : -2;
+    mdf_file *f = &r->f[r->n];
+    memset(f, 0, sizeof *f);
+    f->path = strdup(path);
+    if (!f->path) return NULL;
+    f->rc = -1;             /* not loaded yet */
+    r->n++;
+    return f;
+}
+
+/* Parse data line k of f into its row slot: POS, ID, BC and the packed words
+ * (strtoul per word, as writeMDF_x64 does, with a fast path for plain digit
+ * strings). */
+static void parse_line(mdf_file *f, int k)
+{
+    char *p = f->line[k];
+    size_t wl;
+    const int W = f->words;
+    f->ppos[k] = 0;
+    char *w = next_word(&p, &wl);                 /* CHROM */
+    char *wp = w ? next_word(&p, &wl) : NULL;     /* POS */
+    char *wid = wp ? next_word(&p, &wl) : NULL;   /* ID */
+    size_t idl = wl;
+    char *wbc = wid ? next_word(&p, &wl) : NULL;  /* BC */
+    if (!wbc) {
+        f->parsed[k] = 2;
+        return;
+    }
+    f->ppos[k] = (uint32_t)atoi_word(wp);
+    f->bc[k] = (uint32_t)atoi_word(wbc);
+    uint64_t *row = f->snp + (size_t)k * W;
+    int n = 0;
+    while ((w = next_word(&p, &wl)) != NULL) {
+        if (n == W) {
+            n = W + 1;
+            break;
+        }
+        row[n++] = u64_word(w, wl);
+    }
+    if (n != W) {
+        f->parsed[k] = 3;
+        return;
+    }
+    f->id[k] = wid;
+    f->idlen[k] = (unsigned short)(idl < 65535 ? idl : 65535);
+    f->parsed[k] = 1;
+}
+
+static int ensure_rows(mdf_file *f, int words)
+{
+    if (f->words == words) return QLD_OK;
+    file_free_rows(f);
+    size_t n = f->nlines ? (size_t)f->nlines : 1;
+    f->snp = (uint64_t *)malloc(n * (size_t)words * 8);
+    f->bc = (uint32_t *)malloc(n * 4);
+    f->ppos = (uint32_t *)malloc(n * 4);
+    f->id = (char **)malloc(n * sizeof(char *));
+    f->idlen = (unsigned short *)malloc(n * sizeof(unsigned short));
+    f->parsed = (unsigned char *)calloc(n, 1);
+    if (!f->snp || !f->bc || !f->ppos || !f->id || !f->idlen || !f->parsed) {
+        file_free_rows(f);
+        return fail(QLD_ERR_NOMEM, "out of host memory");
+    }
+    f->words = words;
+    return QLD_OK;
+}
+
+typedef struct {
+    int file, line;
+} row_ref;
+
+int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int pos_wmin,
+                         int pos_wmax, int pos_wset, int words, int nthreads,
+                         qld_snp_table *t)
+{
+    if (!r || !t || nfiles < 0 || words <= 0 || (nfiles && !files))
+        return fail(QLD_ERR_INVALID, "invalid argument");
+    memset(t, 0, sizeof *t);
+    t->words = words;
+    if (nthreads <= 0) nthreads = 1;
+    mdf_file **fs = (mdf_file **)malloc((size_t)(nfiles ? nfiles : 1) * sizeof(mdf_file *));
+    if (!fs) return fail(QLD_ERR_NOMEM, "out of host memory");
+    int rc = QLD_OK;
+    for (int i = 0; i < nfiles && !rc; i++)
+        if (!(fs[i] = reader_file(r, files[i]))) rc = fail(QLD_ERR_NOMEM, "out of host memory");
+    /* load the files this reader has not seen (each listed once) */
+    char errs[1024] = "";
+    if (!rc) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+        for (int i = 0; i < nfiles; i++) {
+            mdf_file *f = fs[i];
+            int first = 1;
+            for (int j = 0; j < i; j++)
+                if (fs[j] == f) first = 0;
+            if (!first || f->rc != -1) continue;
+            f->rc = load_file(f);
+            if (f->rc) {
+#pragma omp critical
+                {
+                    if (!errs[0]) snprintf(errs, sizeof errs, "%s", g_err);
+                }
+            }
+        }
+        for (int i = 0; i < nfiles && !rc; i++)
+            if (fs[i]->rc) {
+                rc = fs[i]->rc;
+                snprintf(g_err, sizeof g_err, "%s", errs[0] ? errs : "load failed");
+            }
+    }
+    /* pass 1: the reference's per-file line loop over the index */
+    row_ref *sel = NULL;
+    long nsel = 0, cap = 0;
+    for (int i = 0; i < nfiles && !rc; i++) {
+        mdf_file *f = fs[i];
+        if (!f->has_data) continue;
+        for (int k = 0; k < f->nlines; k++) {
+            int pos = 0;
+            if (f->kind[k] == 2) {
+                rc = fail(QLD_ERR_INVALID, "%s: line without CHROM and POS fields", f->path);
+                break;
+            }
+            if (f->kind[k] == 1) {
+                pos = f->pos[k];
+                if (pos >= pos_wmin && pos <= pos_wmax) {
+                    if (nsel == cap) {
+                        cap = cap ? cap * 2 : 4096;
+                        row_ref *ns = (row_ref *)realloc(sel, (size_t)cap * sizeof(row_ref));
+                        if (!ns) {
+                            rc = fail(QLD_ERR_NOMEM, "out of host memory");
+                            break;
+                        }
+                        sel = ns;
+                    }
+                    sel[nsel].file = i;
+                    sel[nsel].line = k;
+                    nsel++;
+                }
+            }
+            if (!pos_wset || pos >= pos_wmax) break;
+        }
+    }
+    if (!rc && nsel > INT_MAX) rc = fail(QLD_ERR_INVALID, "too many rows (%ld)", nsel);
+    for (int i = 0; i < nfiles && !rc; i++) rc = ensure_rows(fs[i], words);
+    /* pass 2: parse the selected lines not parsed before (distinct lines) */
+    if (!rc) {
+#pragma omp parallel for schedule(static, 64) num_threads(nthreads)
+        for (long q = 0; q < nsel; q++) {
+            mdf_file *f = fs[sel[q].file];
+            if (!f->parsed[sel[q].line]) {
+                int dup = 0;     /* a file listed twice selects its lines twice */
+                for (int j = 0; j < sel[q].file; j++)
+                    if (fs[j] == f) dup = 1;
+                if (!dup) parse_line(f, sel[q].line);
+            }
+        }
+        for (long q = 0; q < nsel && !rc; q++) {
+            mdf_file *f = fs[sel[q].file];
+            int k = sel[q].line;
+            if (f->parsed[k] == 2)
+                rc = fail(QLD_ERR_INVALID, "%s: row %ld (POS %d): fewer than 4 fields", f->path,
+                          q, f->pos[k]);
+            else if (f->parsed[k] == 3)
+                rc = fail(QLD_ERR_INVALID,
+                          "%s: row %ld (POS %d): the number of packed words differs from the "
+                          "%d the header's samples imply", f->path, q, f->pos[k], words);
+        }
+    }
+    /* assemble */
+    if (!rc) {
+        size_t nr = nsel ? (size_t)nsel : 1;
+        t->rows = (int)nsel;
+        t->snp = (uint64_t *)malloc(nr * (size_t)words * 8);
+        t->pos = (uint32_t *)malloc(nr * 4);
+        t->bc = (uint32_t *)malloc(nr * 4);
+        t->id = (char **)malloc(nr * sizeof(char *));
+        size_t idtot = 0;
+        for (long q = 0; q < nsel; q++) idtot += fs[sel[q].file]->idlen[sel[q].line] + 1u;
+        t->idbuf = (char *)malloc(idtot ? idtot : 1);
+        if (!t->snp || !t->pos || !t->bc || !t->id || !t->idbuf) {
+            rc = fail(QLD_ERR_NOMEM, "out of host memory for %ld rows", nsel);
+        } else {
+            size_t o = 0;
+            for (long q = 0; q < nsel; q++) {
+                const mdf_file *f = fs[sel[q].file];
+                const int k = sel[q].line;
+                t->id[q] = t->idbuf + o;
+                o += f->idlen[k] + 1u;
+            }
+#pragma omp parallel for schedule(static, 256) num_threads(nthreads)
+            for (long q = 0; q < nsel; q++) {
+                const mdf_file *f = fs[sel[q].file];
+                const int k = sel[q].line;
+                memcpy(t->snp + (size_t)q * words, f->snp + (size_t)k * words,
+                       (size_t)words * 8);
+                t->pos[q] = f->ppos[k];
+                t->bc[q] = f->bc[k];
+                memcpy(t->id[q], f->id[k], f->idlen[k]);
+                t->id[q][f->idlen[k]] = '\0';
+            }
+        }
+    }
+    free(sel);
+    free(fs);
+    if (rc) qld_snp_table_free(t);
+    return rc;
 }
 
 int qld_mdf_read_table(char *const *files, int nfiles, int pos_wmin, int pos_wmax,
                        int pos_wset, int words, int nthreads, qld_snp_table *t)
 {
-    if (!t || nfiles < 0 || words <= 0 || (nfiles && !files))
-        return fail(QLD_ERR_INVALID, "invalid argument");
-    memset(t, 0, sizeof *t);
-    t->words = words;
-    file_scan *fs = (file_scan *)calloc((size_t)(nfiles ? nfiles : 1), sizeof(file_scan));
-    if (!fs) return fail(QLD_ERR_NOMEM, "out of host memory");
-    int rc = QLD_OK;
-    char errs[1024] = "";
-    if (nthreads <= 0) nthreads = 1;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-    for (int f = 0; f < nfiles; f++) {
-        fs[f].rc = scan_file(files[f], &fs[f], pos_wmin, pos_wmax, pos_wset);
-        if (fs[f].rc) {
-#pragma omp critical
-            {
-                if (!rc) { rc = fs[f].rc; snprintf(errs, sizeof errs, "%s", g_err); }
-            }
-        }
-    }
-    long total = 0;
-    for (int f = 0; f < nfiles && !rc; f++) total += fs[f].nrows;
-    if (!rc && total > INT_MAX) rc = fail(QLD_ERR_INVALID, "too many rows (%ld)", total);
-    if (!rc) {
-        t->rows = (int)total;
-        size_t nr = total ? (size_t)total : 1;
-        t->snp = (uint64_t *)malloc(nr * (size_t)words * sizeof(uint64_t));
-        t->pos = (uint32_t *)malloc(nr * sizeof(uint32_t));
-        t->bc = (uint32_t *)malloc(nr * sizeof(uint32_t));
-        t->id = (char **)malloc(nr * sizeof(char *));
-        if (!t->snp || !t->pos || !t->bc || !t->id)
-            rc = fail(QLD_ERR_NOMEM, "out of host memory for %ld rows", total);
-    }
-    if (!rc && total) {
-        char **lines = (char **)malloc((size_t)total * sizeof(char *));
-        size_t *idl = (size_t *)malloc((size_t)total * sizeof(size_t));
-        if (!lines || !idl) rc = fail(QLD_ERR_NOMEM, "out of host memory");
-        long r0 = 0;
-        for (int f = 0; f < nfiles && !rc; f++) {
-            memcpy(lines + r0, fs[f].rows, (size_t)fs[f].nrows * sizeof(char *));
-            r0 += fs[f].nrows;
-        }
-        long bad = -1;
-        int badk = 0;
-        if (!rc) {
-#pragma omp parallel for schedule(static, 256) num_threads(nthreads)
-            for (long r = 0; r < total; r++) {
-                int e = parse_row(lines[r], words, t->snp + (size_t)r * words, &t->pos[r],
-                                  &t->bc[r], &t->id[r], &idl[r]);
-                if (e) {
-#pragma omp critical
-                    {
-                        if (bad < 0 || r < bad) { bad = r; badk = e; }
-                    }
-                }
-            }
-            if (bad >= 0)
-                rc = fail(QLD_ERR_INVALID,
-                          badk == -1 ? "row %ld (POS %u): fewer than 4 fields"
-                                     : "row %ld (POS %u): the number of packed words differs "
-                                       "from the %d the header's samples imply",
-                          bad, t->pos[bad], words);
-        }
-        if (!rc) {
-            /* IDs into one arena (the text buffers are released below). */
-            size_t tot = 0;
-            for (long r = 0; r < total; r++) tot += idl[r] + 1;
-            t->idbuf = (char *)malloc(tot ? tot : 1);
-            if (!t->idbuf) rc = fail(QLD_ERR_NOMEM, "out of host memory");
-            size_t o = 0;
-            for (long r = 0; r < total && !rc; r++) {
-                memcpy(t->idbuf + o, t->id[r], idl[r]);
-                t->idbuf[o + idl[r]] = '\0';
-                t->id[r] = t->idbuf + o;
-                o += idl[r] + 1;
-            }
-        }
-        free(lines);
-        free(idl);
-    }
-    for (int f = 0; f < nfiles; f++) {
-        free(fs[f].buf);
-        free(fs[f].rows);
-    }
-    free(fs);
-    if (rc) {
-        if (errs[0]) snprintf(g_err, sizeof g_err, "%s", errs);
-        qld_snp_table_free(t);
-    }
+    qld_mdf_reader *r = qld_mdf_reader_create();
+    if (!r) return fail(QLD_ERR_NOMEM, "out of host memory");
+    int rc = qld_mdf_reader_table(r, files, nfiles, pos_wmin, pos_wmax, pos_wset, words,
+                                  nthreads, t);
+    qld_mdf_reader_free(r);
     return rc;
 }
 

@@ -210,7 +210,7 @@ static int write_hits(void *user, int row0, int rows, const uint64_t *off,
                       const uint32_t *cols, const float *r2)
 {
     hits_writer *w = (hits_writer *)user;
-    if (w->err || off[rows] == 0) return w->err;
+    if (w->err || off[rows] == off[0]) return w->err;
     int r = 0;
     while (r < rows) {
         /* the next group: up to nthreads segments of about SEG_HITS hits */
@@ -248,6 +248,63 @@ static int write_hits(void *user, int row0, int rows, const uint64_t *off,
     return 0;
 }
 
+static int writer_init(hits_writer *w, FILE *fp, const uint32_t *posA, char **idA,
+                       const uint32_t *bcA, int m, const uint32_t *posB, char **idB,
+                       const uint32_t *bcB, int n, int snp_size)
+{
+    memset(w, 0, sizeof *w);
+    w->fp = fp;
+    w->posA = posA;
+    w->bcA = bcA;
+    w->posB = posB;
+    w->bcB = bcB;
+    w->idA = idA;
+    w->idB = idB;
+    w->snp_size = snp_size;
+    w->line_max = LINE_MAX_BYTES + max_id_len(idA, m) + max_id_len(idB, n);
+    w->nthreads = omp_get_max_threads();
+    if (w->nthreads > 64) w->nthreads = 64;
+    if (w->nthreads < 1) w->nthreads = 1;
+    w->fa = (char (*)[32])malloc((size_t)(m > 0 ? m : 1) * 32);
+    w->buf = (char **)calloc((size_t)w->nthreads, sizeof(char *));
+    w->cap = (size_t *)calloc((size_t)w->nthreads, sizeof(size_t));
+    w->len = (size_t *)calloc((size_t)w->nthreads, sizeof(size_t));
+    if (!w->fa || !w->buf || !w->cap || !w->len) return QLD_ERR_NOMEM;
+    /* frequency text of every table-A column, reused by every row */
+#pragma omp parallel for schedule(static) num_threads(w->nthreads)
+    for (int j = 0; j < m; j++)
+        qld_format_float(w->fa[j], 1.0f * (float)bcA[j] / (float)snp_size, 9);
+    return QLD_OK;
+}
+
+static void writer_free(hits_writer *w)
+{
+    for (int k = 0; w->buf && k < w->nthreads; k++) free(w->buf[k]);
+    free(w->buf);
+    free(w->cap);
+    free(w->len);
+    free(w->fa);
+}
+
+long qld_write_hits(FILE *fp,
+                    const uint32_t *posA, char **idA, const uint32_t *bcA, int m,
+                    const uint32_t *posB, char **idB, const uint32_t *bcB, int n,
+                    int snp_size, int row0, int rows, const uint64_t *offsets,
+                    const uint32_t *cols, const float *r2)
+{
+    if (!fp || m < 0 || n < 0 || row0 < 0 || rows < 0 || row0 + rows > n || snp_size <= 0 ||
+        (rows && !offsets))
+        return -1;
+    if (rows == 0 || offsets[rows] == offsets[0]) return 0;
+    hits_writer w;
+    long lines = -1;
+    if (writer_init(&w, fp, posA, idA, bcA, m, posB, idB, bcB, n, snp_size) == QLD_OK &&
+        write_hits(&w, row0, rows, offsets, cols, r2) == 0)
+        lines = w.lines;
+    writer_free(&w);
+    return lines;
+}
+
 int qld_correlate_report(qld_ctx *ctx, FILE *fp,
                          const uint64_t *tableA, const uint32_t *posA, char **idA,
                          const uint32_t *bcA, int m,
@@ -261,39 +318,12 @@ int qld_correlate_report(qld_ctx *ctx, FILE *fp,
     qld_write_results_header(fp);
     if (m == 0 || n == 0) return QLD_OK;
     hits_writer w;
-    memset(&w, 0, sizeof w);
-    w.fp = fp;
-    w.posA = posA;
-    w.bcA = bcA;
-    w.posB = posB;
-    w.bcB = bcB;
-    w.idA = idA;
-    w.idB = idB;
-    w.snp_size = snp_size;
-    w.line_max = LINE_MAX_BYTES + max_id_len(idA, m) + max_id_len(idB, n);
-    w.nthreads = omp_get_max_threads();
-    if (w.nthreads > 64) w.nthreads = 64;
-    if (w.nthreads < 1) w.nthreads = 1;
-    w.fa = (char (*)[32])malloc((size_t)m * 32);
-    w.buf = (char **)calloc((size_t)w.nthreads, sizeof(char *));
-    w.cap = (size_t *)calloc((size_t)w.nthreads, sizeof(size_t));
-    w.len = (size_t *)calloc((size_t)w.nthreads, sizeof(size_t));
-    int rc = QLD_OK;
-    if (!w.fa || !w.buf || !w.cap || !w.len) {
-        rc = QLD_ERR_NOMEM;
-    } else {
-        /* frequency text of every table-A column, reused by every row */
-#pragma omp parallel for schedule(static) num_threads(w.nthreads)
-        for (int j = 0; j < m; j++)
-            qld_format_float(w.fa[j], 1.0f * (float)bcA[j] / (float)snp_size, 9);
+    int rc = writer_init(&w, fp, posA, idA, bcA, m, posB, idB, bcB, n, snp_size);
+    if (rc == QLD_OK) {
         rc = qld_correlate_hits(ctx, tableA, bcA, m, tableB, bcB, n, compressed_snp_size,
                                 snp_size, posWset2 == 0, r2limit, 0, write_hits, &w);
         if (rc != QLD_OK && w.err) rc = QLD_ERR_NOMEM;   /* out of memory or a failed write */
     }
-    for (int k = 0; w.buf && k < w.nthreads; k++) free(w.buf[k]);
-    free(w.buf);
-    free(w.cap);
-    free(w.len);
-    free(w.fa);
+    writer_free(&w);
     return rc;
 }

@@ -225,3 +225,68 @@ def test_cli_reports_equal_reference_on_gpu(engine, tmp_path):
         assert sorted(got) == sorted(exp), name
         for f in exp:
             assert got[f] == exp[f], (name, f, engine)
+
+
+@pytest.mark.gpu
+def test_cli_batched_input_list_equals_single_tasks(tmp_path):
+    """-inputList runs its tasks as one batched plan; every report must equal
+    the report of the same line run as a single task (which the golden cases
+    pin to the reference)."""
+    rng = np.random.default_rng(99)
+    a = qld_amd.mdf_read_header(os.path.join(CLI_DIR, "data", "chrA"))
+    b = qld_amd.mdf_read_header(os.path.join(CLI_DIR, "data", "chrB"))
+    pa = np.arange(a.pos_min, a.pos_max)
+    pb = np.arange(b.pos_min, b.pos_max)
+    lines = []
+    for k in range(40):
+        w1 = np.sort(rng.choice(pa, 2, replace=False))
+        kind = k % 3
+        if kind == 0:
+            lines.append(("data/chrA", w1, "data/chrA", w1))
+        elif kind == 1:
+            lines.append(("data/chrA", w1, "data/chrA", np.sort(rng.choice(pa, 2, replace=False))))
+        else:
+            lines.append(("data/chrA", w1, "data/chrB", np.sort(rng.choice(pb, 2, replace=False))))
+    os.symlink(os.path.join(CLI_DIR, "data"), tmp_path / "data")
+    os.makedirs(tmp_path / "out")
+    with open(tmp_path / "list.csv", "w") as f:
+        for i1, w1, i2, w2 in lines:
+            f.write(f"{i1},{w1[0]},{w1[1]},{i2},{w2[0]},{w2[1]}\n")
+    print("[cli] batched list of", len(lines), "tasks", flush=True)
+    r = subprocess.run([CLI_BIN, "-inputList", "list.csv", "-ploidy", "haploid", "-mdf",
+                        "-r2limit", "0.05", "-output", "out/lst.txt"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    for n, (i1, w1, i2, w2) in enumerate(lines, start=1):
+        print(f"[cli] single task {n}", flush=True)
+        r = subprocess.run([CLI_BIN, "-input", i1, "-posWmin1", str(w1[0]), "-posWmax1",
+                            str(w1[1]), "-input2", i2, "-posWmin2", str(w2[0]), "-posWmax2",
+                            str(w2[1]), "-ploidy", "haploid", "-mdf", "-r2limit", "0.05",
+                            "-output", f"out/single_{n}.txt"], cwd=tmp_path,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        got = open(tmp_path / "out" / f"lst_{n}.txt", "rb").read()
+        want = open(tmp_path / "out" / f"single_{n}.txt", "rb").read()
+        assert got == want, (n, lines[n - 1])
+
+
+def test_mdf_reader_cache_matches_one_shot_reads():
+    """Windows read through one caching reader (files read and rows parsed
+    once) equal independent one-shot reads, in any order, with repeats."""
+    d = os.path.join(CLI_DIR, "data", "chrA")
+    h = qld_amd.mdf_read_header(d)
+    rng = np.random.default_rng(5)
+    reader = qld_amd.MdfReader()
+    for _ in range(25):
+        lo, hi = np.sort(rng.integers(h.pos_min, h.pos_max + 1, 2))
+        files = qld_amd.mdf_find_files(d, "chrA", 25, 96, int(lo), int(hi))
+        for wset in (1, 0):
+            t1 = qld_amd.mdf_read_table(files, int(lo), int(hi), 3, pos_wset=wset)
+            t2 = reader.table(files, int(lo), int(hi), 3, pos_wset=wset)
+            assert (t1["snp"] == t2["snp"]).all() and t1["id"] == t2["id"]
+            assert (t1["pos"] == t2["pos"]).all() and (t1["bc"] == t2["bc"]).all()
+    with pytest.raises(qld_amd.QldError, match="packed words"):
+        reader.table(files, h.pos_min, h.pos_max, 2)     # other packing: re-parsed, rejected
+    t = reader.table(files, int(lo), int(hi), 3)
+    assert len(t["pos"]) == len(qld_amd.mdf_read_table(files, int(lo), int(hi), 3)["pos"])
+    reader.close()

@@ -119,3 +119,68 @@ def test_autotuner_picks_a_candidate(engine_factory):
     plan.free()
     t.free()
     out.free()
+
+
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_i8"])
+@pytest.mark.parametrize("mode", [qld_amd.BATCH_PERSISTENT, qld_amd.BATCH_PLAIN])
+def test_pair_window_plans_mix_triangles_and_rectangles(engine, mode, engine_factory):
+    """qld_plan_pairs: triangles and rectangles (overlapping, disjoint,
+    empty, 1-row) in one plan, each against the oracle."""
+    N, rows = 16622, 3000
+    A, bc = qld_amd.synth_blocks(777, rows, N)
+    rng = np.random.default_rng(mode * 7 + (engine == "mfma_i8"))
+    spec = []
+    for k in range(30):
+        a = int(rng.integers(1, 600))
+        b = int(rng.integers(1, 600))
+        a0 = int(rng.integers(0, rows - a + 1))
+        b0 = int(rng.integers(0, rows - b + 1))
+        spec.append((a0, a, a0, a, 1) if k % 3 == 0 else (a0, a, b0, b, 0))
+    spec += [(10, 0, 20, 5, 0), (100, 1, 200, 1, 0), (50, 1, 50, 1, 1)]
+    w = np.zeros(len(spec), dtype=qld_amd.PAIR_WINDOW_DTYPE)
+    off = 0
+    for k, (a0, a, b0, b, tri) in enumerate(spec):
+        w[k] = (a0, a, b0, b, off, tri, 0)
+        off += a * b
+    e = engine_factory(engine=engine, r2_mode="reference")
+    e.set_tile_cutoff(400)
+    t = e.table(A, bc, N)
+    out = e.alloc(max(off, 1) * 4)
+    plan = e.plan_windows(t, w, out, mode)
+    want_pairs = sum(a * (a - 1) // 2 if tri else a * b for a0, a, b0, b, tri in spec)
+    assert plan.stats()["pairs"] == want_pairs
+    for use_graph in (True, False):
+        out.fill(0x7FC00000)
+        plan.use_graph(use_graph)
+        plan.run()
+        e.sync()
+        got_all = out.download(max(off, 1), np.float32)
+        for k, (a0, a, b0, b, tri) in enumerate(spec):
+            if a == 0 or b == 0 or (tri and a < 2):
+                continue
+            o = int(w[k]["out_offset"])
+            got = got_all[o:o + a * b].reshape(b, a)
+            want = oracle.r2_reference(bc[a0:a0 + a], bc[b0:b0 + b],
+                                       oracle.counts_naive(A[a0:a0 + a], A[b0:b0 + b]), N)
+            sel = np.triu(np.ones((b, a), dtype=bool), 1) if tri else np.ones((b, a), bool)
+            bad = int((got.view(np.uint32)[sel] != want.view(np.uint32)[sel]).sum())
+            say("pair window", k, (a0, a, b0, b, tri), "bad", bad)
+            assert bad == 0
+    plan.free()
+    t.free()
+    out.free()
+
+
+def test_pair_window_validation(engine_factory):
+    e = engine_factory(engine="auto")
+    A, bc = qld_amd.synth_blocks(5, 100, 1000)
+    t = e.table(A, bc, 1000)
+    out = e.alloc(4 * 100 * 100)
+    for bad in [(0, 10, 95, 10, 0), (0, 10, 5, 10, 1), (-1, 5, 0, 5, 0)]:
+        w = np.zeros(1, dtype=qld_amd.PAIR_WINDOW_DTYPE)
+        w[0] = (bad[0], bad[1], bad[2], bad[3], 0, bad[4], 0)
+        with pytest.raises(qld_amd.QldError) as x:
+            e.plan_windows(t, w, out)
+        assert x.value.code == qld_amd.QLD_ERR_INVALID
+    t.free()
+    out.free()
