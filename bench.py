@@ -182,6 +182,8 @@ def rect_bench(args, steps):
     A, bA, B, bB = qld_amd.synth_regions(seed, m, n, N)
     W = A.shape[1]
     eng = qld_amd.Engine(engine=args.engine, r2_mode=args.r2_mode, timing=True)
+    if args.tile:
+        eng.set_plain_tile(args.tile)
     dA = eng.alloc(A.nbytes).upload(A)
     dB = eng.alloc(B.nbytes).upload(B)
     dbA = eng.alloc(bA.nbytes).upload(bA)
@@ -468,6 +470,8 @@ def main():
     ap.add_argument("--no-rect", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-report", action="store_true")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="tile edge of the single-region MFMA launch (0: engine default)")
     ap.add_argument("--no-list", action="store_true")
     ap.add_argument("--list-snps", type=int, default=100000)
     ap.add_argument("--list-windows", type=int, default=1024)
@@ -497,6 +501,8 @@ def main():
     pairs = m * (m - 1) // 2
 
     eng = qld_amd.Engine(engine=args.engine, r2_mode=args.r2_mode, timing=True)
+    if args.tile:
+        eng.set_plain_tile(args.tile)
     dA = eng.alloc(A.nbytes).upload(A)
     dbc = eng.alloc(bc.nbytes).upload(bc)
     dres = eng.alloc(m * m * 4)
@@ -535,6 +541,8 @@ def main():
     # The other r2 mode on the same data and engine, for comparison.
     other = "reference" if args.r2_mode == "exact" else "exact"
     eng2 = qld_amd.Engine(engine=args.engine, r2_mode=other, timing=True)
+    if args.tile:
+        eng2.set_plain_tile(args.tile)
     dA2 = eng2.alloc(A.nbytes).upload(A)
     dbc2 = eng2.alloc(bc.nbytes).upload(bc)
     dres2 = eng2.alloc(m * m * 4)

@@ -27,6 +27,7 @@
 #include "ld_tiles.h"
 #include "ld_mfma.h"
 
+#include <stdlib.h>
 #include <vector>
 
 namespace {
@@ -36,34 +37,40 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int THREADS = 256;     // 4 wave64 in a 2 x 2 arrangement
-constexpr int LDS_RING = 131072; // bytes of LDS for the k-stage ring (256-tiles)
-
-// Geometry of a TILE x TILE workgroup tile whose k stages carry SBY bytes of
-// every row (SBY/32 MFMA k-steps) in a ring of NST LDS buffers.
-template <int TILE, int SBY, int NST>
+// Geometry of a TILE x TILE workgroup tile computed by WGM x WGN wave64s
+// (each a WTM x WTN block of 32 x 32 MFMA tiles) whose k stages carry SBY
+// bytes of every row (SBY/32 MFMA k-steps) in a ring of NST LDS buffers.
+template <int TILE, int SBY, int NST, int WGM, int WGN>
 struct Geo {
+    static constexpr int EDGE = TILE;
     static constexpr int SB = SBY;
     static constexpr int NSTAGE = NST;
+    static constexpr int WAVES_N = WGN;
+    static constexpr int WAVES = WGM * WGN;
+    static constexpr int THREADS = 64 * WAVES;
     static constexpr int KSTEPS = SB / 32;
     static constexpr int CHUNKS = SB / 16;           // 16-byte chunks per LDS row
     static constexpr int ROWS_PER_LOAD = 1024 / SB;  // rows per 1 KiB wave load
-    static constexpr int WT = TILE / 2;              // wave tile edge
-    static constexpr int NB = WT / 32;               // 32 x 32 MFMA blocks per wave edge
-    static constexpr int ROWS_PER_WAVE = TILE / 4;   // rows each wave stages per operand
+    static constexpr int WTM = TILE / WGM;           // wave tile rows (operand A)
+    static constexpr int WTN = TILE / WGN;           // wave tile columns (operand B)
+    static constexpr int NBM = WTM / 32;             // 32 x 32 MFMA blocks per wave
+    static constexpr int NBN = WTN / 32;
+    static constexpr int ROWS_PER_WAVE = TILE / WAVES;   // rows each wave stages per operand
     static constexpr int LOADS = ROWS_PER_WAVE / ROWS_PER_LOAD;
     static constexpr int TILE_BYTES = TILE * SB;     // one operand, one stage
     static constexpr int STAGE_BYTES = 2 * TILE_BYTES;
     static constexpr int LDS_BYTES = NSTAGE * STAGE_BYTES;
     // Epilogue: each wave stages 32 rows of its counts at a time, plus the
-    // r^2 terms of its WT rows.
-    static constexpr int EP_PITCH = WT + 4;          // words; rows land 4 banks apart
+    // r^2 terms of its WTM rows.
+    static constexpr int EP_PITCH = WTN + 4;         // words; rows land 4 banks apart
     static constexpr int EP_WAVE_BYTES = 32 * EP_PITCH * 4;
-    static constexpr int EP_TERMS_OFF = 4 * EP_WAVE_BYTES;
-    static constexpr int LANES_PER_ROW = WT / 4;     // each lane owns 4 columns
+    static constexpr int EP_TERMS_OFF = WAVES * EP_WAVE_BYTES;
+    static constexpr int LANES_PER_ROW = WTN / 4;    // each lane owns 4 columns
     static constexpr int ROWS_PER_IT = 64 / LANES_PER_ROW;
-    static_assert(EP_TERMS_OFF + 4 * WT * 16 <= LDS_BYTES, "epilogue LDS");
+    static constexpr int RPL = WTM / 64 > 0 ? WTM / 64 : 1;   // epilogue rows per lane
+    static_assert(EP_TERMS_OFF + WAVES * WTM * 16 <= LDS_BYTES, "epilogue LDS");
     static_assert(LOADS >= 1 && NSTAGE >= 2, "stage geometry");
+    static_assert(ROWS_PER_IT >= 1 && 32 % ROWS_PER_IT == 0, "epilogue geometry");
 
     // Physical 16-byte chunk of logical chunk c of an LDS row: consecutive
     // rows reading the same logical chunk hit distinct bank groups (8 rows
@@ -74,9 +81,13 @@ struct Geo {
     }
 };
 
-// Kernel configurations: (tile, stage bytes, ring depth).
-#define QLD_CFG_L256 256, 128, 2
-#define QLD_CFG_S128 128, 128, 2
+// Kernel configurations: (tile, stage bytes, ring depth, waves down, waves
+// across).  256-tiles: 8 waves of 128 x 64 (two per SIMD, 0.75 KiB of LDS
+// operand reads per MFMA) over a 4-deep ring of 64-byte stages, or the
+// earlier 4 waves of 128 x 128 (one per SIMD, 2-deep ring of 128 bytes).
+#define QLD_CFG_L256 256, 128, 2, 2, 2
+#define QLD_CFG_L256W8 256, 64, 4, 2, 4
+#define QLD_CFG_S128 128, 128, 2, 2, 2
 
 // v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (scale 2^0): on gfx950
 // it retires twice the flops per cycle of the 16x16x128 form.
@@ -251,15 +262,15 @@ struct Epi<QLD_R2_EXACT> {
 // k loop over expanded operands: a ring of LDS stages filled by DMA.
 template <class OP, class G>
 __device__ __forceinline__ void kloop_dma(const WorkDesc &d, int i0, int j0,
-                                          typename OP::acc_t (&acc)[G::NB][G::NB],
+                                          typename OP::acc_t (&acc)[G::NBM][G::NBN],
                                           uint8_t *smem)
 {
-    constexpr int TILE = 2 * G::WT, SB = G::SB, NSTAGE = G::NSTAGE, KSTEPS = G::KSTEPS;
-    constexpr int NB = G::NB, WT = G::WT;
+    constexpr int TILE = G::EDGE, SB = G::SB, NSTAGE = G::NSTAGE, KSTEPS = G::KSTEPS;
+    constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN;
     auto sA = [&](int b) { return smem + b * G::STAGE_BYTES; };
     auto sB = [&](int b) { return smem + b * G::STAGE_BYTES + G::TILE_BYTES; };
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     const int64_t rb = d.row_bytes;
     const int stages = (int)(rb / SB);
 
@@ -287,27 +298,27 @@ __device__ __forceinline__ void kloop_dma(const WorkDesc &d, int i0, int j0,
             stage_load<G>(rsI, sA(nxt), rb, (s + NSTAGE - 1) * SB, wave, lane);
             stage_load<G>(rsJ, sB(nxt), rb, (s + NSTAGE - 1) * SB, wave, lane);
         }
-        v4i av[KSTEPS][NB], bv[KSTEPS][NB];
+        v4i av[KSTEPS][NBM], bv[KSTEPS][NBN];
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks) {
             const int c = 2 * ks + half;
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                int row = wn * WT + b * 32 + r32;
+            for (int b = 0; b < NBN; ++b) {
+                int row = wn * WTN + b * 32 + r32;
                 bv[ks][b] = *(const v4i *)(sB(cur) + row * SB + (G::swz(row, c) << 4));
             }
 #pragma unroll
-            for (int a = 0; a < NB; ++a) {
-                int row = wm * WT + a * 32 + r32;
+            for (int a = 0; a < NBM; ++a) {
+                int row = wm * WTM + a * 32 + r32;
                 av[ks][a] = *(const v4i *)(sA(cur) + row * SB + (G::swz(row, c) << 4));
             }
         }
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks)
 #pragma unroll
-            for (int a = 0; a < NB; ++a)
+            for (int a = 0; a < NBM; ++a)
 #pragma unroll
-                for (int b = 0; b < NB; ++b)
+                for (int b = 0; b < NBN; ++b)
                     acc[a][b] = OP::mma(av[ks][a], bv[ks][b], acc[a][b]);
     }
 }
@@ -333,15 +344,15 @@ __device__ __forceinline__ v4i fp4_chunk(uint32_t x)
 // read next.  Memory traffic is 1 bit per sample instead of 4.
 template <class G>
 __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
-                                             v16f (&acc)[G::NB][G::NB], uint8_t *smem)
+                                             v16f (&acc)[G::NBM][G::NBN], uint8_t *smem)
 {
-    constexpr int TILE = 2 * G::WT, SB = G::SB, KSTEPS = G::KSTEPS;
-    constexpr int NB = G::NB, WT = G::WT;
-    static_assert(G::NSTAGE == 2 && SB == 128 && 2 * TILE == THREADS, "packed geometry");
+    constexpr int TILE = G::EDGE, SB = G::SB, KSTEPS = G::KSTEPS;
+    constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN;
+    static_assert(G::NSTAGE == 2 && SB == 128 && 2 * TILE == G::THREADS, "packed geometry");
     auto sA = [&](int b) { return smem + b * G::STAGE_BYTES; };
     auto sB = [&](int b) { return smem + b * G::STAGE_BYTES + G::TILE_BYTES; };
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     const int W = d.words;
     const int stages = (W + 3) / 4;
 
@@ -370,27 +381,27 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 
     const int r32 = lane & 31, half = lane >> 5;
     auto compute = [&](int buf) {
-        v4i av[KSTEPS][NB], bv[KSTEPS][NB];
+        v4i av[KSTEPS][NBM], bv[KSTEPS][NBN];
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks) {
             const int c = 2 * ks + half;
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                int row = wn * WT + b * 32 + r32;
+            for (int b = 0; b < NBN; ++b) {
+                int row = wn * WTN + b * 32 + r32;
                 bv[ks][b] = *(const v4i *)(sB(buf) + row * SB + (G::swz(row, c) << 4));
             }
 #pragma unroll
-            for (int a = 0; a < NB; ++a) {
-                int row = wm * WT + a * 32 + r32;
+            for (int a = 0; a < NBM; ++a) {
+                int row = wm * WTM + a * 32 + r32;
                 av[ks][a] = *(const v4i *)(sA(buf) + row * SB + (G::swz(row, c) << 4));
             }
         }
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks)
 #pragma unroll
-            for (int a = 0; a < NB; ++a)
+            for (int a = 0; a < NBM; ++a)
 #pragma unroll
-                for (int b = 0; b < NB; ++b)
+                for (int b = 0; b < NBN; ++b)
                     acc[a][b] = FP4::mma(av[ks][a], bv[ks][b], acc[a][b]);
     };
 
@@ -430,37 +441,36 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 // one 16-byte store (rows of WT floats coalesce).
 template <class OP, int MODE, class G>
 __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
-                                              typename OP::acc_t (&acc)[G::NB][G::NB],
+                                              typename OP::acc_t (&acc)[G::NBM][G::NBN],
                                               const uint32_t *bc_row, const uint32_t *bc_col,
                                               uint8_t *smem)
 {
     typedef Epi<MODE> E;
-    constexpr int NB = G::NB, WT = G::WT;
-    constexpr int RPL = WT / 64 > 0 ? WT / 64 : 1;
+    constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, RPL = G::RPL;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     const int r32 = lane & 31, half = lane >> 5;
     const int cq = lane % G::LANES_PER_ROW, rq = lane / G::LANES_PER_ROW;
-    const int jc = j0 + wn * WT + 4 * cq;               // first of this lane's 4 columns
+    const int jc = j0 + wn * WTN + 4 * cq;              // first of this lane's 4 columns
     uint32_t *ep = (uint32_t *)(smem + wave * G::EP_WAVE_BYTES);
-    uint4 *terms = (uint4 *)(smem + G::EP_TERMS_OFF) + wave * WT;
+    uint4 *terms = (uint4 *)(smem + G::EP_TERMS_OFF) + wave * WTM;
 #pragma unroll
     for (int q = 0; q < RPL; ++q)
-        if (lane + q * 64 < WT) terms[q * 64 + lane] = E::pack(E::make(bc_row[q], d));
+        if (lane + q * 64 < WTM) terms[q * 64 + lane] = E::pack(E::make(bc_row[q], d));
     typename E::term col[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) col[q] = E::make(bc_col[q], d);
     const bool vec_ok = d.res && (d.ldr & 3) == 0 && (((uintptr_t)d.res) & 15) == 0;
 #pragma unroll
-    for (int a = 0; a < NB; ++a) {
+    for (int a = 0; a < NBM; ++a) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+        for (int b = 0; b < NBN; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 int row = 8 * (r >> 2) + 4 * half + (r & 3);
                 ep[row * G::EP_PITCH + b * 32 + r32] = OP::bits(acc[a][b][r]);
             }
-        const int ibase = i0 + wm * WT + a * 32;
+        const int ibase = i0 + wm * WTM + a * 32;
 #pragma unroll 2
         for (int it = 0; it < 32 / G::ROWS_PER_IT; ++it) {
             const int rl = it * G::ROWS_PER_IT + rq;
@@ -499,30 +509,30 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
 template <class OP, int MODE, class G, int PACKED>
 __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uint8_t *smem)
 {
-    constexpr int NB = G::NB, WT = G::WT, TILE = 2 * G::WT;
-    constexpr int RPL = WT / 64 > 0 ? WT / 64 : 1;      // epilogue rows per lane
+    constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, TILE = G::EDGE;
+    constexpr int RPL = G::RPL;                         // epilogue rows per lane
     const int i0 = ti * TILE, j0 = tj * TILE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
 
     // Bit counts of this lane's epilogue rows/columns, fetched now so their
     // latency hides under the k loop.
     uint32_t bc_row[RPL];
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
-        int i = i0 + wm * WT + q * 64 + lane;
-        bc_row[q] = (lane + q * 64 < WT && i < d.ni) ? d.bcI[i] : 0u;
+        int i = i0 + wm * WTM + q * 64 + lane;
+        bc_row[q] = (lane + q * 64 < WTM && i < d.ni) ? d.bcI[i] : 0u;
     }
-    const int jc = j0 + wn * WT + 4 * (lane % G::LANES_PER_ROW);
+    const int jc = j0 + wn * WTN + 4 * (lane % G::LANES_PER_ROW);
     uint32_t bc_col[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? d.bcJ[jc + q] : 0u;
 
-    typename OP::acc_t acc[NB][NB];
+    typename OP::acc_t acc[NBM][NBN];
 #pragma unroll
-    for (int a = 0; a < NB; ++a)
+    for (int a = 0; a < NBM; ++a)
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+        for (int b = 0; b < NBN; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
 
@@ -550,7 +560,7 @@ __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, uint8_t *s
 // list, then steals from the other queues.  Every claim advances a head, so
 // the loop ends once all eight ranges are exhausted.
 template <class OP, int MODE, class G, int PERSIST, int PACKED>
-__global__ __launch_bounds__(THREADS, 1) void ld_mfma_kernel(KArgs p)
+__global__ __launch_bounds__(G::THREADS, 1) void ld_mfma_kernel(KArgs p)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     if (!PERSIST) {
@@ -766,7 +776,12 @@ static hipError_t set_lds_limit()
 }
 
 typedef Geo<QLD_CFG_L256> GeoL;
+typedef Geo<QLD_CFG_L256W8> GeoL8;
 typedef Geo<QLD_CFG_S128> GeoS;
+
+// 256-tile variant: 8 waves (default) or 4 (QLD_TILE256_WAVES=4 in the
+// environment), read once when the first context is created.
+static int g_l256_waves = 8;
 
 template <class OP, int MODE>
 static hipError_t set_lds_limits_mode()
@@ -774,6 +789,8 @@ static hipError_t set_lds_limits_mode()
     hipError_t e;
     if ((e = set_lds_limit<OP, MODE, GeoL, 0>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoL, 1>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, GeoL8, 0>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, GeoL8, 1>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoS, 0>()) != hipSuccess) return e;
     return set_lds_limit<OP, MODE, GeoS, 1>();
 }
@@ -790,6 +807,8 @@ static hipError_t set_lds_limits_packed()
 // process, before any launch -- and so before any graph capture).
 hipError_t mfma_init_attributes()
 {
+    const char *w = getenv("QLD_TILE256_WAVES");
+    if (w && atoi(w) == 4) g_l256_waves = 4;
     hipError_t e;
     if ((e = set_lds_limits_mode<FP4, QLD_R2_REFERENCE>()) != hipSuccess) return e;
     if ((e = set_lds_limits_mode<FP4, QLD_R2_EXACT>()) != hipSuccess) return e;
@@ -803,7 +822,7 @@ template <class OP, int MODE, class G, int PERSIST, int PACKED = 0>
 static hipError_t launch_one(const KArgs &a, unsigned grid, hipStream_t s)
 {
     hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE, G, PERSIST, PACKED>), dim3(grid),
-                       dim3(THREADS), (size_t)G::LDS_BYTES, s, a);
+                       dim3(G::THREADS), (size_t)G::LDS_BYTES, s, a);
     return hipGetLastError();
 }
 
@@ -819,6 +838,7 @@ template <class OP, int MODE>
 static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStream_t s)
 {
     if (tile == QLD_TILE_SMALL) return launch_mode<OP, MODE, GeoS>(a, persistent, s);
+    if (g_l256_waves == 8) return launch_mode<OP, MODE, GeoL8>(a, persistent, s);
     return launch_mode<OP, MODE, GeoL>(a, persistent, s);
 }
 
