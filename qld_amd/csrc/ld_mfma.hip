@@ -62,7 +62,7 @@ struct Geo {
     static constexpr int LDS_BYTES = NSTAGE * STAGE_BYTES;
     // Epilogue: each wave stages 32 rows of its counts at a time, plus the
     // r^2 terms of its WTM rows.
-    static constexpr int EP_PITCH = WTN + 4;         // words; rows land 4 banks apart
+    static constexpr int EP_PITCH = WTN + 8;         // words: rows r and r+4 land 32 banks apart
     static constexpr int EP_WAVE_BYTES = 32 * EP_PITCH * 4;
     static constexpr int EP_TERMS_OFF = WAVES * EP_WAVE_BYTES;
     static constexpr int LANES_PER_ROW = WTN / 4;    // each lane owns 4 columns
@@ -72,12 +72,15 @@ struct Geo {
     static_assert(LOADS >= 1 && NSTAGE >= 2, "stage geometry");
     static_assert(ROWS_PER_IT >= 1 && 32 % ROWS_PER_IT == 0, "epilogue geometry");
 
-    // Physical 16-byte chunk of logical chunk c of an LDS row: consecutive
-    // rows reading the same logical chunk hit distinct bank groups (8 rows
-    // per 128 bytes of banks).  Its own inverse.
+    // Physical 16-byte chunk of logical chunk c of an LDS row.  The
+    // ds_read_b128 of an MFMA operand fetch reads one logical chunk of 16
+    // consecutive rows per 64-bank pass (8 rows per 32-bank pass); the row
+    // function spreads every such group over distinct 16-byte bank groups
+    // (checked exhaustively for both pass shapes).  Its own inverse.
     static __device__ __forceinline__ int swz(int row, int c)
     {
-        return CHUNKS == 8 ? (c ^ (row & 7)) : (c ^ ((row >> 1) & 3));
+        if (CHUNKS == 8) return c ^ ((row ^ (row >> 3)) & 7);
+        return c ^ (((row >> 2) & 1) | ((((row >> 1) ^ (row >> 3)) & 1) << 1));
     }
 };
 
@@ -88,6 +91,7 @@ struct Geo {
 #define QLD_CFG_L256 256, 128, 2, 2, 2
 #define QLD_CFG_L256W8 256, 64, 4, 2, 4
 #define QLD_CFG_S128 128, 128, 2, 2, 2
+#define QLD_CFG_S128R4 128, 64, 4, 2, 2
 
 // v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (scale 2^0): on gfx950
 // it retires twice the flops per cycle of the 16x16x128 form.
@@ -139,18 +143,18 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t panel_rsrc(const uint8_t *base
 }
 
 // Queue one k stage of one operand: wave w fills tile rows
-// [w*R, w*R + R), R = TILE/4.  A wave load covers 1 KiB = ROWS_PER_LOAD rows
-// x CHUNKS chunks; lane L lands at row L/CHUNKS, physical chunk L%CHUNKS,
-// which holds logical chunk swz(row, L%CHUNKS).
+// [w*R, w*R + R), R = TILE/WAVES.  A wave load covers 1 KiB = ROWS_PER_LOAD
+// rows x CHUNKS chunks; lane L lands at row L/CHUNKS of the load, physical
+// chunk L%CHUNKS, which holds logical chunk swz(row, L%CHUNKS).
 template <class G>
 __device__ __forceinline__ void stage_load(__amdgpu_buffer_rsrc_t rs, uint8_t *lds_tile,
                                            int64_t row_bytes, int k0, int wave, int lane)
 {
     const int r_in = lane / G::CHUNKS, pc = lane % G::CHUNKS;
-    const int c = G::swz(r_in, pc);
 #pragma unroll
     for (int q = 0; q < G::LOADS; ++q) {
         int row = wave * G::ROWS_PER_WAVE + q * G::ROWS_PER_LOAD + r_in;
+        const int c = G::swz(row, pc);
         uint32_t voff = (uint32_t)((int64_t)row * row_bytes + k0 + c * 16);
         uint8_t *dst = lds_tile + (wave * G::ROWS_PER_WAVE + q * G::ROWS_PER_LOAD) * G::SB;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)dst, 16, voff, 0, 0, 0);
@@ -778,10 +782,14 @@ static hipError_t set_lds_limit()
 typedef Geo<QLD_CFG_L256> GeoL;
 typedef Geo<QLD_CFG_L256W8> GeoL8;
 typedef Geo<QLD_CFG_S128> GeoS;
+typedef Geo<QLD_CFG_S128R4> GeoS4;
 
 // 256-tile variant: 8 waves (default) or 4 (QLD_TILE256_WAVES=4 in the
 // environment), read once when the first context is created.
 static int g_l256_waves = 8;
+// 128-tile ring: two 128-byte stages (default) or four 64-byte stages
+// (QLD_TILE128_RING=4).
+static int g_s128_ring = 2;
 
 template <class OP, int MODE>
 static hipError_t set_lds_limits_mode()
@@ -791,6 +799,8 @@ static hipError_t set_lds_limits_mode()
     if ((e = set_lds_limit<OP, MODE, GeoL, 1>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoL8, 0>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoL8, 1>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, GeoS4, 0>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<OP, MODE, GeoS4, 1>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoS, 0>()) != hipSuccess) return e;
     return set_lds_limit<OP, MODE, GeoS, 1>();
 }
@@ -809,6 +819,8 @@ hipError_t mfma_init_attributes()
 {
     const char *w = getenv("QLD_TILE256_WAVES");
     if (w && atoi(w) == 4) g_l256_waves = 4;
+    const char *r = getenv("QLD_TILE128_RING");
+    if (r && atoi(r) == 4) g_s128_ring = 4;
     hipError_t e;
     if ((e = set_lds_limits_mode<FP4, QLD_R2_REFERENCE>()) != hipSuccess) return e;
     if ((e = set_lds_limits_mode<FP4, QLD_R2_EXACT>()) != hipSuccess) return e;
@@ -837,7 +849,9 @@ static hipError_t launch_mode(const KArgs &a, int persistent, hipStream_t s)
 template <class OP, int MODE>
 static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStream_t s)
 {
-    if (tile == QLD_TILE_SMALL) return launch_mode<OP, MODE, GeoS>(a, persistent, s);
+    if (tile == QLD_TILE_SMALL)
+        return g_s128_ring == 4 ? launch_mode<OP, MODE, GeoS4>(a, persistent, s)
+                                : launch_mode<OP, MODE, GeoS>(a, persistent, s);
     if (g_l256_waves == 8) return launch_mode<OP, MODE, GeoL8>(a, persistent, s);
     return launch_mode<OP, MODE, GeoL>(a, persistent, s);
 }
