@@ -44,11 +44,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(engine, r2_mode, snps, samples):
+def measured_traffic(engine, r2_mode, snps, samples, operands="direct"):
     """HBM bytes per launch of the contraction kernel for this workload, read
     from the committed rocprofv3 PMC summary (FETCH_SIZE x2 + WRITE_SIZE x1,
     scaled by a copy kernel of known size); None when no pass covers it."""
-    key = f"engine={engine} mode={r2_mode} snps={snps} samples={samples} "
+    key = (f"engine={engine} operands={operands} mode={r2_mode} snps={snps} "
+           f"samples={samples} ")
     try:
         for path in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
             with open(os.path.join(ROOT, "profiles", path)) as f:
@@ -83,7 +84,7 @@ def roofline(engine, pairs, words, samples, kernel_s, traffic=None, rows=None):
         if rows:
             # each operand row read once (fp4: 4 bits per sample, padded to the
             # 64-sample word) + 4 B of r2 written per pair
-            opb = {"mfma_fp4": 32, "mfma_i8": 64}.get(engine, 8)
+            opb = {"mfma_i8": 64}.get(engine, 8)   # fp4 reads the packed rows
             out["algorithmic_bytes"] = rows * words * opb + pairs * 4
     return out
 
@@ -184,6 +185,8 @@ def rect_bench(args, steps):
     eng = qld_amd.Engine(engine=args.engine, r2_mode=args.r2_mode, timing=True)
     if args.tile:
         eng.set_plain_tile(args.tile)
+    if args.fp4_operands:
+        eng.set_fp4_operands(args.fp4_operands)
     dA = eng.alloc(A.nbytes).upload(A)
     dB = eng.alloc(B.nbytes).upload(B)
     dbA = eng.alloc(bA.nbytes).upload(bA)
@@ -470,6 +473,8 @@ def main():
     ap.add_argument("--no-rect", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-report", action="store_true")
+    ap.add_argument("--fp4-operands", default="",
+                    help="fp4 operand source: expanded | packed | direct (engine default if empty)")
     ap.add_argument("--tile", type=int, default=0,
                     help="tile edge of the single-region MFMA launch (0: engine default)")
     ap.add_argument("--no-list", action="store_true")
@@ -503,6 +508,8 @@ def main():
     eng = qld_amd.Engine(engine=args.engine, r2_mode=args.r2_mode, timing=True)
     if args.tile:
         eng.set_plain_tile(args.tile)
+    if args.fp4_operands:
+        eng.set_fp4_operands(args.fp4_operands)
     dA = eng.alloc(A.nbytes).upload(A)
     dbc = eng.alloc(bc.nbytes).upload(bc)
     dres = eng.alloc(m * m * 4)
@@ -543,6 +550,8 @@ def main():
     eng2 = qld_amd.Engine(engine=args.engine, r2_mode=other, timing=True)
     if args.tile:
         eng2.set_plain_tile(args.tile)
+    if args.fp4_operands:
+        eng2.set_fp4_operands(args.fp4_operands)
     dA2 = eng2.alloc(A.nbytes).upload(A)
     dbc2 = eng2.alloc(bc.nbytes).upload(bc)
     dres2 = eng2.alloc(m * m * 4)
@@ -571,7 +580,8 @@ def main():
         f"{other}_mode": {"ms_per_step": round(other_ms, 4),
                           "gpairs_s": round(pairs / other_ms / 1e6, 3)},
         "roofline": roofline(engine, pairs, W, N, kernel_s,
-                             measured_traffic(engine, args.r2_mode, m, N), rows=m),
+                             measured_traffic(engine, args.r2_mode, m, N,
+                                              args.fp4_operands or "direct"), rows=m),
     }
     if args.copy_probe:
         nb = args.copy_probe << 20

@@ -132,17 +132,21 @@ int qld_last_contract_ms(qld_ctx *ctx, float *ms);
 int qld_timer_start(qld_ctx *ctx);
 int qld_timer_stop(qld_ctx *ctx, float *ms);
 
-/* Operand source of the fp4 engine for plain calls: EXPANDED (default)
- * first writes an fp4 copy of the tables (4 bits per sample) that the
- * kernel streams into LDS by DMA; PACKED reads the packed rows and expands
- * bits to e2m1 inside the kernel (1 bit of memory traffic per sample, no
- * workspace; measured slower: 4.9 vs 3.5 ms on 30,000 x 16,622). */
+/* Operand source of the fp4 engine (plain calls, and tables created after
+ * the call): DIRECT (default) reads the packed rows straight into registers
+ * and expands bits to e2m1 there (no LDS ring, no workspace, no expanded
+ * copy, 1 bit of memory traffic per sample); EXPANDED first writes an fp4
+ * copy of the tables (4 bits per sample) that the kernel streams into LDS by
+ * DMA; PACKED reads the packed rows into LDS and expands them there
+ * (128-tiles). */
+#define QLD_FP4_DIRECT   2
 #define QLD_FP4_PACKED   1
 #define QLD_FP4_EXPANDED 0
 int qld_set_fp4_operands(qld_ctx *ctx, int mode);
 
-/* Tile edge of the MFMA engines for plain calls with expanded operands:
- * 256, 128, or 0 for the default (128, measured fastest on MI355X). */
+/* Tile edge of the MFMA engines for plain calls: 256, 128, or 0 for the
+ * default (direct fp4: 256 when both sides reach the tile cutoff, else 128;
+ * expanded operands: 128). */
 int qld_set_plain_tile(qld_ctx *ctx, int tile);
 
 /* Engine the last call used (QLD_ENGINE_*), after AUTO resolution. */

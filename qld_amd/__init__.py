@@ -539,8 +539,11 @@ class Engine:
         PAIR_WINDOW_DTYPE (triangles and rectangles, qld_plan_pairs)."""
         return Plan(self, table, windows, out, mode)
 
-    def set_fp4_operands(self, packed: bool):
-        self._check(hip_lib().qld_set_fp4_operands(self._ctx, 1 if packed else 0))
+    def set_fp4_operands(self, mode):
+        """True / "packed" (1), False / "expanded" (0), or "direct" (2)."""
+        m = {"expanded": 0, "packed": 1, "direct": 2}.get(mode, None) if isinstance(mode, str) \
+            else (int(mode) if not isinstance(mode, bool) else (1 if mode else 0))
+        self._check(hip_lib().qld_set_fp4_operands(self._ctx, m))
 
     def set_plain_tile(self, tile: int):
         self._check(hip_lib().qld_set_plain_tile(self._ctx, int(tile)))

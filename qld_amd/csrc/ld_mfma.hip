@@ -68,7 +68,8 @@ struct Geo {
     static constexpr int LANES_PER_ROW = WTN / 4;    // each lane owns 4 columns
     static constexpr int ROWS_PER_IT = 64 / LANES_PER_ROW;
     static constexpr int RPL = WTM / 64 > 0 ? WTM / 64 : 1;   // epilogue rows per lane
-    static_assert(EP_TERMS_OFF + WAVES * WTM * 16 <= LDS_BYTES, "epilogue LDS");
+    static constexpr int EP_BYTES = EP_TERMS_OFF + WAVES * WTM * 16;
+    static_assert(EP_BYTES <= LDS_BYTES, "epilogue LDS");
     static_assert(LOADS >= 1 && NSTAGE >= 2, "stage geometry");
     static_assert(ROWS_PER_IT >= 1 && 32 % ROWS_PER_IT == 0, "epilogue geometry");
 
@@ -159,6 +160,31 @@ __device__ __forceinline__ void stage_load(__amdgpu_buffer_rsrc_t rs, uint8_t *l
         uint8_t *dst = lds_tile + (wave * G::ROWS_PER_WAVE + q * G::ROWS_PER_LOAD) * G::SB;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)dst, 16, voff, 0, 0, 0);
     }
+}
+
+// The wave's index in the workgroup as a scalar: buffer descriptors, LDS
+// DMA destinations and tile origins derived from it stay in SGPRs (a value
+// the compiler cannot prove wave-uniform costs a readfirstlane loop around
+// every buffer load).
+__device__ __forceinline__ int wave_id()
+{
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+// A value loaded identically by every lane (a work item, a job descriptor),
+// moved to SGPRs dword by dword.  Without it the compiler keeps such loads in
+// VGPRs (it cannot prove the memory unclobbered) and everything derived from
+// them -- buffer descriptors above all -- needs a readfirstlane loop per use.
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T &v)
+{
+    static_assert(sizeof(T) % 4 == 0, "dword-sized");
+    T out;
+    const int *src = reinterpret_cast<const int *>(&v);
+    int *dst = reinterpret_cast<int *>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return out;
 }
 
 template <int N>
@@ -273,7 +299,7 @@ __device__ __forceinline__ void kloop_dma(const WorkDesc &d, int i0, int j0,
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN;
     auto sA = [&](int b) { return smem + b * G::STAGE_BYTES; };
     auto sB = [&](int b) { return smem + b * G::STAGE_BYTES + G::TILE_BYTES; };
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     const int64_t rb = d.row_bytes;
     const int stages = (int)(rb / SB);
@@ -355,7 +381,7 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
     static_assert(G::NSTAGE == 2 && SB == 128 && 2 * TILE == G::THREADS, "packed geometry");
     auto sA = [&](int b) { return smem + b * G::STAGE_BYTES; };
     auto sB = [&](int b) { return smem + b * G::STAGE_BYTES + G::TILE_BYTES; };
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     const int W = d.words;
     const int stages = (W + 3) / 4;
@@ -437,6 +463,78 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
     if (s + 1 < stages) step(s + 1, R2);
 }
 
+// k loop straight from the packed rows (fp4, no LDS, no barrier).  For each
+// group of 4 k-steps (4 packed words = 256 samples of a row) lane l of a
+// 32 x 32 operand fetch loads 16 bytes -- words 2h, 2h+1 of the group, h =
+// l/32 -- of row l%32 with one buffer_load_dwordx4, and k-step ks uses
+// dword ks of them, expanded to e2m1 nibbles in registers (fp4_chunk).  Both
+// operands use the same sample -> K mapping, so the products pair samples
+// correctly.  The next group's words are in flight while this group's MFMAs
+// run.  Rows past the job end read 0 through the buffer bound; words past
+// the row end (W not a multiple of 4) are zeroed.
+template <class G>
+__device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0,
+                                           v16f (&acc)[G::NBM][G::NBN])
+{
+    constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
+    const int r32 = lane & 31, h = lane >> 5;
+    const int W = d.words;
+    const int64_t rowb = (int64_t)W * 8;
+    const int a0 = i0 + wm * WTM, b0 = j0 + wn * WTN;
+    auto rsrc = [&](const uint64_t *T, int r0, int n, int rows) {
+        int64_t have = (int64_t)(n - r0);
+        have = have < 0 ? 0 : (have > rows ? rows : have);
+        const uint64_t base = wave_uniform((uint64_t)(uintptr_t)(T + (int64_t)r0 * W));
+        return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)base, 0,
+                                                 wave_uniform((int)(have * rowb)), 0x00020000);
+    };
+    const __amdgpu_buffer_rsrc_t rsA = rsrc(d.PI, a0, d.ni, WTM);
+    const __amdgpu_buffer_rsrc_t rsB = rsrc(d.PJ, b0, d.nj, WTN);
+    const uint32_t offA = (uint32_t)(r32 * rowb + 16 * h);
+    const uint32_t offB = offA;
+    const uint32_t stepA = (uint32_t)(32 * rowb);      // next 32-row block
+    const int groups = (W + 3) / 4;
+
+    v4i pa[NBM], pb[NBN], qa[NBM], qb[NBN];
+    auto fetch = [&](v4i (&xa)[NBM], v4i (&xb)[NBN], int g) {
+        const uint32_t k = (uint32_t)g * 32u;
+#pragma unroll
+        for (int a = 0; a < NBM; ++a)
+            xa[a] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsA, offA + a * stepA + k, 0, 0);
+#pragma unroll
+        for (int b = 0; b < NBN; ++b)
+            xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB + b * stepA + k, 0, 0);
+    };
+    auto compute = [&](const v4i (&xa)[NBM], const v4i (&xb)[NBN], int g) {
+        // words 4g + 2h and 4g + 2h + 1 of the row exist?
+        const bool w0 = 4 * g + 2 * h < W, w1 = 4 * g + 2 * h + 1 < W;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const bool ok = (ks < 2) ? w0 : w1;
+            v4i bv[NBN];
+#pragma unroll
+            for (int b = 0; b < NBN; ++b) bv[b] = fp4_chunk(ok ? (uint32_t)xb[b][ks] : 0u);
+#pragma unroll
+            for (int a = 0; a < NBM; ++a) {
+                const v4i av = fp4_chunk(ok ? (uint32_t)xa[a][ks] : 0u);
+#pragma unroll
+                for (int b = 0; b < NBN; ++b) acc[a][b] = FP4::mma(av, bv[b], acc[a][b]);
+            }
+        }
+    };
+    fetch(pa, pb, 0);
+    int g = 0;
+    for (; g + 2 <= groups; g += 2) {
+        fetch(qa, qb, g + 1);
+        compute(pa, pb, g);
+        if (g + 2 < groups) fetch(pa, pb, g + 2);
+        compute(qa, qb, g + 1);
+    }
+    if (g < groups) compute(pa, pb, g);
+}
+
 // Fused epilogue.  Block (a, b) register r holds the pair
 //   row wm*WT + a*32 + 8*(r/4) + 4*(lane/32) + r%4, col wn*WT + b*32 + lane%32
 // of the tile.  The r^2 terms of the wave's WT rows go to LDS once; then
@@ -451,7 +549,7 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
 {
     typedef Epi<MODE> E;
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, RPL = G::RPL;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     const int r32 = lane & 31, half = lane >> 5;
     const int cq = lane % G::LANES_PER_ROW, rq = lane / G::LANES_PER_ROW;
@@ -516,7 +614,7 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uin
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, TILE = G::EDGE;
     constexpr int RPL = G::RPL;                         // epilogue rows per lane
     const int i0 = ti * TILE, j0 = tj * TILE;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
 
     // Bit counts of this lane's epilogue rows/columns, fetched now so their
@@ -540,18 +638,22 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uin
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
 
-    if constexpr (PACKED) kloop_packed<G>(d, i0, j0, acc, smem);
-    else kloop_dma<OP, G>(d, i0, j0, acc, smem);
-    lds_barrier();       // every wave is done with the ring before the epilogue reuses it
+    if constexpr (PACKED == 2) {
+        kloop_regs<G>(d, i0, j0, acc);       // no LDS ring: the epilogue regions are per wave
+    } else {
+        if constexpr (PACKED == 1) kloop_packed<G>(d, i0, j0, acc, smem);
+        else kloop_dma<OP, G>(d, i0, j0, acc, smem);
+        lds_barrier();   // every wave is done with the ring before the epilogue reuses it
+    }
     tile_epilogue<OP, MODE, G>(d, i0, j0, acc, bc_row, bc_col, smem);
 }
 
 template <class OP, int MODE, class G, int PACKED>
 __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, uint8_t *smem)
 {
-    const WorkItem it = p.items[idx];
+    const WorkItem it = wave_uniform(p.items[idx]);
     if (p.descs) {
-        const WorkDesc d = p.descs[it.desc];
+        const WorkDesc d = wave_uniform(p.descs[it.desc]);
         mfma_tile<OP, MODE, G, PACKED>(d, it.ti, it.tj, smem);
     } else {
         mfma_tile<OP, MODE, G, PACKED>(p.inline_desc, it.ti, it.tj, smem);
@@ -583,7 +685,10 @@ __global__ __launch_bounds__(G::THREADS, 1) void ld_mfma_kernel(KArgs p)
                 claimed = begin + k < end ? begin + k : -1;
             }
             __syncthreads();
-            const int64_t idx = claimed;
+            const int64_t c = claimed;      // the same in every lane: make it scalar
+            const int64_t idx =
+                (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(c >> 32)) << 32) |
+                          (uint32_t)__builtin_amdgcn_readfirstlane((int)c));
             __syncthreads();
             if (idx < 0) break;
             run_item<OP, MODE, G, PACKED>(p, idx, smem);
@@ -771,12 +876,18 @@ __global__ void zero_queue_kernel(uint32_t *q)
     if (threadIdx.x < 16) q[threadIdx.x] = 0u;
 }
 
+// Dynamic LDS of a kernel variant: the k-stage ring (which the epilogue
+// reuses), or only the epilogue regions when operands go straight to
+// registers.
+template <class G, int PACKED>
+constexpr int lds_bytes() { return PACKED == 2 ? G::EP_BYTES : G::LDS_BYTES; }
+
 template <class OP, int MODE, class G, int PERSIST, int PACKED = 0>
 static hipError_t set_lds_limit()
 {
     return hipFuncSetAttribute((const void *)ld_mfma_kernel<OP, MODE, G, PERSIST, PACKED>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)G::LDS_BYTES);
+                               lds_bytes<G, PACKED>());
 }
 
 typedef Geo<QLD_CFG_L256> GeoL;
@@ -810,7 +921,11 @@ static hipError_t set_lds_limits_packed()
 {
     hipError_t e;
     if ((e = set_lds_limit<FP4, MODE, GeoS, 0, 1>()) != hipSuccess) return e;
-    return set_lds_limit<FP4, MODE, GeoS, 1, 1>();
+    if ((e = set_lds_limit<FP4, MODE, GeoS, 1, 1>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoS, 0, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoS, 1, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoL8, 0, 2>()) != hipSuccess) return e;
+    return set_lds_limit<FP4, MODE, GeoL8, 1, 2>();
 }
 
 // Raise the dynamic LDS limit of every kernel variant (done once per
@@ -834,7 +949,7 @@ template <class OP, int MODE, class G, int PERSIST, int PACKED = 0>
 static hipError_t launch_one(const KArgs &a, unsigned grid, hipStream_t s)
 {
     hipLaunchKernelGGL((ld_mfma_kernel<OP, MODE, G, PERSIST, PACKED>), dim3(grid),
-                       dim3(G::THREADS), (size_t)G::LDS_BYTES, s, a);
+                       dim3(G::THREADS), (size_t)(lds_bytes<G, PACKED>()), s, a);
     return hipGetLastError();
 }
 
@@ -876,6 +991,16 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
         hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, queue);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (packed == 2) {      // packed rows straight into registers
+        if (engine != QLD_ENGINE_MFMA_FP4) return hipErrorInvalidValue;
+        if (tile == QLD_TILE_SMALL)
+            return r2_mode == QLD_R2_EXACT
+                       ? launch_mode<FP4, QLD_R2_EXACT, GeoS, 2>(a, persistent, s)
+                       : launch_mode<FP4, QLD_R2_REFERENCE, GeoS, 2>(a, persistent, s);
+        return r2_mode == QLD_R2_EXACT
+                   ? launch_mode<FP4, QLD_R2_EXACT, GeoL8, 2>(a, persistent, s)
+                   : launch_mode<FP4, QLD_R2_REFERENCE, GeoL8, 2>(a, persistent, s);
     }
     if (packed) {
         if (engine != QLD_ENGINE_MFMA_FP4 || tile != QLD_TILE_SMALL) return hipErrorInvalidValue;

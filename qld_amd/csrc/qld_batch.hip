@@ -34,6 +34,7 @@ struct PlainLaunch {
 
 struct qld_plan {
     int mode = QLD_BATCH_PERSISTENT;
+    int direct = 0;                       // packed rows into registers (fp4)
     int engine = QLD_ENGINE_MFMA_FP4;
     int r2_mode = QLD_R2_REFERENCE;
     std::vector<PlanClass> classes;       // persistent mode
@@ -56,7 +57,7 @@ static void table_release(qld_ctx *ctx, qld_table *t)
 {
     qld_ctx_mfree(ctx, t->packed, t->bytes_packed);
     qld_ctx_mfree(ctx, t->bc, t->bytes_bc);
-    qld_ctx_mfree(ctx, t->X, t->bytes_x);
+    if (t->X) qld_ctx_mfree(ctx, t->X, t->bytes_x);
     delete t;
 }
 
@@ -79,21 +80,24 @@ static int table_create(qld_ctx *ctx, const void *rows_src, const void *bc_src,
     t->words = words;
     t->snp_size = snp_size;
     t->engine = ctx->cfg.engine == QLD_ENGINE_MFMA_I8 ? QLD_ENGINE_MFMA_I8 : QLD_ENGINE_MFMA_FP4;
-    t->row_bytes = mfma_row_bytes(t->engine, words);
+    // fp4 tables in direct mode keep only the packed rows (the kernels
+    // expand bits in registers); otherwise an expanded operand copy too.
+    t->direct = t->engine == QLD_ENGINE_MFMA_FP4 && ctx->fp4_packed == QLD_FP4_DIRECT;
+    t->row_bytes = t->direct ? 0 : mfma_row_bytes(t->engine, words);
     t->bytes_packed = (size_t)rows * words * 8;
     t->bytes_bc = (size_t)rows * 4;
     t->bytes_x = (size_t)rows * t->row_bytes;
     int rc;
     if ((rc = table_alloc(ctx, (void **)&t->packed, t->bytes_packed)) ||
         (rc = table_alloc(ctx, (void **)&t->bc, t->bytes_bc)) ||
-        (rc = table_alloc(ctx, (void **)&t->X, t->bytes_x))) {
+        (!t->direct && (rc = table_alloc(ctx, (void **)&t->X, t->bytes_x)))) {
         table_release(ctx, t);
         return rc;
     }
     hipError_t e = hipMemcpyAsync(t->packed, rows_src, t->bytes_packed, kind, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(t->bc, bc_src, t->bytes_bc, kind, ctx->stream);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !t->direct)
         e = launch_expand(t->engine, t->packed, rows, words, t->X, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -186,13 +190,15 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
     if (p->mode == QLD_BATCH_PERSISTENT) {
         for (auto &c : p->classes) {
             if (!c.nitems) continue;
-            e = launch_mfma_items(p->engine, p->r2_mode, c.tile, 0, c.d_descs, none, c.d_items,
+            e = launch_mfma_items(p->engine, p->r2_mode, c.tile, p->direct ? QLD_FP4_DIRECT : 0,
+                                  c.d_descs, none, c.d_items,
                                   (int64_t)c.nitems, c.grid, c.d_queue, nullptr, ctx->stream);
             if (e != hipSuccess) return e;
         }
     } else {
         for (auto &l : p->plain) {
-            e = launch_mfma_items(p->engine, p->r2_mode, l.tile, 0, nullptr, l.desc,
+            e = launch_mfma_items(p->engine, p->r2_mode, l.tile, p->direct ? QLD_FP4_DIRECT : 0,
+                                  nullptr, l.desc,
                                   p->d_items_all + l.item_off, l.item_cnt, 0, nullptr, nullptr,
                                   ctx->stream);
             if (e != hipSuccess) return e;
@@ -246,6 +252,7 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
     if (!p) return QLD_ERR_NOMEM;
     p->mode = mode;
     p->engine = t->engine;
+    p->direct = t->direct;
     p->r2_mode = ctx->cfg.r2_mode;
 
     // Windows in table order, so that each XCD's contiguous share of the
@@ -259,11 +266,19 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
 
     auto desc_of = [&](const PairWin &w) {
         WorkDesc d;
-        const uint8_t *XI = t->X + (int64_t)w.b_start * t->row_bytes;
-        const uint8_t *XJ = t->X + (int64_t)w.a_start * t->row_bytes;
-        fill_work_desc(d, XI, XJ, t->bc + w.b_start, t->bc + w.a_start, w.b_size, w.a_size,
-                       t->row_bytes, w.tri, t->snp_size, out + w.out_offset, w.a_size, nullptr,
-                       w.a_size);
+        if (t->direct) {
+            fill_work_desc(d, nullptr, nullptr, t->bc + w.b_start, t->bc + w.a_start, w.b_size,
+                           w.a_size, 0, w.tri, t->snp_size, out + w.out_offset, w.a_size,
+                           nullptr, w.a_size);
+            set_packed_operands(d, t->packed + (int64_t)w.b_start * t->words,
+                                t->packed + (int64_t)w.a_start * t->words, t->words);
+        } else {
+            const uint8_t *XI = t->X + (int64_t)w.b_start * t->row_bytes;
+            const uint8_t *XJ = t->X + (int64_t)w.a_start * t->row_bytes;
+            fill_work_desc(d, XI, XJ, t->bc + w.b_start, t->bc + w.a_start, w.b_size, w.a_size,
+                           t->row_bytes, w.tri, t->snp_size, out + w.out_offset, w.a_size,
+                           nullptr, w.a_size);
+        }
         return d;
     };
 

@@ -41,13 +41,14 @@ struct qld_ctx {
     int tiles_key[4] = {-1, -1, -1, -1};         // (ni, nj, tri, tile edge)
     int tile_cutoff = 1024;   // windows of at least this many SNPs use 256-tiles
     int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape
-    int fp4_packed = 0;       // fp4 engine reads packed rows (1) or an expanded copy (0)
+    int fp4_packed = QLD_FP4_DIRECT;   // fp4 operands: QLD_FP4_DIRECT, _PACKED or _EXPANDED
     char err[512] = {0};
 };
 
 // A resident table: packed rows, bit counts and the expanded MFMA operand.
 struct qld_table {
     int rows = 0, words = 0, snp_size = 0, engine = QLD_ENGINE_MFMA_FP4;
+    int direct = 0;           // fp4 kernels read `packed` into registers; no X
     int64_t row_bytes = 0;
     uint64_t *packed = nullptr;
     uint32_t *bc = nullptr;

@@ -328,15 +328,16 @@ int qld_pick_engine(const qld_ctx *ctx, const LdJob &j)
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-// Tile edge for one plain job.  256 x 256 tiles halve operand traffic per
-// pair but leave one workgroup per CU, so the r^2 epilogue cannot overlap
-// other work; 128 x 128 tiles run two workgroups per CU.  Measured on MI355X
-// (30,000 SNPs x 16,622 samples, fp4): 4.27 ms with 128-tiles vs 4.64 ms
-// with 256-tiles, so 128 is the default.
-static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job)
+// Tile edge for one plain job.  Expanded operands: 128 x 128 tiles (two
+// workgroups per CU) measured faster than 256 x 256 (3.47 vs 4.03 ms on the
+// 30,000-SNP triangle).  Direct fp4 operands: 256-tiles of 8 waves once both
+// sides reach the tile cutoff (2.91 vs 3.22 ms on the same triangle), 128
+// below it.
+static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job, int pmode)
 {
-    (void)job;
     if (ctx->plain_tile) return ctx->plain_tile;
+    if (pmode == QLD_FP4_DIRECT && job.m >= ctx->tile_cutoff && job.n >= ctx->tile_cutoff)
+        return QLD_TILE_LARGE;
     return QLD_TILE_SMALL;
 }
 
@@ -366,8 +367,9 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
 {
     // The fp4 engine reads packed rows directly (expanding in the kernel,
     // 128-tiles) unless the context asks for an expanded operand copy.
-    const bool packed = engine == QLD_ENGINE_MFMA_FP4 && ctx->fp4_packed;
-    const int T = packed ? QLD_TILE_SMALL : plain_tile_edge(ctx, job);
+    const int pmode = engine == QLD_ENGINE_MFMA_FP4 ? ctx->fp4_packed : QLD_FP4_EXPANDED;
+    const bool packed = pmode != QLD_FP4_EXPANDED;
+    const int T = pmode == QLD_FP4_PACKED ? QLD_TILE_SMALL : plain_tile_edge(ctx, job, pmode);
     if ((job.n + T - 1) / T > 65535 || (job.m + T - 1) / T > 65535) {
         qld_set_error(ctx, "too many rows for one launch (m=%d n=%d)", job.m, job.n);
         return QLD_ERR_INVALID;
@@ -405,7 +407,7 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     int64_t grid = (int64_t)ctx->num_cu * (T == QLD_TILE_SMALL ? 2 : 1);
     if (grid > ntiles) grid = ntiles;
     if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
-    e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, packed ? 1 : 0, nullptr, d,
+    e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, pmode, nullptr, d,
                           (const WorkItem *)ctx->tiles.ptr, ntiles, (int)grid,
                           (uint32_t *)ctx->queue.ptr, nullptr, ctx->stream);
     if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
@@ -563,8 +565,9 @@ extern "C" int qld_timer_stop(qld_ctx *ctx, float *ms)
 
 extern "C" int qld_set_fp4_operands(qld_ctx *ctx, int mode)
 {
-    if (!ctx || (mode != QLD_FP4_PACKED && mode != QLD_FP4_EXPANDED)) return QLD_ERR_INVALID;
-    ctx->fp4_packed = mode == QLD_FP4_PACKED;
+    if (!ctx || (mode != QLD_FP4_PACKED && mode != QLD_FP4_EXPANDED && mode != QLD_FP4_DIRECT))
+        return QLD_ERR_INVALID;
+    ctx->fp4_packed = mode;
     return QLD_OK;
 }
 

@@ -150,28 +150,47 @@ static int launch_batch(qld_server *s, Slot &sl)
     if (rc) return rc;
     WorkDesc *hd = (WorkDesc *)sl.h_stage;
     WorkItem *hi = (WorkItem *)((char *)sl.h_stage + desc_bytes);
-    size_t cnt = 0;
+    // Requests on direct-mode tables (packed rows into registers) and on
+    // tables with an expanded operand run as two launches; the items of each
+    // group are contiguous.
+    size_t cnt = 0, split = 0;
     int engine = QLD_ENGINE_MFMA_FP4;
-    for (size_t k = 0; k < nd; ++k) {
-        Request &r = s->reqs[sl.reqs[k]];
-        const qld_table *t = r.t;
-        engine = t->engine;
-        const uint8_t *X = t->X + (int64_t)r.start * t->row_bytes;
-        const uint32_t *bc = t->bc + r.start;
-        float *out = (float *)(s->arena_mem + r.out_off);
-        fill_work_desc(hd[k], X, X, bc, bc, r.size, r.size, t->row_bytes, 1, t->snp_size,
-                       out, r.size, nullptr, r.size);
-        append_job_tiles(hi, &cnt, (uint32_t)k, r.size, r.size, 1, T);
-        s->pairs += (int64_t)r.size * (r.size - 1) / 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) split = cnt;
+        for (size_t k = 0; k < nd; ++k) {
+            Request &r = s->reqs[sl.reqs[k]];
+            const qld_table *t = r.t;
+            if (t->direct != (pass == 0)) continue;
+            engine = t->engine;
+            const uint32_t *bc = t->bc + r.start;
+            float *out = (float *)(s->arena_mem + r.out_off);
+            if (t->direct) {
+                fill_work_desc(hd[k], nullptr, nullptr, bc, bc, r.size, r.size, 0, 1,
+                               t->snp_size, out, r.size, nullptr, r.size);
+                const uint64_t *P = t->packed + (int64_t)r.start * t->words;
+                set_packed_operands(hd[k], P, P, t->words);
+            } else {
+                const uint8_t *X = t->X + (int64_t)r.start * t->row_bytes;
+                fill_work_desc(hd[k], X, X, bc, bc, r.size, r.size, t->row_bytes, 1,
+                               t->snp_size, out, r.size, nullptr, r.size);
+            }
+            append_job_tiles(hi, &cnt, (uint32_t)k, r.size, r.size, 1, T);
+            s->pairs += (int64_t)r.size * (r.size - 1) / 2;
+        }
     }
     hipError_t e = hipMemcpyAsync(sl.d_stage, sl.h_stage, bytes, hipMemcpyHostToDevice,
                                   sl.stream);
     WorkDesc none;
     memset(&none, 0, sizeof(none));
-    if (e == hipSuccess)
+    const WorkItem *d_items = (const WorkItem *)((char *)sl.d_stage + desc_bytes);
+    if (e == hipSuccess && split > 0)
+        e = launch_mfma_items(QLD_ENGINE_MFMA_FP4, ctx->cfg.r2_mode, T, QLD_FP4_DIRECT,
+                              (const WorkDesc *)sl.d_stage, none, d_items, (int64_t)split, 0,
+                              nullptr, nullptr, sl.stream);
+    if (e == hipSuccess && cnt > split)
         e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, 0, (const WorkDesc *)sl.d_stage, none,
-                              (const WorkItem *)((char *)sl.d_stage + desc_bytes),
-                              (int64_t)cnt, 0, nullptr, nullptr, sl.stream);
+                              d_items + split, (int64_t)(cnt - split), 0, nullptr, nullptr,
+                              sl.stream);
     for (int64_t id : sl.reqs) {
         Request &r = s->reqs[id];
         if (e == hipSuccess && (r.flags & QLD_REQ_COPY) && r.host_out)

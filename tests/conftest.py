@@ -51,14 +51,17 @@ def engine_factory():
     made = []
 
     def make(**kw):
-        # "mfma_fp4p" = the fp4 engine reading packed rows (in-kernel
-        # expansion) instead of the default expanded operand copy.
-        packed = kw.get("engine") == "mfma_fp4p"
-        if packed:
+        # fp4 operand sources: "mfma_fp4" = the default (packed rows straight
+        # into registers), "mfma_fp4d" = the same, named; "mfma_fp4p" = packed
+        # rows expanded into LDS; "mfma_fp4x" = the expanded operand copy
+        # streamed into LDS.
+        mode = {"mfma_fp4p": "packed", "mfma_fp4d": "direct",
+                "mfma_fp4x": "expanded"}.get(kw.get("engine"))
+        if mode:
             kw["engine"] = "mfma_fp4"
         e = qld_amd.Engine(**kw)
-        if packed:
-            e.set_fp4_operands(True)
+        if mode:
+            e.set_fp4_operands(mode)
         made.append(e)
         return e
     yield make

@@ -17,7 +17,7 @@ from conftest import golden_cases, load_case
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = ["popcount", "mfma_fp4", "mfma_fp4p", "mfma_i8"]
+ENGINES = ["popcount", "mfma_fp4", "mfma_fp4p", "mfma_fp4x", "mfma_i8"]
 # Shapes large enough for 256 x 256 tiles on every CU (the AUTO engine path).
 BIG_SHAPES = [(1100, 900), (600, 3000)]
 TILES = [256, 128]
@@ -202,7 +202,7 @@ def test_errors_are_status_codes(engine_factory):
     assert res.shape == (10, 10)
 
 
-@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_i8"])
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x", "mfma_i8"])
 @pytest.mark.parametrize("mode", ["reference", "exact"])
 @pytest.mark.parametrize("tile", TILES)
 def test_forced_tile_shapes(engine, mode, tile, engine_factory):

@@ -22,8 +22,12 @@ def tables(engine_factory):
     e = engine_factory(engine="auto", r2_mode="reference")
     A16, b16 = qld_amd.synth_blocks(161, 3000, 16622)
     A100, b100 = qld_amd.synth_blocks(100, 700, 100000)
+    # one table of each fp4 operand form, so that batches mix the two
+    # launch groups (packed rows into registers / expanded copy via LDS)
     t16 = e.table(A16, b16, 16622)
+    e.set_fp4_operands("expanded")
     t100 = e.table(A100, b100, 100000)
+    e.set_fp4_operands("direct")
     yield e, [(t16, A16, b16, 16622), (t100, A100, b100, 100000)]
     t16.free()
     t100.free()

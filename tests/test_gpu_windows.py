@@ -38,7 +38,7 @@ def _check_windows(out, wins, A, bc, N, mode="reference"):
         assert bad == 0, f"window start={s} size={n}: {bad} cells differ"
 
 
-@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_i8"])
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x", "mfma_i8"])
 @pytest.mark.parametrize("mode", [qld_amd.BATCH_PERSISTENT, qld_amd.BATCH_PLAIN])
 @pytest.mark.parametrize("cutoff", [300, 0, 1 << 30])
 def test_window_plans(engine, mode, cutoff, engine_factory):
@@ -121,7 +121,7 @@ def test_autotuner_picks_a_candidate(engine_factory):
     out.free()
 
 
-@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_i8"])
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x", "mfma_i8"])
 @pytest.mark.parametrize("mode", [qld_amd.BATCH_PERSISTENT, qld_amd.BATCH_PLAIN])
 def test_pair_window_plans_mix_triangles_and_rectangles(engine, mode, engine_factory):
     """qld_plan_pairs: triangles and rectangles (overlapping, disjoint,
