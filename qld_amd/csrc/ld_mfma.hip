@@ -463,6 +463,10 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
     if (s + 1 < stages) step(s + 1, R2);
 }
 
+#ifndef QLD_DIRECT_DEPTH
+#define QLD_DIRECT_DEPTH 2   // register sets of packed words in flight (2 or 3)
+#endif
+
 // k loop straight from the packed rows (fp4, no LDS, no barrier).  For each
 // group of 4 k-steps (4 packed words = 256 samples of a row) lane l of a
 // 32 x 32 operand fetch loads 16 bytes -- words 2h, 2h+1 of the group, h =
@@ -473,12 +477,11 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 // run.  Rows past the job end read 0 through the buffer bound; words past
 // the row end (W not a multiple of 4) are zeroed.
 template <class G>
-__device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0,
+__device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, int wm, int wn,
                                            v16f (&acc)[G::NBM][G::NBN])
 {
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN;
-    const int lane = threadIdx.x & 63, wave = wave_id();
-    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
+    const int lane = threadIdx.x & 63;
     const int r32 = lane & 31, h = lane >> 5;
     const int W = d.words;
     const int64_t rowb = (int64_t)W * 8;
@@ -497,7 +500,8 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0,
     const uint32_t stepA = (uint32_t)(32 * rowb);      // next 32-row block
     const int groups = (W + 3) / 4;
 
-    v4i pa[NBM], pb[NBN], qa[NBM], qb[NBN];
+    // Words past the row end (the last group when W % 4 != 0) are zeroed
+    // once per fetch, so the k steps need no masking.
     auto fetch = [&](v4i (&xa)[NBM], v4i (&xb)[NBN], int g) {
         const uint32_t k = (uint32_t)g * 32u;
 #pragma unroll
@@ -506,33 +510,64 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0,
 #pragma unroll
         for (int b = 0; b < NBN; ++b)
             xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB + b * stepA + k, 0, 0);
-    };
-    auto compute = [&](const v4i (&xa)[NBM], const v4i (&xb)[NBN], int g) {
-        // words 4g + 2h and 4g + 2h + 1 of the row exist?
-        const bool w0 = 4 * g + 2 * h < W, w1 = 4 * g + 2 * h + 1 < W;
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            const bool ok = (ks < 2) ? w0 : w1;
-            v4i bv[NBN];
-#pragma unroll
-            for (int b = 0; b < NBN; ++b) bv[b] = fp4_chunk(ok ? (uint32_t)xb[b][ks] : 0u);
+        if (4 * g + 4 > W) {
+            const bool w0 = 4 * g + 2 * h < W, w1 = 4 * g + 2 * h + 1 < W;
 #pragma unroll
             for (int a = 0; a < NBM; ++a) {
-                const v4i av = fp4_chunk(ok ? (uint32_t)xa[a][ks] : 0u);
+                if (!w0) xa[a][0] = xa[a][1] = 0;
+                if (!w1) xa[a][2] = xa[a][3] = 0;
+            }
+#pragma unroll
+            for (int b = 0; b < NBN; ++b) {
+                if (!w0) xb[b][0] = xb[b][1] = 0;
+                if (!w1) xb[b][2] = xb[b][3] = 0;
+            }
+        }
+    };
+    auto compute = [&](const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            v4i bv[NBN];
+#pragma unroll
+            for (int b = 0; b < NBN; ++b) bv[b] = fp4_chunk((uint32_t)xb[b][ks]);
+#pragma unroll
+            for (int a = 0; a < NBM; ++a) {
+                const v4i av = fp4_chunk((uint32_t)xa[a][ks]);
 #pragma unroll
                 for (int b = 0; b < NBN; ++b) acc[a][b] = FP4::mma(av, bv[b], acc[a][b]);
             }
         }
     };
+#if QLD_DIRECT_DEPTH == 3
+    // Three register sets in rotation: groups g+1 and g+2 in flight while
+    // group g computes.
+    v4i xa0[NBM], xb0[NBN], xa1[NBM], xb1[NBN], xa2[NBM], xb2[NBN];
+    fetch(xa0, xb0, 0);
+    if (1 < groups) fetch(xa1, xb1, 1);
+    int g = 0;
+    for (; g + 3 <= groups; g += 3) {
+        if (g + 2 < groups) fetch(xa2, xb2, g + 2);
+        compute(xa0, xb0);
+        if (g + 3 < groups) fetch(xa0, xb0, g + 3);
+        compute(xa1, xb1);
+        if (g + 4 < groups) fetch(xa1, xb1, g + 4);
+        compute(xa2, xb2);
+    }
+    if (g < groups) compute(xa0, xb0);
+    if (g + 1 < groups) compute(xa1, xb1);
+#else
+    // Two register sets: group g+1 in flight while group g computes.
+    v4i pa[NBM], pb[NBN], qa[NBM], qb[NBN];
     fetch(pa, pb, 0);
     int g = 0;
     for (; g + 2 <= groups; g += 2) {
         fetch(qa, qb, g + 1);
-        compute(pa, pb, g);
+        compute(pa, pb);
         if (g + 2 < groups) fetch(pa, pb, g + 2);
-        compute(qa, qb, g + 1);
+        compute(qa, qb);
     }
-    if (g < groups) compute(pa, pb, g);
+    if (g < groups) compute(pa, pb);
+#endif
 }
 
 // Fused epilogue.  Block (a, b) register r holds the pair
@@ -542,15 +577,14 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0,
 // and every lane turns 4 consecutive counts of a row into r^2, written with
 // one 16-byte store (rows of WT floats coalesce).
 template <class OP, int MODE, class G>
-__device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
-                                              typename OP::acc_t (&acc)[G::NBM][G::NBN],
+__device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0, int wm, int wn,
+                                              int wave, typename OP::acc_t (&acc)[G::NBM][G::NBN],
                                               const uint32_t *bc_row, const uint32_t *bc_col,
                                               uint8_t *smem)
 {
     typedef Epi<MODE> E;
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, RPL = G::RPL;
-    const int lane = threadIdx.x & 63, wave = wave_id();
-    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
+    const int lane = threadIdx.x & 63;
     const int r32 = lane & 31, half = lane >> 5;
     const int cq = lane % G::LANES_PER_ROW, rq = lane / G::LANES_PER_ROW;
     const int jc = j0 + wn * WTN + 4 * cq;              // first of this lane's 4 columns
@@ -608,14 +642,16 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
 
 // One tile of one job: k loop (expanded operands by DMA, or packed operands
 // expanded in the kernel), then the fused epilogue.
+// wm, wn: the wave's block of the tile; wave: the wave's LDS epilogue
+// region.  Cooperative k loops (LDS ring) need wm, wn = this wave's own.
 template <class OP, int MODE, class G, int PACKED>
-__device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uint8_t *smem)
+__device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int wm, int wn,
+                                          int wave, uint8_t *smem)
 {
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, TILE = G::EDGE;
     constexpr int RPL = G::RPL;                         // epilogue rows per lane
     const int i0 = ti * TILE, j0 = tj * TILE;
-    const int lane = threadIdx.x & 63, wave = wave_id();
-    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
+    const int lane = threadIdx.x & 63;
 
     // Bit counts of this lane's epilogue rows/columns, fetched now so their
     // latency hides under the k loop.
@@ -639,24 +675,25 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, uin
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
 
     if constexpr (PACKED == 2) {
-        kloop_regs<G>(d, i0, j0, acc);       // no LDS ring: the epilogue regions are per wave
+        kloop_regs<G>(d, i0, j0, wm, wn, acc);   // no LDS ring: epilogue regions are per wave
     } else {
         if constexpr (PACKED == 1) kloop_packed<G>(d, i0, j0, acc, smem);
         else kloop_dma<OP, G>(d, i0, j0, acc, smem);
         lds_barrier();   // every wave is done with the ring before the epilogue reuses it
     }
-    tile_epilogue<OP, MODE, G>(d, i0, j0, acc, bc_row, bc_col, smem);
+    tile_epilogue<OP, MODE, G>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
 }
 
 template <class OP, int MODE, class G, int PACKED>
-__device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, uint8_t *smem)
+__device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, int wm, int wn, int wave,
+                                         uint8_t *smem)
 {
     const WorkItem it = wave_uniform(p.items[idx]);
     if (p.descs) {
         const WorkDesc d = wave_uniform(p.descs[it.desc]);
-        mfma_tile<OP, MODE, G, PACKED>(d, it.ti, it.tj, smem);
+        mfma_tile<OP, MODE, G, PACKED>(d, it.ti, it.tj, wm, wn, wave, smem);
     } else {
-        mfma_tile<OP, MODE, G, PACKED>(p.inline_desc, it.ti, it.tj, smem);
+        mfma_tile<OP, MODE, G, PACKED>(p.inline_desc, it.ti, it.tj, wm, wn, wave, smem);
     }
 }
 
@@ -669,8 +706,10 @@ template <class OP, int MODE, class G, int PERSIST, int PACKED>
 __global__ __launch_bounds__(G::THREADS, 1) void ld_mfma_kernel(KArgs p)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    const int wave = wave_id();
     if (!PERSIST) {
-        run_item<OP, MODE, G, PACKED>(p, (int64_t)blockIdx.x, smem);
+        run_item<OP, MODE, G, PACKED>(p, (int64_t)blockIdx.x, wave / G::WAVES_N,
+                                      wave % G::WAVES_N, wave, smem);
         return;
     }
     __shared__ int64_t claimed;
@@ -691,7 +730,8 @@ __global__ __launch_bounds__(G::THREADS, 1) void ld_mfma_kernel(KArgs p)
                           (uint32_t)__builtin_amdgcn_readfirstlane((int)c));
             __syncthreads();
             if (idx < 0) break;
-            run_item<OP, MODE, G, PACKED>(p, idx, smem);
+            run_item<OP, MODE, G, PACKED>(p, idx, wave / G::WAVES_N, wave % G::WAVES_N, wave,
+                                          smem);
         }
     }
 }
@@ -925,6 +965,8 @@ static hipError_t set_lds_limits_packed()
     if ((e = set_lds_limit<FP4, MODE, GeoS, 0, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoS, 1, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoL8, 0, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoL, 0, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoL, 1, 2>()) != hipSuccess) return e;
     return set_lds_limit<FP4, MODE, GeoL8, 1, 2>();
 }
 
@@ -998,6 +1040,10 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
             return r2_mode == QLD_R2_EXACT
                        ? launch_mode<FP4, QLD_R2_EXACT, GeoS, 2>(a, persistent, s)
                        : launch_mode<FP4, QLD_R2_REFERENCE, GeoS, 2>(a, persistent, s);
+        if (g_l256_waves == 4)
+            return r2_mode == QLD_R2_EXACT
+                       ? launch_mode<FP4, QLD_R2_EXACT, GeoL, 2>(a, persistent, s)
+                       : launch_mode<FP4, QLD_R2_REFERENCE, GeoL, 2>(a, persistent, s);
         return r2_mode == QLD_R2_EXACT
                    ? launch_mode<FP4, QLD_R2_EXACT, GeoL8, 2>(a, persistent, s)
                    : launch_mode<FP4, QLD_R2_REFERENCE, GeoL8, 2>(a, persistent, s);
