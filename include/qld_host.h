@@ -129,6 +129,12 @@ int qld_mdf_find_files(const char *dir, const char *id, int snips_per_file,
                        char ***files, int *nfiles);
 void qld_mdf_free_files(char **files, int nfiles);
 
+/* The same selection for chunks whose names contain `suffix` (".mdf" for
+ * MDF directories, ".vcf.gz" for the parser's VCF chunks). */
+int qld_find_chunks(const char *dir, const char *id, int snips_per_file, int total_snips,
+                    int pos_wmin, int pos_wmax, const char *suffix, char ***files,
+                    int *nfiles);
+
 /* A loaded table: one SNP per row, `words` uint64 per row, with the POS, ID
  * and BC columns of the MDF rows (table_x64, correlate_MEM.h:66-75). */
 typedef struct qld_snp_table {
@@ -161,6 +167,40 @@ int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int 
                          int pos_wmax, int pos_wset, int words, int nthreads,
                          qld_snp_table *t);
 void qld_mdf_reader_free(qld_mdf_reader *r);
+
+/* ---- VCF chunk input ------------------------------------------------------ */
+
+/* Rows from the parser's gzip VCF chunks, as the reference correlator builds
+ * them without -mdf (writeToTable_x64, correlate_IOx64.c:125-349, and
+ * snip_processing.c): single-base REF/ALT, FILTER PASS or ".", INFO AF not
+ * 0 or 1 and VT (if present) SNP, GT in FORMAT; the selected samples' GT
+ * alleles as bases (unphased alleles shuffled by rand()), monomorphic sites
+ * and sites with N before their second base dropped, the two bases coded
+ * '0'/'1' in order of appearance and packed MSB-first; BC = bit count.
+ * The reader owns the rand() stream (glibc's default sequence, which the
+ * reference draws from unseeded): use one reader per run and read tables
+ * in the reference's order (task by task, table A then B). */
+typedef struct qld_vcf_reader qld_vcf_reader;
+qld_vcf_reader *qld_vcf_reader_create(void);
+void qld_vcf_reader_free(qld_vcf_reader *r);
+const char *qld_vcf_last_error(void);
+
+/* First line and last #CHROM line of a header file (malloc'd); each chunk
+ * must start with exactly these two lines. */
+int qld_vcf_header_lines(const char *header_path, char **line1, char **chrom_line);
+
+/* Samples of the #CHROM line selected by a list file (whitespace-separated
+ * names; NULL = all): mask[s] per sample column (malloc'd), *valid_count =
+ * selected samples x ploidy (makeValidList, correlate_IO.c:259-290). */
+int qld_vcf_sample_mask(const char *chrom_line, const char *list_file, int ploidy,
+                        unsigned char **mask, int *nsamples, int *valid_count);
+
+/* Rows of `files` in [pos_wmin, pos_wmax] with the reference's line loop
+ * (as qld_mdf_read_table); `snip_size` is the header's sample size. */
+int qld_vcf_read_table(qld_vcf_reader *r, char *const *files, int nfiles, int pos_wmin,
+                       int pos_wmax, int pos_wset, const char *line1, const char *chrom_line,
+                       const unsigned char *mask, int nsamples, int valid_count, int ploidy,
+                       int snip_size, qld_snp_table *t);
 
 /* Write one MDF chunk file exactly as the MDF tool does
  * (process_data.c:447-460): `##fileformat=MDFv0.1`, `##SNPs:<rows>`, the

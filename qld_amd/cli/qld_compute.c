@@ -6,7 +6,8 @@
  * prepares a task, input_list.c:205-323 expands -inputList, and
  * correlate_function.c:60-290 loads the tables and correlates), with the
  * pairwise counts and r^2 computed on the GPU through qld_correlate_report().
- * Input is MDF (-mdf); every task is computed (the reference's thread split
+ * Input is the parser's VCF chunks or, with -mdf, MDF chunks; every task is
+ * computed (the reference's thread split
  * can drop tasks, correlate_function.c:53-57; here there is no split).
  */
 #define _GNU_SOURCE
@@ -32,6 +33,8 @@ typedef struct {
 typedef struct {
     char input[QLD_PATH_MAX], input2[QLD_PATH_MAX], output[QLD_PATH_MAX];
     char list[QLD_PATH_MAX];
+    char samples1[QLD_PATH_MAX], samples2[QLD_PATH_MAX];
+    int has_samples1, has_samples2;
     int in2set, wset1, wset2, listset, outset, pathset;
     int wmin1, wmax1, wmin2, wmax2;
     int ploidy, threads, mdf, gpu, device, dump, plain_tasks;
@@ -71,7 +74,9 @@ static void usage(void)
            "  -posWmin2/-posWmax2 POS   window of the second region\n"
            "  -inputList FILE     lines input,posWmin1,posWmax1,input2,posWmin2,posWmax2\n"
            "  -r2limit R          report pairs with r2 >= R (default 0.2)\n"
-           "  -mdf                MDF input (required: this build reads MDF)\n"
+           "  -mdf                MDF input (default: the parser's VCF chunks)\n"
+           "  -sampleList FILE    samples of the first input (VCF input)\n"
+           "  -sampleList2 FILE   samples of the second input (default: -sampleList)\n"
            "  -gpu                accepted; the engine always runs on the GPU\n"
            "  -blis, -sorted, -competing, -rTable, -threads T, -seed S\n"
            "                      accepted, no effect (MDF files are read on all\n"
@@ -131,9 +136,15 @@ static void parse_args(int argc, char **argv, options *o)
             }
             continue;
         }
-        if (!strcmp(a, "-sampleList") || !strcmp(a, "-sampleList2"))
-            die("\n ERROR: %s: sample selection applies to VCF input; MDF rows are "
-                "already packed for their samples\n\n", a);
+        if (!strcmp(a, "-sampleList") || !strcmp(a, "-sampleList2")) {
+            const int second = !strcmp(a, "-sampleList2");
+            if (need_arg(i, argc)) {
+                snprintf(second ? o->samples2 : o->samples1, QLD_PATH_MAX, "%s", argv[++i]);
+                if (second) o->has_samples2 = 1;
+                else o->has_samples1 = 1;
+            }
+            continue;
+        }
         if (!strcmp(a, "-ploidy")) {
             if (!need_arg(i, argc)) die("Please insert supported ploidy (see --help for more info)%s\n", "");
             const char *p = argv[++i];
@@ -201,9 +212,15 @@ static void parse_args(int argc, char **argv, options *o)
     }
     if (!o->pathset) die("\n ERROR: Please specify a path for the input with -input%s\n\n", "");
     if (!o->outset) die("\n ERROR: Please specify an alignment folder with -output%s\n\n", "");
-    if (!o->mdf)
-        die("\n ERROR: this build reads MDF input; add -mdf (VCF chunks are converted to "
-            "MDF by the MDF tool first)%s\n\n", "");
+    if (o->has_samples2 && !o->has_samples1)
+        die("\n ERROR: Please set sampleList before setting sampleList2%s\n\n", "");
+    if (o->has_samples1 && !o->has_samples2) {
+        memcpy(o->samples2, o->samples1, sizeof o->samples2);
+        o->has_samples2 = 1;
+    }
+    if (o->mdf && o->has_samples1)
+        die("\n ERROR: -sampleList applies to VCF input; MDF rows are already packed for "
+            "their samples%s\n\n", "");
     long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
     o->threads = ncpu < 1 ? 1 : ncpu > 32 ? 32 : (int)ncpu;
 }
@@ -217,8 +234,9 @@ static void check(int rc, const char *what)
 }
 
 /* Window and task set-up of preprocess_data (load_balancer.c:122-357). */
-static void prepare_task(task *t, FILE *rep, int id)
+static void prepare_task(const options *o, task *t, FILE *rep, int id)
 {
+    const char *suffix = o->mdf ? ".mdf" : ".vcf.gz";
     check(qld_mdf_read_header(t->input, &t->h1), "header of -input");
     if (t->in2set) {
         check(qld_mdf_read_header(t->input2, &t->h2), "header of -input2");
@@ -261,10 +279,10 @@ static void prepare_task(task *t, FILE *rep, int id)
     int spf = t->h1.snips_per_file, spf2 = t->h2.snips_per_file;
     if (spf < 0 || spf > t->h1.total_snips) spf = t->h1.total_snips;
     if (t->in2set && (spf2 < 0 || spf2 > t->h2.total_snips)) spf2 = t->h2.total_snips;
-    check(qld_mdf_find_files(t->input, t->h1.id, spf, t->h1.total_snips, t->wmin1, t->wmax1,
-                             &t->files1, &t->nfiles1), "chunks of -input");
-    check(qld_mdf_find_files(t->in2set ? t->input2 : t->input, t->h2.id, spf2,
-                             t->h2.total_snips, t->wmin2, t->wmax2, &t->files2, &t->nfiles2),
+    check(qld_find_chunks(t->input, t->h1.id, spf, t->h1.total_snips, t->wmin1, t->wmax1,
+                          suffix, &t->files1, &t->nfiles1), "chunks of -input");
+    check(qld_find_chunks(t->in2set ? t->input2 : t->input, t->h2.id, spf2, t->h2.total_snips,
+                          t->wmin2, t->wmax2, suffix, &t->files2, &t->nfiles2),
           "chunks of the second region");
     if (rep) {
         /* the task line of input_report.txt (input_list.c:43-59) */
@@ -322,7 +340,7 @@ static task *list_tasks(options *o, int *ntask, FILE *rep)
         t.in2set = o->in2set;
         t.wset1 = o->wset1;
         t.wset2 = o->wset2;
-        prepare_task(&t, rep, n);
+        prepare_task(o, &t, rep, n);
         task *nt = (task *)realloc(ts, (size_t)(n + 1) * sizeof(task));
         if (!nt) die("ERROR: out of host memory%s\n", "");
         ts = nt;
@@ -374,24 +392,70 @@ typedef struct {
     int upper;      /* the report prints j > i only (second window unset) */
 } loaded;
 
+static void vcheck(int rc, const char *what)
+{
+    if (rc != QLD_OK) {
+        fprintf(stderr, "\n ERROR: %s: %s\n\n", what, qld_vcf_last_error());
+        exit(1);
+    }
+}
+
+/* The VCF reader: one rand() stream for the whole run, drawn in the
+ * reference's order (tasks in order, table A before table B). */
+static qld_vcf_reader *g_vcf;
+
+static void load_vcf(const options *o, const task *t, int in2set, loaded *L)
+{
+    char *l1a, *cla, *l1b, *clb;
+    vcheck(qld_vcf_header_lines(t->h1.header_path, &l1a, &cla), "header of -input");
+    vcheck(qld_vcf_header_lines((in2set ? t->h2 : t->h1).header_path, &l1b, &clb),
+           "header of the second region");
+    unsigned char *m1, *m2;
+    int n1, n2, v1, v2;
+    vcheck(qld_vcf_sample_mask(cla, o->has_samples1 ? o->samples1 : NULL, o->ploidy, &m1, &n1,
+                               &v1), "samples of -input");
+    vcheck(qld_vcf_sample_mask(clb, o->has_samples2 ? o->samples2 : NULL, o->ploidy, &m2, &n2,
+                               &v2), "samples of the second region");
+    if (v1 <= 0 || v2 <= 0) die("\n ERROR: no sample selected%s\n\n", "");
+    vcheck(qld_vcf_read_table(g_vcf, t->files1, t->nfiles1, t->wmin1, t->wmax1, t->wset1, l1a,
+                              cla, m1, n1, v1, o->ploidy, t->h1.snip_size, &L->A),
+           "reading the first region");
+    if (in2set || t->wset2)
+        vcheck(qld_vcf_read_table(g_vcf, t->files2, t->nfiles2, t->wmin2, t->wmax2, t->wset2,
+                                  l1b, clb, m2, n2, v2, o->ploidy,
+                                  (in2set ? t->h2 : t->h1).snip_size, &L->B),
+               "reading the second region");
+    free(l1a);
+    free(cla);
+    free(l1b);
+    free(clb);
+    free(m1);
+    free(m2);
+}
+
 static void load_task(const options *o, const task *t, loaded *L)
 {
     memset(L, 0, sizeof *L);
     int in2set = t->in2set && strcmp(t->input, t->input2) != 0;
-    const int words1 = (t->h1.samples * o->ploidy + 63) / 64;
-    const int words2 = (t->h2.samples * o->ploidy + 63) / 64;
-    check(qld_mdf_reader_table(g_reader, t->files1, t->nfiles1, t->wmin1, t->wmax1, t->wset1,
-                               words1, o->threads, &L->A), "reading the first region");
     L->single = !t->wset2 && !in2set;
     L->upper = !t->wset2;
-    if (in2set)
-        check(qld_mdf_reader_table(g_reader, t->files2, t->nfiles2, t->wmin2, t->wmax2,
-                                   t->wset2, words2, o->threads, &L->B),
-              "reading the second region");
-    else if (t->wset2)
-        check(qld_mdf_reader_table(g_reader, t->files2, t->nfiles2, t->wmin2, t->wmax2,
-                                   t->wset2, words1, o->threads, &L->B),
-              "reading the second region");
+    if (!o->mdf) {
+        load_vcf(o, t, in2set, L);
+    } else {
+        const int words1 = (t->h1.samples * o->ploidy + 63) / 64;
+        const int words2 = (t->h2.samples * o->ploidy + 63) / 64;
+        check(qld_mdf_reader_table(g_reader, t->files1, t->nfiles1, t->wmin1, t->wmax1,
+                                   t->wset1, words1, o->threads, &L->A),
+              "reading the first region");
+        if (in2set)
+            check(qld_mdf_reader_table(g_reader, t->files2, t->nfiles2, t->wmin2, t->wmax2,
+                                       t->wset2, words2, o->threads, &L->B),
+                  "reading the second region");
+        else if (t->wset2)
+            check(qld_mdf_reader_table(g_reader, t->files2, t->nfiles2, t->wmin2, t->wmax2,
+                                       t->wset2, words1, o->threads, &L->B),
+                  "reading the second region");
+    }
     if (!L->single && L->B.words != L->A.words) {
         fprintf(stderr, "\n ERROR: the two regions pack %d and %d words per SNP\n\n",
                 L->A.words, L->B.words);
@@ -612,9 +676,10 @@ int main(int argc, char **argv)
     options o;
     parse_args(argc, argv, &o);
     g_reader = qld_mdf_reader_create();
-    if (!g_reader) die("ERROR: out of host memory%s\n", "");
+    g_vcf = qld_vcf_reader_create();
+    if (!g_reader || !g_vcf) die("ERROR: out of host memory%s\n", "");
     if (o.gpu) printf("GPU kernel active\n");
-    printf("MDF input active\n");
+    if (o.mdf) printf("MDF input active\n");
 
     FILE *rep = fopen("input_report.txt", "w");
     task *tasks;
@@ -634,7 +699,7 @@ int main(int argc, char **argv)
         tasks[0].wmax1 = o.wmax1;
         tasks[0].wmin2 = o.wmin2;
         tasks[0].wmax2 = o.wmax2;
-        prepare_task(&tasks[0], rep, 0);
+        prepare_task(&o, &tasks[0], rep, 0);
         ntask = 1;
     }
     if (rep) fclose(rep);
@@ -664,5 +729,6 @@ int main(int argc, char **argv)
     printf("Output saved in: %s\n", dirname(o.output));
     free(tasks);
     qld_mdf_reader_free(g_reader);
+    qld_vcf_reader_free(g_vcf);
     return 0;
 }

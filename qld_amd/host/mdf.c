@@ -245,6 +245,14 @@ int qld_mdf_find_files(const char *dir, const char *id, int snips_per_file,
                        int total_snips, int pos_wmin, int pos_wmax,
                        char ***files_out, int *nfiles_out)
 {
+    return qld_find_chunks(dir, id, snips_per_file, total_snips, pos_wmin, pos_wmax, ".mdf",
+                           files_out, nfiles_out);
+}
+
+int qld_find_chunks(const char *dir, const char *id, int snips_per_file, int total_snips,
+                    int pos_wmin, int pos_wmax, const char *suffix, char ***files_out,
+                    int *nfiles_out)
+{
     if (!dir || !id || !files_out || !nfiles_out) return fail(QLD_ERR_INVALID, "null argument");
     char d[QLD_PATH_MAX];
     dir_with_slash(d, sizeof d, dir);
@@ -263,7 +271,7 @@ int qld_mdf_find_files(const char *dir, const char *id, int snips_per_file,
             char path[QLD_PATH_MAX];
             snprintf(path, sizeof path, "%s%s", d, e->d_name);
             if (!is_regular(path)) continue;
-            if (!strstr(e->d_name, id) || strstr(e->d_name, "header") || !strstr(e->d_name, ".mdf"))
+            if (!strstr(e->d_name, id) || strstr(e->d_name, "header") || !strstr(e->d_name, suffix))
                 continue;
             char tid[256];
             int tmin, tmax, tpmin, tpmax;

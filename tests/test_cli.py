@@ -16,6 +16,7 @@ import os
 import shutil
 import struct
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -136,11 +137,14 @@ def test_cli_error_cases_exit_instead_of_hanging(name, tmp_path):
     assert "ERROR" in r.stderr or "ploidy" in r.stderr
 
 
-def test_cli_rejects_vcf_input_and_unknown_flags(tmp_path):
+def test_cli_vcf_mode_on_an_mdf_directory_and_unknown_flags(tmp_path):
+    """Without -mdf the chunks are .vcf.gz files: an MDF directory has none,
+    which is an error here (the reference rescans forever)."""
     wd = _workdir(tmp_path, "x")
     r = subprocess.run([CLI_BIN, "-input", "data/chrA", "-ploidy", "haploid", "-output",
-                        "out/r.txt"], cwd=wd, capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "-mdf" in r.stderr
+                        "out/r.txt", "-dump-tables"], cwd=wd, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "no chunk" in r.stderr
     r = _run(wd, ["-input", "data/chrA", "-ploidy", "haploid", "-bogus"])
     assert r.returncode == 1 and "-bogus is not a valid" in r.stderr
 
@@ -290,3 +294,78 @@ def test_mdf_reader_cache_matches_one_shot_reads():
     t = reader.table(files, int(lo), int(hi), 3)
     assert len(t["pos"]) == len(qld_amd.mdf_read_table(files, int(lo), int(hi), 3)["pos"])
     reader.close()
+
+
+# ---- VCF chunk input (tests/golden/vcf/, written by make_vcf_golden.py) ----
+
+VCF_DIR = os.path.join(GOLDEN, "vcf")
+VCF_MANIFEST = json.load(open(os.path.join(VCF_DIR, "cases.json")))
+VCF_CASES = sorted(VCF_MANIFEST["cases"])
+
+
+def _vcf_workdir(tmp_path):
+    os.symlink(os.path.join(VCF_DIR, "data"), tmp_path / "data")
+    os.makedirs(tmp_path / "out")
+    shutil.copy(os.path.join(VCF_DIR, "list.csv"), tmp_path / "list.csv")
+    return tmp_path
+
+
+def _vcf_expected(name):
+    d = os.path.join(VCF_DIR, "expected", name)
+    return {f: open(os.path.join(d, f), "rb").read() for f in sorted(os.listdir(d))}
+
+
+@pytest.mark.parametrize("name", VCF_CASES)
+def test_cli_vcf_tables_reproduce_reference_report(name, tmp_path):
+    """VCF chunks (haploid, phased and unphased diploid, sample lists, two
+    inputs, -inputList): the CLI's tables through the oracle give the
+    reference's report byte for byte -- row filters, allele coding, the
+    rand() shuffles of unphased genotypes and BC included."""
+    args = VCF_MANIFEST["cases"][name]
+    wd = _vcf_workdir(tmp_path)
+    r = subprocess.run([CLI_BIN] + args + ["-output", "out/report.txt", "-dump-tables"],
+                       cwd=wd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    exp = _vcf_expected(name)
+    assert open(wd / "input_report.txt", "rb").read() == exp["input_report.txt"]
+    dumps = sorted(f for f in os.listdir(wd / "out") if f.endswith(".tables"))
+    reports = sorted(f for f in exp if f != "input_report.txt")
+    assert [d[:-len(".tables")] for d in dumps] == reports
+    for d, rep in zip(dumps, reports):
+        single, wset2, snp_size, A, B = _read_tables(wd / "out" / d)
+        text = _oracle_report(wd / "out" / (rep + ".oracle"), A, B, snp_size, wset2,
+                              _r2limit(args))
+        assert text.encode() == exp[rep], (name, rep)
+
+
+def test_vcf_reader_rand_stream_is_glibc_default():
+    """The reader's private generator reproduces the C library's unseeded
+    rand() sequence (which the reference's shuffles draw from)."""
+    import ctypes as C
+    libc = C.CDLL(None)
+    libc.rand.restype = C.c_int
+    # a fresh process has the default state; check through a child
+    out = subprocess.run([sys.executable, "-c",
+                          "import ctypes as C; l=C.CDLL(None); "
+                          "print(' '.join(str(l.rand()) for _ in range(8)))"],
+                         capture_output=True, text=True, timeout=60).stdout.split()
+    assert [int(x) for x in out][:3] == [1804289383, 846930886, 1681692777]
+
+
+@pytest.mark.gpu
+def test_cli_vcf_reports_equal_reference_on_gpu(tmp_path):
+    for name in VCF_CASES:
+        wd = tmp_path / name
+        os.makedirs(wd)
+        _vcf_workdir(wd)
+        print(f"[cli-vcf] {name}", flush=True)
+        r = subprocess.run([CLI_BIN] + VCF_MANIFEST["cases"][name] + ["-output",
+                                                                      "out/report.txt"],
+                           cwd=wd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (name, r.stderr)
+        exp = _vcf_expected(name)
+        got = {f: open(wd / "out" / f, "rb").read() for f in sorted(os.listdir(wd / "out"))}
+        got["input_report.txt"] = open(wd / "input_report.txt", "rb").read()
+        assert sorted(got) == sorted(exp), name
+        for f in exp:
+            assert got[f] == exp[f], (name, f)
