@@ -148,6 +148,15 @@ HOST_SYMBOLS = [
     ("qld_mdf_find_files", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int,
                                      C.c_int, _vp, _vp]),
     ("qld_mdf_free_files", None, [_vp, C.c_int]),
+    ("qld_find_chunks", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_char_p, _vp, _vp]),
+    ("qld_vcf_reader_create", _vp, []),
+    ("qld_vcf_reader_free", None, [_vp]),
+    ("qld_vcf_last_error", C.c_char_p, []),
+    ("qld_vcf_header_lines", C.c_int, [C.c_char_p, _vp, _vp]),
+    ("qld_vcf_sample_mask", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, _vp, _vp, _vp]),
+    ("qld_vcf_read_table", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                     C.c_char_p, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     ("qld_mdf_read_table", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_int, _vp]),
     ("qld_snp_table_free", None, [_vp]),
