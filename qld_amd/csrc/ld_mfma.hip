@@ -28,6 +28,7 @@
 #include "ld_mfma.h"
 
 #include <stdlib.h>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -103,6 +104,15 @@ struct FP4 {
         v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
         v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
         return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, 127,
+                                                                0, 127);
+    }
+    // The same product with operand A scaled by 2^(EA-127) (E8M0 exponent).
+    template <int EA>
+    static __device__ __forceinline__ acc_t mma_scaled(v4i a, v4i b, acc_t c)
+    {
+        v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
+        v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, EA,
                                                                 0, 127);
     }
     // Counts travel through LDS as raw 32-bit words.
@@ -367,6 +377,27 @@ __device__ __forceinline__ v4i fp4_chunk(uint32_t x)
     return c;
 }
 
+// Bit-plane expansion of 128 packed samples (4 dwords) for the direct k
+// loop: plane P keeps bit P of every nibble of every dword in place, so a
+// plane costs one AND (plane 3, whose bit would be the e2m1 sign, is first
+// moved down to bit 2).  The nibble values are 0.5 (bit 0), 1.0 (bit 1) and
+// 2.0 (bit 2); the A operand's E8M0 scale brings every product of two set
+// bits back to exactly 1.0 (scale 4 for plane 0, 1 for plane 1, 1/4 for
+// planes 2 and 3).  The four planes of a lane's 128 bits are the k values
+// of four consecutive MFMA k steps; the sample -> K mapping is the same for
+// both operands.
+template <int P>
+__device__ __forceinline__ v4i fp4_plane(const v4i &x)
+{
+    v4i c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        c[i] = P < 3 ? (int)((uint32_t)x[i] & (0x11111111u << P))
+                     : (int)(((uint32_t)x[i] >> 1) & 0x44444444u);
+    return c;
+}
+template <int P> struct PlaneScale { static constexpr int EA = P == 0 ? 129 : P == 1 ? 127 : 125; };
+
 // k loop over packed operands (fp4 only): lane t stages row t%TILE of
 // operand t/TILE.  Each 128-byte LDS stage row is the expansion of 4 packed
 // words (chunk c = packed dword c); packed words are fetched three stages
@@ -466,6 +497,9 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 #ifndef QLD_DIRECT_DEPTH
 #define QLD_DIRECT_DEPTH 2   // register sets of packed words in flight (2 or 3)
 #endif
+#ifndef QLD_FP4_PLANES
+#define QLD_FP4_PLANES 1     // direct k loop: bit planes + MFMA scales (0: fp4_chunk)
+#endif
 
 // k loop straight from the packed rows (fp4, no LDS, no barrier).  For each
 // group of 4 k-steps (4 packed words = 256 samples of a row) lane l of a
@@ -524,6 +558,27 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
             }
         }
     };
+#if QLD_FP4_PLANES
+    auto plane = [&](auto pc, const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
+        constexpr int P = decltype(pc)::value;
+        v4i bv[NBN];
+#pragma unroll
+        for (int b = 0; b < NBN; ++b) bv[b] = fp4_plane<P>(xb[b]);
+#pragma unroll
+        for (int a = 0; a < NBM; ++a) {
+            const v4i av = fp4_plane<P>(xa[a]);
+#pragma unroll
+            for (int b = 0; b < NBN; ++b)
+                acc[a][b] = FP4::mma_scaled<PlaneScale<P>::EA>(av, bv[b], acc[a][b]);
+        }
+    };
+    auto compute = [&](const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
+        plane(std::integral_constant<int, 0>(), xa, xb);
+        plane(std::integral_constant<int, 1>(), xa, xb);
+        plane(std::integral_constant<int, 2>(), xa, xb);
+        plane(std::integral_constant<int, 3>(), xa, xb);
+    };
+#else
     auto compute = [&](const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
@@ -538,6 +593,7 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
             }
         }
     };
+#endif
 #if QLD_DIRECT_DEPTH == 3
     // Three register sets in rotation: groups g+1 and g+2 in flight while
     // group g computes.
