@@ -140,6 +140,16 @@ struct KArgs {
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
+// Global-memory views of the job's pointers: accesses through them compile
+// to global_* instructions instead of flat_* ones (a pending flat access
+// makes the compiler wait for every outstanding load at each later wait).
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) float g_f32;
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((address_space(1))) const uint32_t g_cu32;
+typedef __attribute__((address_space(1))) v4f g_v4f;
+typedef __attribute__((address_space(1))) v4u g_v4u;
 
 template <int TILE>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t panel_rsrc(const uint8_t *base, int row0,
@@ -494,9 +504,6 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
     if (s + 1 < stages) step(s + 1, R2);
 }
 
-#ifndef QLD_DIRECT_DEPTH
-#define QLD_DIRECT_DEPTH 2   // register sets of packed words in flight (2 or 3)
-#endif
 #ifndef QLD_FP4_PLANES
 #define QLD_FP4_PLANES 1     // direct k loop: bit planes + MFMA scales (0: fp4_chunk)
 #endif
@@ -534,8 +541,6 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     const uint32_t stepA = (uint32_t)(32 * rowb);      // next 32-row block
     const int groups = (W + 3) / 4;
 
-    // Words past the row end (the last group when W % 4 != 0) are zeroed
-    // once per fetch, so the k steps need no masking.
     auto fetch = [&](v4i (&xa)[NBM], v4i (&xb)[NBN], int g) {
         const uint32_t k = (uint32_t)g * 32u;
 #pragma unroll
@@ -544,18 +549,19 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
 #pragma unroll
         for (int b = 0; b < NBN; ++b)
             xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB + b * stepA + k, 0, 0);
-        if (4 * g + 4 > W) {
-            const bool w0 = 4 * g + 2 * h < W, w1 = 4 * g + 2 * h + 1 < W;
+    };
+    // Words past the row end (only in group W/4, when W % 4 != 0) are zeroed.
+    auto mask_tail = [&](v4i (&xa)[NBM], v4i (&xb)[NBN], int g) {
+        const bool w0 = 4 * g + 2 * h < W, w1 = 4 * g + 2 * h + 1 < W;
 #pragma unroll
-            for (int a = 0; a < NBM; ++a) {
-                if (!w0) xa[a][0] = xa[a][1] = 0;
-                if (!w1) xa[a][2] = xa[a][3] = 0;
-            }
+        for (int a = 0; a < NBM; ++a) {
+            if (!w0) xa[a][0] = xa[a][1] = 0;
+            if (!w1) xa[a][2] = xa[a][3] = 0;
+        }
 #pragma unroll
-            for (int b = 0; b < NBN; ++b) {
-                if (!w0) xb[b][0] = xb[b][1] = 0;
-                if (!w1) xb[b][2] = xb[b][3] = 0;
-            }
+        for (int b = 0; b < NBN; ++b) {
+            if (!w0) xb[b][0] = xb[b][1] = 0;
+            if (!w1) xb[b][2] = xb[b][3] = 0;
         }
     };
 #if QLD_FP4_PLANES
@@ -594,49 +600,48 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
         }
     };
 #endif
-#if QLD_DIRECT_DEPTH == 3
-    // Three register sets in rotation: groups g+1 and g+2 in flight while
-    // group g computes.
-    v4i xa0[NBM], xb0[NBN], xa1[NBM], xb1[NBN], xa2[NBM], xb2[NBN];
-    fetch(xa0, xb0, 0);
-    if (1 < groups) fetch(xa1, xb1, 1);
-    int g = 0;
-    for (; g + 3 <= groups; g += 3) {
-        if (g + 2 < groups) fetch(xa2, xb2, g + 2);
-        compute(xa0, xb0);
-        if (g + 3 < groups) fetch(xa0, xb0, g + 3);
-        compute(xa1, xb1);
-        if (g + 4 < groups) fetch(xa1, xb1, g + 4);
-        compute(xa2, xb2);
-    }
-    if (g < groups) compute(xa0, xb0);
-    if (g + 1 < groups) compute(xa1, xb1);
-#else
-    // Two register sets: group g+1 in flight while group g computes.
+    // Two register sets: group g+1 in flight while group g computes.  The
+    // partial group W/4 (W % 4 != 0) goes first, masked; the loop's loads
+    // are then unconditional, so it carries no register copies (a group read
+    // past the last full one is either the next row's words or, past the
+    // block, zeros through the descriptor bound, and is never computed).
+    const int full = W >> 2;
     v4i pa[NBM], pb[NBN], qa[NBM], qb[NBN];
-    fetch(pa, pb, 0);
-    int g = 0;
-    for (; g + 2 <= groups; g += 2) {
-        fetch(qa, qb, g + 1);
+    if (full < groups) {
+        fetch(pa, pb, full);
+        mask_tail(pa, pb, full);
         compute(pa, pb);
-        if (g + 2 < groups) fetch(pa, pb, g + 2);
-        compute(qa, qb);
     }
-    if (g < groups) compute(pa, pb);
-#endif
+    fetch(pa, pb, 0);
+    // Drain every outstanding memory operation once here (the previous
+    // tile's r^2 stores among them).  The compiler treats loads and stores
+    // pending together as completing out of order and would otherwise wait
+    // for everything (vmcnt 0) at every k step of the loop instead of for
+    // the older group only.
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+    int g = 0;
+    for (; g + 2 <= full; g += 2) {
+        // The scheduling barriers keep each group's loads a whole compute
+        // ahead of their use (left alone, the scheduler sinks them next to
+        // the first use to save registers, and the waits then stall).
+        fetch(qa, qb, g + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(pa, pb);
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(pa, pb, g + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(qa, qb);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (g < full) compute(pa, pb);
 }
 
-// Fused epilogue.  Block (a, b) register r holds the pair
-//   row wm*WT + a*32 + 8*(r/4) + 4*(lane/32) + r%4, col wn*WT + b*32 + lane%32
-// of the tile.  The r^2 terms of the wave's WT rows go to LDS once; then
-// for each a the wave stages those 32 rows of counts in its own LDS region
-// and every lane turns 4 consecutive counts of a row into r^2, written with
-// one 16-byte store (rows of WT floats coalesce).
-template <class OP, int MODE, class G>
-__device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0, int wm, int wn,
-                                              int wave, typename OP::acc_t (&acc)[G::NBM][G::NBN],
-                                              const uint32_t *bc_row, const uint32_t *bc_col,
-                                              uint8_t *smem)
+template <class OP, int MODE, class G, bool INTERIOR>
+__device__ __forceinline__ void tile_epilogue_impl(const WorkDesc &d, int i0, int j0, int wm, int wn,
+                                                   int wave,
+                                                   typename OP::acc_t (&acc)[G::NBM][G::NBN],
+                                                   const uint32_t *bc_row, const uint32_t *bc_col,
+                                                   uint8_t *smem)
 {
     typedef Epi<MODE> E;
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, RPL = G::RPL;
@@ -652,7 +657,6 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
     typename E::term col[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) col[q] = E::make(bc_col[q], d);
-    const bool vec_ok = d.res && (d.ldr & 3) == 0 && (((uintptr_t)d.res) & 15) == 0;
 #pragma unroll
     for (int a = 0; a < NBM; ++a) {
 #pragma unroll
@@ -669,31 +673,65 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
             const int i = ibase + rl;
             const uint4 c4 = *(const uint4 *)(ep + rl * G::EP_PITCH + 4 * cq);
             const typename E::term rowt = E::unpack(terms[a * 32 + rl]);
-            if (i >= d.ni) continue;
+            if (!INTERIOR && i >= d.ni) continue;
             const float cv[4] = {OP::count(c4.x), OP::count(c4.y), OP::count(c4.z),
                                  OP::count(c4.w)};
             float rv[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) rv[q] = E::r2(cv[q], col[q], rowt, d);
-            const bool full = jc + 3 < d.nj && (!d.tri || jc > i);
-            if (d.res) {
-                float *dst = d.res + (int64_t)i * d.ldr + jc;
-                if (full && vec_ok) {
-                    *(float4 *)dst = make_float4(rv[0], rv[1], rv[2], rv[3]);
-                } else {
+            if constexpr (INTERIOR) {
+                // Every column of the tile is inside the job (and above the
+                // diagonal for a triangle) and rows are 16-byte aligned.
+                if (d.res) {
+                    const v4f v = {rv[0], rv[1], rv[2], rv[3]};
+                    *(g_v4f *)((g_f32 *)d.res + (int64_t)i * d.ldr + jc) = v;
+                }
+                if (d.cnt) {
+                    const v4u v = {(uint32_t)cv[0], (uint32_t)cv[1], (uint32_t)cv[2],
+                                   (uint32_t)cv[3]};
+                    *(g_v4u *)((g_u32 *)d.cnt + (int64_t)i * d.ldc + jc) = v;
+                }
+            } else {
+                if (d.res) {
+                    g_f32 *dst = (g_f32 *)d.res + (int64_t)i * d.ldr + jc;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         if (jc + q < d.nj && (!d.tri || jc + q > i)) dst[q] = rv[q];
                 }
-            }
-            if (d.cnt) {
-                uint32_t *dst = d.cnt + (int64_t)i * d.ldc + jc;
+                if (d.cnt) {
+                    g_u32 *dst = (g_u32 *)d.cnt + (int64_t)i * d.ldc + jc;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (jc + q < d.nj && (!d.tri || jc + q > i)) dst[q] = (uint32_t)cv[q];
+                    for (int q = 0; q < 4; ++q)
+                        if (jc + q < d.nj && (!d.tri || jc + q > i)) dst[q] = (uint32_t)cv[q];
+                }
             }
         }
     }
+}
+
+// Fused epilogue.  Block (a, b) register r holds the pair
+//   row wm*WT + a*32 + 8*(r/4) + 4*(lane/32) + r%4, col wn*WT + b*32 + lane%32
+// of the tile.  The r^2 terms of the wave's WT rows go to LDS once; then
+// for each a the wave stages those 32 rows of counts in its own LDS region
+// and every lane turns 4 consecutive counts of a row into r^2.  A tile
+// wholly inside the job (and, for a triangle, wholly above the diagonal)
+// with 16-byte aligned rows writes them with one 16-byte store per lane and
+// row (rows of WT floats coalesce); any other tile checks every element.
+template <class OP, int MODE, class G>
+__device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0, int wm, int wn,
+                                              int wave, typename OP::acc_t (&acc)[G::NBM][G::NBN],
+                                              const uint32_t *bc_row, const uint32_t *bc_col,
+                                              uint8_t *smem)
+{
+    constexpr int TILE = G::EDGE;
+    const bool inside = i0 + TILE <= d.ni && j0 + TILE <= d.nj && (!d.tri || j0 >= i0 + TILE);
+    const bool aligned =
+        (!d.res || ((d.ldr & 3) == 0 && (((uintptr_t)d.res) & 15) == 0)) &&
+        (!d.cnt || ((d.ldc & 3) == 0 && (((uintptr_t)d.cnt) & 15) == 0));
+    if (inside && aligned)
+        tile_epilogue_impl<OP, MODE, G, true>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
+    else
+        tile_epilogue_impl<OP, MODE, G, false>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
 }
 
 // One tile of one job: k loop (expanded operands by DMA, or packed operands
@@ -715,12 +753,12 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
         int i = i0 + wm * WTM + q * 64 + lane;
-        bc_row[q] = (lane + q * 64 < WTM && i < d.ni) ? d.bcI[i] : 0u;
+        bc_row[q] = (lane + q * 64 < WTM && i < d.ni) ? ((g_cu32 *)d.bcI)[i] : 0u;
     }
     const int jc = j0 + wn * WTN + 4 * (lane % G::LANES_PER_ROW);
     uint32_t bc_col[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? d.bcJ[jc + q] : 0u;
+    for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? ((g_cu32 *)d.bcJ)[jc + q] : 0u;
 
     typename OP::acc_t acc[NBM][NBN];
 #pragma unroll
