@@ -38,16 +38,21 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-// Geometry of a TILE x TILE workgroup tile computed by WGM x WGN wave64s
-// (each a WTM x WTN block of 32 x 32 MFMA tiles) whose k stages carry SBY
-// bytes of every row (SBY/32 MFMA k-steps) in a ring of NST LDS buffers.
-template <int TILE, int SBY, int NST, int WGM, int WGN>
+// Geometry of a TILE x TILE tile computed by WGM x WGN wave64s (each a
+// WTM x WTN block of 32 x 32 MFMA tiles) whose k stages carry SBY bytes of
+// every row (SBY/32 MFMA k-steps) in a ring of NST LDS buffers.  SPLIT > 1
+// (direct operands only) hands the tile to SPLIT workgroups, each taking
+// WGM/SPLIT rows of waves, sized so that SPLIT workgroups share a CU: while
+// one is in its r^2 epilogue, the other keeps the matrix cores busy.
+template <int TILE, int SBY, int NST, int WGM, int WGN, int SPLIT_ = 1>
 struct Geo {
     static constexpr int EDGE = TILE;
     static constexpr int SB = SBY;
     static constexpr int NSTAGE = NST;
+    static constexpr int SPLIT = SPLIT_;
+    static constexpr int WAVES_M = WGM / SPLIT;      // wave rows per workgroup
     static constexpr int WAVES_N = WGN;
-    static constexpr int WAVES = WGM * WGN;
+    static constexpr int WAVES = WAVES_M * WGN;
     static constexpr int THREADS = 64 * WAVES;
     static constexpr int KSTEPS = SB / 32;
     static constexpr int CHUNKS = SB / 16;           // 16-byte chunks per LDS row
@@ -72,6 +77,7 @@ struct Geo {
     static constexpr int EP_BYTES = EP_TERMS_OFF + WAVES * WTM * 16;
     static_assert(EP_BYTES <= LDS_BYTES, "epilogue LDS");
     static_assert(LOADS >= 1 && NSTAGE >= 2, "stage geometry");
+    static_assert(WGM % SPLIT == 0, "split geometry");
     static_assert(ROWS_PER_IT >= 1 && 32 % ROWS_PER_IT == 0, "epilogue geometry");
 
     // Physical 16-byte chunk of logical chunk c of an LDS row.  The
@@ -87,11 +93,15 @@ struct Geo {
 };
 
 // Kernel configurations: (tile, stage bytes, ring depth, waves down, waves
-// across).  256-tiles: 8 waves of 128 x 64 (two per SIMD, 0.75 KiB of LDS
-// operand reads per MFMA) over a 4-deep ring of 64-byte stages, or the
-// earlier 4 waves of 128 x 128 (one per SIMD, 2-deep ring of 128 bytes).
+// across[, workgroups per tile]).  256-tiles with operands in LDS: 8 waves
+// of 128 x 64 (two per SIMD, 0.75 KiB of LDS operand reads per MFMA) over a
+// 4-deep ring of 64-byte stages, or the earlier 4 waves of 128 x 128 (one
+// per SIMD, 2-deep ring of 128 bytes).  Direct operands: the same 8 waves
+// of 128 x 64 as two workgroups of 4 (L256W8H; no ring, the stage fields
+// only size the unused ring geometry).
 #define QLD_CFG_L256 256, 128, 2, 2, 2
 #define QLD_CFG_L256W8 256, 64, 4, 2, 4
+#define QLD_CFG_L256W8H 256, 64, 4, 2, 4, 2
 #define QLD_CFG_S128 128, 128, 2, 2, 2
 #define QLD_CFG_S128R4 128, 64, 4, 2, 2
 
@@ -621,17 +631,18 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     int g = 0;
     for (; g + 2 <= full; g += 2) {
-        // The scheduling barriers keep each group's loads a whole compute
-        // ahead of their use (left alone, the scheduler sinks them next to
-        // the first use to save registers, and the waits then stall).
+        // Split tiles (short rows): scheduling barriers keep each group's
+        // loads a whole compute ahead of their use (left alone, the
+        // scheduler sinks them next to the first use).  With long rows the
+        // compiler's own placement measured faster.
         fetch(qa, qb, g + 1);
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
         compute(pa, pb);
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
         fetch(pa, pb, g + 2);
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
         compute(qa, qb);
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
     }
     if (g < full) compute(pa, pb);
 }
@@ -768,6 +779,7 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
 
+    static_assert(G::SPLIT == 1 || PACKED == 2, "split tiles need direct operands");
     if constexpr (PACKED == 2) {
         kloop_regs<G>(d, i0, j0, wm, wn, acc);   // no LDS ring: epilogue regions are per wave
     } else {
@@ -791,27 +803,30 @@ __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, int wm, in
     }
 }
 
-// One workgroup per item (PERSIST == 0; workgroup b takes item b) or a
+// One workgroup per unit (PERSIST == 0; workgroup b takes unit b) or a
 // fixed grid draining eight per-XCD queues: workgroup b starts on queue
-// b % 8 -- the XCD it runs on -- whose items are a contiguous range of the
-// list, then steals from the other queues.  Every claim advances a head, so
-// the loop ends once all eight ranges are exhausted.
+// b % 8 -- the XCD it runs on -- whose units come from a contiguous range of
+// the item list, then steals from the other queues.  Every claim advances a
+// head, so the loop ends once all eight ranges are exhausted.  A unit is
+// item u / SPLIT, wave rows (u % SPLIT) * WAVES_M onwards.
 template <class OP, int MODE, class G, int PERSIST, int PACKED>
-__global__ __launch_bounds__(G::THREADS, 1) void ld_mfma_kernel(KArgs p)
+__global__ __launch_bounds__(G::THREADS, G::SPLIT) void ld_mfma_kernel(KArgs p)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     const int wave = wave_id();
+    const int wr = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     if (!PERSIST) {
-        run_item<OP, MODE, G, PACKED>(p, (int64_t)blockIdx.x, wave / G::WAVES_N,
-                                      wave % G::WAVES_N, wave, smem);
+        const int64_t u = (int64_t)blockIdx.x;
+        run_item<OP, MODE, G, PACKED>(p, u / G::SPLIT, (int)(u % G::SPLIT) * G::WAVES_M + wr,
+                                      wn, wave, smem);
         return;
     }
     __shared__ int64_t claimed;
     const int x0 = (int)(blockIdx.x % QLD_NUM_XCD);
     for (int q = 0; q < QLD_NUM_XCD; ++q) {
         const int x = (x0 + q) % QLD_NUM_XCD;
-        const int64_t begin = p.qbeg[x];
-        const int64_t end = p.qbeg[x + 1];
+        const int64_t begin = p.qbeg[x] * G::SPLIT;
+        const int64_t end = p.qbeg[x + 1] * G::SPLIT;
         for (;;) {
             if (threadIdx.x == 0) {
                 int64_t k = (int64_t)atomicAdd(&p.queue[x], 1u);
@@ -824,7 +839,8 @@ __global__ __launch_bounds__(G::THREADS, 1) void ld_mfma_kernel(KArgs p)
                           (uint32_t)__builtin_amdgcn_readfirstlane((int)c));
             __syncthreads();
             if (idx < 0) break;
-            run_item<OP, MODE, G, PACKED>(p, idx, wave / G::WAVES_N, wave % G::WAVES_N, wave,
+            run_item<OP, MODE, G, PACKED>(p, idx / G::SPLIT,
+                                          (int)(idx % G::SPLIT) * G::WAVES_M + wr, wn, wave,
                                           smem);
         }
     }
@@ -1026,12 +1042,24 @@ static hipError_t set_lds_limit()
 
 typedef Geo<QLD_CFG_L256> GeoL;
 typedef Geo<QLD_CFG_L256W8> GeoL8;
+typedef Geo<QLD_CFG_L256W8H> GeoL8H;
 typedef Geo<QLD_CFG_S128> GeoS;
 typedef Geo<QLD_CFG_S128R4> GeoS4;
 
-// 256-tile variant: 8 waves (default) or 4 (QLD_TILE256_WAVES=4 in the
-// environment), read once when the first context is created.
+// 256-tile variant for operands in LDS: 8 waves of 128 x 64 (default) or 4
+// of 128 x 128 (QLD_TILE256_WAVES=4 in the environment), read once when the
+// first context is created.
 static int g_l256_waves = 8;
+
+// Direct operands, 256-tiles: rows of at most this many packed words (about
+// 41,000 samples) use two workgroups of 4 waves per tile, two per CU, with
+// each group's loads held a whole compute ahead (L256W8H: the r^2 epilogue
+// is a large share of a short k loop, and one workgroup's epilogue overlaps
+// the other's MFMAs); longer rows use one workgroup of 8 waves and the
+// compiler's own load placement (L256W8).  Measured on MI355X: 16,622
+// samples 3.02 vs 3.21 ms (30,000-SNP triangle), 100,000 samples 4.84 vs
+// 4.10 ms (8,000 x 12,000 rectangle).
+#define QLD_SPLIT_MAX_WORDS 640
 // 128-tile ring: two 128-byte stages (default) or four 64-byte stages
 // (QLD_TILE128_RING=4).
 static int g_s128_ring = 2;
@@ -1058,9 +1086,9 @@ static hipError_t set_lds_limits_packed()
     if ((e = set_lds_limit<FP4, MODE, GeoS, 1, 1>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoS, 0, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoS, 1, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoL8H, 0, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoL8H, 1, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoL8, 0, 2>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<FP4, MODE, GeoL, 0, 2>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<FP4, MODE, GeoL, 1, 2>()) != hipSuccess) return e;
     return set_lds_limit<FP4, MODE, GeoL8, 1, 2>();
 }
 
@@ -1093,8 +1121,8 @@ template <class OP, int MODE, class G, int PACKED = 0>
 static hipError_t launch_mode(const KArgs &a, int persistent, hipStream_t s)
 {
     if (persistent > 0)
-        return launch_one<OP, MODE, G, 1, PACKED>(a, (unsigned)persistent, s);
-    return launch_one<OP, MODE, G, 0, PACKED>(a, (unsigned)a.nitems, s);
+        return launch_one<OP, MODE, G, 1, PACKED>(a, (unsigned)persistent * G::SPLIT, s);
+    return launch_one<OP, MODE, G, 0, PACKED>(a, (unsigned)(a.nitems * G::SPLIT), s);
 }
 
 template <class OP, int MODE>
@@ -1103,15 +1131,15 @@ static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStrea
     if (tile == QLD_TILE_SMALL)
         return g_s128_ring == 4 ? launch_mode<OP, MODE, GeoS4>(a, persistent, s)
                                 : launch_mode<OP, MODE, GeoS>(a, persistent, s);
-    if (g_l256_waves == 8) return launch_mode<OP, MODE, GeoL8>(a, persistent, s);
-    return launch_mode<OP, MODE, GeoL>(a, persistent, s);
+    if (g_l256_waves == 4) return launch_mode<OP, MODE, GeoL>(a, persistent, s);
+    return launch_mode<OP, MODE, GeoL8>(a, persistent, s);
 }
 
 hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              const WorkDesc *descs, const WorkDesc &inline_desc,
                              const WorkItem *items, int64_t nitems,
                              int persistent, uint32_t *queue, const int64_t *qbeg,
-                             hipStream_t s)
+                             hipStream_t s, int words)
 {
     if (nitems <= 0) return hipSuccess;
     KArgs a;
@@ -1134,13 +1162,14 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
             return r2_mode == QLD_R2_EXACT
                        ? launch_mode<FP4, QLD_R2_EXACT, GeoS, 2>(a, persistent, s)
                        : launch_mode<FP4, QLD_R2_REFERENCE, GeoS, 2>(a, persistent, s);
-        if (g_l256_waves == 4)
+        if (words <= 0 && !descs) words = inline_desc.words;
+        if (words > QLD_SPLIT_MAX_WORDS)
             return r2_mode == QLD_R2_EXACT
-                       ? launch_mode<FP4, QLD_R2_EXACT, GeoL, 2>(a, persistent, s)
-                       : launch_mode<FP4, QLD_R2_REFERENCE, GeoL, 2>(a, persistent, s);
+                       ? launch_mode<FP4, QLD_R2_EXACT, GeoL8, 2>(a, persistent, s)
+                       : launch_mode<FP4, QLD_R2_REFERENCE, GeoL8, 2>(a, persistent, s);
         return r2_mode == QLD_R2_EXACT
-                   ? launch_mode<FP4, QLD_R2_EXACT, GeoL8, 2>(a, persistent, s)
-                   : launch_mode<FP4, QLD_R2_REFERENCE, GeoL8, 2>(a, persistent, s);
+                   ? launch_mode<FP4, QLD_R2_EXACT, GeoL8H, 2>(a, persistent, s)
+                   : launch_mode<FP4, QLD_R2_REFERENCE, GeoL8H, 2>(a, persistent, s);
     }
     if (packed) {
         if (engine != QLD_ENGINE_MFMA_FP4 || tile != QLD_TILE_SMALL) return hipErrorInvalidValue;
