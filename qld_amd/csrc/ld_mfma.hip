@@ -1,8 +1,11 @@
 // Matrix-core engines: the pair counts as a 0/1 matrix product on MFMA.
 //
-// Each packed haplotype bit is first expanded to one matrix element:
+// Each packed haplotype bit becomes one matrix element:
 //   FP4 engine : e2m1 nibble 0x2 (= 1.0) or 0x0, two samples per byte,
-//                v_mfma_scale_f32_32x32x64_f8f6f4 (scale 2^0), fp32 sums
+//                v_mfma_scale_f32_32x32x64_f8f6f4 (scale 2^0), fp32 sums;
+//                with direct operands the packed dwords are split into bit
+//                planes in registers instead (fp4_plane: nibbles 0.5, 1 or
+//                2, brought back to products of 1.0 by the MFMA's scale)
 //   I8 engine  : byte 0x01 or 0x00, v_mfma_i32_32x32x32_i8, int32 sums
 // (On gfx950 the 32x32 forms retire more work per cycle than the 16x16
 // ones: measured 8.1 vs 4.8 PFLOP/s for fp4, 3.8 vs 2.9 POP/s for int8.)
@@ -11,7 +14,8 @@
 // The pairing of samples to K positions only has to be the same for every
 // row; the expansion keeps each 64-bit word's samples together.
 //
-// Work decomposition (both engines), tile edge T = 256 or 128
+// Work decomposition (both engines), tile edge T = 256 or 128 (the direct
+// k loop and the 256-tile configurations below refine this)
 //   workgroup : 4 wave64 (2 x 2), one T x T tile of pairs of one job
 //   wave      : T/2 x T/2 = (T/64)^2 MFMA blocks of 32 x 32 (256 or 64
 //               fp32/int32 accumulators per lane)
