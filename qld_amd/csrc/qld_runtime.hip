@@ -330,9 +330,9 @@ static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // Tile edge for one plain job.  Expanded operands: 128 x 128 tiles (two
 // workgroups per CU) measured faster than 256 x 256 (3.47 vs 4.03 ms on the
-// 30,000-SNP triangle).  Direct fp4 operands: 256-tiles of 8 waves once both
-// sides reach the tile cutoff (2.91 vs 3.22 ms on the same triangle), 128
-// below it.
+// 30,000-SNP triangle).  Direct fp4 operands: 256-tiles once both sides
+// reach the tile cutoff (2.91 vs 3.22 ms on the same triangle with 8-wave
+// workgroups; the launcher splits them further for short rows), 128 below.
 static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job, int pmode)
 {
     if (ctx->plain_tile) return ctx->plain_tile;
@@ -398,10 +398,10 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
         fill_work_desc(d, XB, XA, job.bcB, job.bcA, job.n, job.m, rb, job.tri, job.snp_size,
                        job.res, job.ldr, job.cnt, job.ldc);
     }
-    // Persistent launch: two workgroups per CU (128-tiles) or one (256)
-    // drain per-XCD queues over equal shares of the tile list; measured
-    // 3.47 ms vs 4.11 ms for one workgroup per tile on the 30,000-SNP
-    // triangle.
+    // Persistent launch: two workgroups per CU (128-tiles) or one per CU
+    // per tile share (256; the launcher doubles it for split tiles) drain
+    // per-XCD queues over equal shares of the tile list; measured 3.47 ms
+    // vs 4.11 ms for one workgroup per tile on the 30,000-SNP triangle.
     if ((rc = ctx->queue.reserve(ctx, 16 * sizeof(uint32_t)))) return rc;
     const int64_t ntiles = (int64_t)ctx->tiles_host.size();
     int64_t grid = (int64_t)ctx->num_cu * (T == QLD_TILE_SMALL ? 2 : 1);
