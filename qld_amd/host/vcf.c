@@ -173,24 +173,31 @@ int qld_vcf_sample_mask(const char *chrom_line, const char *list_file, int ploid
         FILE *fp = fopen(list_file, "r");
         if (!fp) return vfail(QLD_ERR_NOTFOUND, "cannot open sample list %s", list_file);
         char w[1024];
-        while (fscanf(fp, "%1023s", w) == 1) {
+        int oom = 0;
+        while (!oom && fscanf(fp, "%1023s", w) == 1) {
             char **nn = (char **)realloc(names, (size_t)(nnames + 1) * sizeof(char *));
-            if (!nn) { fclose(fp); return vfail(QLD_ERR_NOMEM, "out of host memory"); }
-            names = nn;
-            names[nnames++] = strdup(w);
+            if (nn) names = nn;
+            char *d = nn ? strdup(w) : NULL;
+            if (d) names[nnames++] = d;
+            else oom = 1;
         }
         fclose(fp);
+        if (oom) {
+            for (int k = 0; k < nnames; k++) free(names[k]);
+            free(names);
+            return vfail(QLD_ERR_NOMEM, "out of host memory");
+        }
     }
     size_t cap = strlen(chrom_line) + 1;
     char *w = (char *)malloc(cap);
     unsigned char *m = NULL;
-    int n = 0, valid = 0, eol, fields = 0;
+    int n = 0, valid = 0, eol, fields = 0, oom = !w;
     char *p = (char *)chrom_line;
-    while (vword(&p, w, cap, &eol))
+    while (!oom && vword(&p, w, cap, &eol))
         if (++fields >= 9) break;
-    while (vword(&p, w, cap, &eol)) {
+    while (!oom && vword(&p, w, cap, &eol)) {
         unsigned char *nm = (unsigned char *)realloc(m, (size_t)(n + 1));
-        if (!nm) break;
+        if (!nm) { oom = 1; break; }
         m = nm;
         int ok = nnames == 0;
         for (int k = 0; k < nnames && !ok; k++) ok = !strcmp(names[k], w);
@@ -201,6 +208,10 @@ int qld_vcf_sample_mask(const char *chrom_line, const char *list_file, int ploid
     free(w);
     for (int k = 0; k < nnames; k++) free(names[k]);
     free(names);
+    if (oom) {
+        free(m);
+        return vfail(QLD_ERR_NOMEM, "out of host memory");
+    }
     *mask = m;
     *nsamples = n;
     *valid_count = valid;
@@ -550,9 +561,11 @@ static int vtab_push(vtab *t, int pos, const char *id, const char *snip, int n)
         w[i / 64] |= bit << (63 - i % 64);
         bc += (uint32_t)bit;
     }
+    char *idc = strdup(id);
+    if (!idc) return vfail(QLD_ERR_NOMEM, "out of host memory");
     t->pos[t->rows] = (uint32_t)pos;
     t->bc[t->rows] = bc;
-    t->id[t->rows] = strdup(id);
+    t->id[t->rows] = idc;
     t->rows++;
     return QLD_OK;
 }
@@ -716,13 +729,16 @@ int qld_vcf_read_table(qld_vcf_reader *r, char *const *files, int nfiles, int po
         for (int i = 0; i < t.rows; i++) idtot += strlen(t.id[i]) + 1;
         out->rows = t.rows;
         out->words = t.words;
-        out->snp = (uint64_t *)realloc(t.snp, nr * (size_t)t.words * 8);
-        out->pos = (uint32_t *)realloc(t.pos, nr * 4);
-        out->bc = (uint32_t *)realloc(t.bc, nr * 4);
+        /* shrink (or, for an empty table, allocate) to the final size; a
+         * failed realloc leaves the block with t, which is freed below */
+        uint64_t *snp = (uint64_t *)realloc(t.snp, nr * (size_t)t.words * 8);
+        if (snp) { out->snp = snp; t.snp = NULL; }
+        uint32_t *pos = (uint32_t *)realloc(t.pos, nr * 4);
+        if (pos) { out->pos = pos; t.pos = NULL; }
+        uint32_t *bc = (uint32_t *)realloc(t.bc, nr * 4);
+        if (bc) { out->bc = bc; t.bc = NULL; }
         out->id = (char **)malloc(nr * sizeof(char *));
         out->idbuf = (char *)malloc(idtot ? idtot : 1);
-        t.snp = NULL;
-        t.pos = t.bc = NULL;
         if (!out->snp || !out->pos || !out->bc || !out->id || !out->idbuf) {
             rc = vfail(QLD_ERR_NOMEM, "out of host memory");
         } else {
