@@ -31,10 +31,15 @@ struct WorkDesc {
     float pad_;
     double dN;
     double invN;
-    // Packed-operand launches (fp4 expanded inside the kernel): the packed
-    // rows themselves, `words` uint64 per row.
+    // Packed-operand launches (fp4 expanded inside the kernel), `words`
+    // uint64 per row.  Packed-into-LDS launches: PI/PJ are the job's packed
+    // rows, row-major.  Direct launches: PI/PJ are row-block-tiled tables
+    // (see launch_tile_rows) of rowsI/rowsJ rows whose rows rowI0/rowJ0
+    // onwards are the job's rows.
     const uint64_t *PI;
     const uint64_t *PJ;
+    int64_t rowI0, rowJ0;
+    int rowsI, rowsJ;
     int words;
     int pad2_;
 };
@@ -52,6 +57,22 @@ void fill_work_desc(WorkDesc &d, const uint8_t *XI, const uint8_t *XJ,
 
 // Packed-operand variant: the same job read straight from packed rows.
 void set_packed_operands(WorkDesc &d, const uint64_t *PI, const uint64_t *PJ, int words);
+
+// Direct-operand variant: rows rowI0.. of tiled table TI (rowsI rows) and
+// rows rowJ0.. of tiled table TJ (rowsJ rows).
+void set_tiled_operands(WorkDesc &d, const uint64_t *TI, int64_t rowI0, int rowsI,
+                        const uint64_t *TJ, int64_t rowJ0, int rowsJ, int words);
+
+// Row-block-tiled copy of a packed table, the layout the direct kernels
+// read: for each block of 32 rows and each group of 4 words, the 32 rows'
+// 32-byte pieces are contiguous (1 KiB), i.e. uint64 index
+//   ((row/32 * G + word/4) * 32 + row%32) * 4 + word%4,   G = ceil(words/4),
+// with rows past `rows` and words past `words` zero.  One wave's load of a
+// 32-row MFMA block is then 1 KiB of consecutive bytes instead of 32 pieces
+// 8*words bytes apart.
+int64_t tiled_bytes(int rows, int words);
+hipError_t launch_tile_rows(const uint64_t *P, int rows, int words, uint64_t *T,
+                            hipStream_t s);
 
 // Packed words per row after padding to whole 128-byte k stages.
 int mfma_padded_words(int engine, int words);

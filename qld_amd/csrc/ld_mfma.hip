@@ -539,44 +539,47 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     const int lane = threadIdx.x & 63;
     const int r32 = lane & 31, h = lane >> 5;
     const int W = d.words;
-    const int64_t rowb = (int64_t)W * 8;
-    const int a0 = i0 + wm * WTM, b0 = j0 + wn * WTN;
-    auto rsrc = [&](const uint64_t *T, int r0, int n, int rows) {
-        int64_t have = (int64_t)(n - r0);
-        have = have < 0 ? 0 : (have > rows ? rows : have);
-        const uint64_t base = wave_uniform((uint64_t)(uintptr_t)(T + (int64_t)r0 * W));
-        return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)base, 0,
-                                                 wave_uniform((int)(have * rowb)), 0x00020000);
-    };
-    const __amdgpu_buffer_rsrc_t rsA = rsrc(d.PI, a0, d.ni, WTM);
-    const __amdgpu_buffer_rsrc_t rsB = rsrc(d.PJ, b0, d.nj, WTN);
-    const uint32_t offA = (uint32_t)(r32 * rowb + 16 * h);
-    const uint32_t offB = offA;
-    const uint32_t stepA = (uint32_t)(32 * rowb);      // next 32-row block
     const int groups = (W + 3) / 4;
+    const uint32_t blk = (uint32_t)groups * 1024u;     // bytes per 32-row block
+    const int a0 = i0 + wm * WTM, b0 = j0 + wn * WTN;
+    // Operand rows [r0, r0 + rows) of a tiled table of tab_rows rows: a
+    // buffer descriptor over the 32-row blocks they touch (never past the
+    // table: anything beyond reads as zero), and each lane's byte offset in
+    // it for its row of 32-row MFMA block q (rows need not start on a block).
+    auto rsrc = [&](const uint64_t *T, int64_t r0, int tab_rows, int rows) {
+        const int64_t first = r0 >> 5;
+        int64_t last = (r0 + rows + 31) >> 5;
+        const int64_t tab_blocks = ((int64_t)tab_rows + 31) >> 5;
+        if (last > tab_blocks) last = tab_blocks;
+        const int64_t nb = last > first ? last - first : 0;
+        const uint64_t base =
+            wave_uniform((uint64_t)(uintptr_t)((const char *)T + first * (int64_t)blk));
+        return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)base, 0,
+                                                 wave_uniform((int)(nb * blk)), 0x00020000);
+    };
+    auto lane_off = [&](int64_t r0, int q) {
+        const int rel = (int)(r0 & 31) + 32 * q + r32;
+        return (uint32_t)(rel >> 5) * blk + (uint32_t)(rel & 31) * 32u + 16u * (uint32_t)h;
+    };
+    const int64_t rA = d.rowI0 + a0, rB = d.rowJ0 + b0;
+    const __amdgpu_buffer_rsrc_t rsA = rsrc(d.PI, rA, d.rowsI, WTM);
+    const __amdgpu_buffer_rsrc_t rsB = rsrc(d.PJ, rB, d.rowsJ, WTN);
+    uint32_t offA[NBM], offB[NBN];
+#pragma unroll
+    for (int a = 0; a < NBM; ++a) offA[a] = lane_off(rA, a);
+#pragma unroll
+    for (int b = 0; b < NBN; ++b) offB[b] = lane_off(rB, b);
 
+    // Group g (4 words = 256 samples of every row) of the wave's blocks;
+    // the tiled table is zero past each row's last word, so no masking.
     auto fetch = [&](v4i (&xa)[NBM], v4i (&xb)[NBN], int g) {
-        const uint32_t k = (uint32_t)g * 32u;
+        const uint32_t k = (uint32_t)g * 1024u;
 #pragma unroll
         for (int a = 0; a < NBM; ++a)
-            xa[a] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsA, offA + a * stepA + k, 0, 0);
+            xa[a] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsA, offA[a] + k, 0, 0);
 #pragma unroll
         for (int b = 0; b < NBN; ++b)
-            xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB + b * stepA + k, 0, 0);
-    };
-    // Words past the row end (only in group W/4, when W % 4 != 0) are zeroed.
-    auto mask_tail = [&](v4i (&xa)[NBM], v4i (&xb)[NBN], int g) {
-        const bool w0 = 4 * g + 2 * h < W, w1 = 4 * g + 2 * h + 1 < W;
-#pragma unroll
-        for (int a = 0; a < NBM; ++a) {
-            if (!w0) xa[a][0] = xa[a][1] = 0;
-            if (!w1) xa[a][2] = xa[a][3] = 0;
-        }
-#pragma unroll
-        for (int b = 0; b < NBN; ++b) {
-            if (!w0) xb[b][0] = xb[b][1] = 0;
-            if (!w1) xb[b][2] = xb[b][3] = 0;
-        }
+            xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB[b] + k, 0, 0);
     };
 #if QLD_FP4_PLANES
     auto plane = [&](auto pc, const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
@@ -615,17 +618,11 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     };
 #endif
     // Two register sets: group g+1 in flight while group g computes.  The
-    // partial group W/4 (W % 4 != 0) goes first, masked; the loop's loads
-    // are then unconditional, so it carries no register copies (a group read
-    // past the last full one is either the next row's words or, past the
-    // block, zeros through the descriptor bound, and is never computed).
-    const int full = W >> 2;
+    // loads in the loop are unconditional, so it carries no register copies
+    // (the group read past the last is beyond the rows' blocks or the
+    // descriptor, and is never computed).
+    const int full = groups;
     v4i pa[NBM], pb[NBN], qa[NBM], qb[NBN];
-    if (full < groups) {
-        fetch(pa, pb, full);
-        mask_tail(pa, pb, full);
-        compute(pa, pb);
-    }
     fetch(pa, pb, 0);
     // Drain every outstanding memory operation once here (the previous
     // tile's r^2 stores among them).  The compiler treats loads and stores
@@ -943,6 +940,8 @@ void fill_work_desc(WorkDesc &d, const uint8_t *XI, const uint8_t *XJ,
     d.invN = e.invN;
     d.PI = nullptr;
     d.PJ = nullptr;
+    d.rowI0 = d.rowJ0 = 0;
+    d.rowsI = d.rowsJ = 0;
     d.words = 0;
     d.pad2_ = 0;
 }
@@ -951,7 +950,27 @@ void set_packed_operands(WorkDesc &d, const uint64_t *PI, const uint64_t *PJ, in
 {
     d.PI = PI;
     d.PJ = PJ;
+    d.rowI0 = d.rowJ0 = 0;
+    d.rowsI = d.ni;
+    d.rowsJ = d.nj;
     d.words = words;
+}
+
+void set_tiled_operands(WorkDesc &d, const uint64_t *TI, int64_t rowI0, int rowsI,
+                        const uint64_t *TJ, int64_t rowJ0, int rowsJ, int words)
+{
+    d.PI = TI;
+    d.PJ = TJ;
+    d.rowI0 = rowI0;
+    d.rowJ0 = rowJ0;
+    d.rowsI = rowsI;
+    d.rowsJ = rowsJ;
+    d.words = words;
+}
+
+int64_t tiled_bytes(int rows, int words)
+{
+    return (int64_t)((rows + 31) / 32) * ((words + 3) / 4) * 1024;
 }
 
 size_t job_tile_count(int ni, int nj, int tri, int T)
@@ -1022,6 +1041,40 @@ hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
     else
         hipLaunchKernelGGL(expand_i8_kernel, dim3(blocks), dim3(256), 0, s, T, rows,
                            words, Wp, X, rb);
+    return hipGetLastError();
+}
+
+// One thread per 32-byte piece (row r, word group g) of the tiled table:
+// consecutive threads write consecutive pieces; each reads its row's four
+// words (zero past the row or the table).
+__global__ void tile_rows_kernel(const uint64_t *P, int rows, int W, int groups,
+                                 int64_t pieces, uint64_t *T)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pieces) return;
+    const int r = (int)(t & 31);
+    const int64_t bg = t >> 5;
+    const int g = (int)(bg % groups);
+    const int row = (int)(bg / groups) * 32 + r;
+    uint64_t v[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int word = 4 * g + w;
+        v[w] = (row < rows && word < W) ? P[(int64_t)row * W + word] : 0ull;
+    }
+    ulonglong2 *dst = (ulonglong2 *)(T + t * 4);
+    dst[0] = make_ulonglong2(v[0], v[1]);
+    dst[1] = make_ulonglong2(v[2], v[3]);
+}
+
+hipError_t launch_tile_rows(const uint64_t *P, int rows, int words, uint64_t *T,
+                            hipStream_t s)
+{
+    if (rows <= 0 || words <= 0) return hipSuccess;
+    const int groups = (words + 3) / 4;
+    const int64_t pieces = (int64_t)((rows + 31) / 32) * groups * 32;
+    hipLaunchKernelGGL(tile_rows_kernel, dim3((unsigned)((pieces + 255) / 256)), dim3(256), 0, s,
+                       P, rows, words, groups, pieces, T);
     return hipGetLastError();
 }
 

@@ -81,11 +81,12 @@ static int table_create(qld_ctx *ctx, const void *rows_src, const void *bc_src,
     t->words = words;
     t->snp_size = snp_size;
     t->engine = ctx->cfg.engine == QLD_ENGINE_MFMA_I8 ? QLD_ENGINE_MFMA_I8 : QLD_ENGINE_MFMA_FP4;
-    // fp4 tables in direct mode keep only the packed rows (the kernels
-    // expand bits in registers); otherwise an expanded operand copy too.
+    // fp4 tables in direct mode keep only a row-block-tiled copy of the
+    // packed rows (the kernels read it straight into registers); otherwise
+    // the packed rows and an expanded operand copy.
     t->direct = t->engine == QLD_ENGINE_MFMA_FP4 && ctx->fp4_packed == QLD_FP4_DIRECT;
     t->row_bytes = t->direct ? 0 : mfma_row_bytes(t->engine, words);
-    t->bytes_packed = (size_t)rows * words * 8;
+    t->bytes_packed = t->direct ? (size_t)tiled_bytes(rows, words) : (size_t)rows * words * 8;
     t->bytes_bc = (size_t)rows * 4;
     t->bytes_x = (size_t)rows * t->row_bytes;
     int rc;
@@ -95,12 +96,29 @@ static int table_create(qld_ctx *ctx, const void *rows_src, const void *bc_src,
         table_release(ctx, t);
         return rc;
     }
-    hipError_t e = hipMemcpyAsync(t->packed, rows_src, t->bytes_packed, kind, ctx->stream);
+    hipError_t e = hipSuccess;
+    uint64_t *staged = nullptr;           // host rows staged on the device for tiling
+    const size_t row_major = (size_t)rows * words * 8;
+    if (t->direct) {
+        const uint64_t *src = (const uint64_t *)rows_src;
+        if (kind == hipMemcpyHostToDevice) {
+            if ((rc = qld_ctx_malloc(ctx, row_major, (void **)&staged))) {
+                table_release(ctx, t);
+                return rc;
+            }
+            e = hipMemcpyAsync(staged, rows_src, row_major, kind, ctx->stream);
+            src = staged;
+        }
+        if (e == hipSuccess) e = launch_tile_rows(src, rows, words, t->packed, ctx->stream);
+    } else {
+        e = hipMemcpyAsync(t->packed, rows_src, t->bytes_packed, kind, ctx->stream);
+    }
     if (e == hipSuccess)
         e = hipMemcpyAsync(t->bc, bc_src, t->bytes_bc, kind, ctx->stream);
     if (e == hipSuccess && !t->direct)
         e = launch_expand(t->engine, t->packed, rows, words, t->X, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (staged) qld_ctx_mfree(ctx, staged, row_major);
     if (e != hipSuccess) {
         table_release(ctx, t);
         return qld_hip_fail(ctx, e, "qld_table_create");
@@ -273,8 +291,8 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
             fill_work_desc(d, nullptr, nullptr, t->bc + w.b_start, t->bc + w.a_start, w.b_size,
                            w.a_size, 0, w.tri, t->snp_size, out + w.out_offset, w.a_size,
                            nullptr, w.a_size);
-            set_packed_operands(d, t->packed + (int64_t)w.b_start * t->words,
-                                t->packed + (int64_t)w.a_start * t->words, t->words);
+            set_tiled_operands(d, t->packed, w.b_start, t->rows, t->packed, w.a_start,
+                               t->rows, t->words);
         } else {
             const uint8_t *XI = t->X + (int64_t)w.b_start * t->row_bytes;
             const uint8_t *XJ = t->X + (int64_t)w.a_start * t->row_bytes;

@@ -48,7 +48,8 @@ struct qld_ctx {
 // A resident table: packed rows, bit counts and the expanded MFMA operand.
 struct qld_table {
     int rows = 0, words = 0, snp_size = 0, engine = QLD_ENGINE_MFMA_FP4;
-    int direct = 0;           // fp4 kernels read `packed` into registers; no X
+    int direct = 0;           // `packed` is the row-block-tiled copy the fp4
+                              // kernels read into registers; no X
     int64_t row_bytes = 0;
     uint64_t *packed = nullptr;
     uint32_t *bc = nullptr;

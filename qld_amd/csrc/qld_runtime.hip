@@ -378,7 +378,23 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     if (rc) return rc;
     hipError_t e = hipSuccess;
     WorkDesc d;
-    if (packed) {
+    if (pmode == QLD_FP4_DIRECT) {
+        // Job rows i come from table B, columns j from table A, both read
+        // from row-block-tiled copies in the workspace (made per call: the
+        // caller's rows may change between calls).
+        const bool same = job.A == job.B && job.m == job.n;
+        const size_t bytesA = align256((size_t)tiled_bytes(job.m, job.words));
+        const size_t bytesB = same ? 0 : align256((size_t)tiled_bytes(job.n, job.words));
+        if ((rc = ctx->ws.reserve(ctx, bytesA + bytesB))) return rc;
+        uint64_t *TA = (uint64_t *)ctx->ws.ptr;
+        uint64_t *TB = same ? TA : (uint64_t *)((char *)ctx->ws.ptr + bytesA);
+        e = launch_tile_rows(job.A, job.m, job.words, TA, ctx->stream);
+        if (e == hipSuccess && !same) e = launch_tile_rows(job.B, job.n, job.words, TB, ctx->stream);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "row tiling kernel");
+        fill_work_desc(d, nullptr, nullptr, job.bcB, job.bcA, job.n, job.m, 0, job.tri,
+                       job.snp_size, job.res, job.ldr, job.cnt, job.ldc);
+        set_tiled_operands(d, TB, 0, job.n, TA, 0, job.m, job.words);
+    } else if (packed) {
         // Job rows i come from table B, columns j from table A.
         fill_work_desc(d, nullptr, nullptr, job.bcB, job.bcA, job.n, job.m, 0, job.tri,
                        job.snp_size, job.res, job.ldr, job.cnt, job.ldc);

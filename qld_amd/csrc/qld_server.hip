@@ -167,8 +167,8 @@ static int launch_batch(qld_server *s, Slot &sl)
             if (t->direct) {
                 fill_work_desc(hd[k], nullptr, nullptr, bc, bc, r.size, r.size, 0, 1,
                                t->snp_size, out, r.size, nullptr, r.size);
-                const uint64_t *P = t->packed + (int64_t)r.start * t->words;
-                set_packed_operands(hd[k], P, P, t->words);
+                set_tiled_operands(hd[k], t->packed, r.start, t->rows, t->packed, r.start,
+                                   t->rows, t->words);
             } else {
                 const uint8_t *X = t->X + (int64_t)r.start * t->row_bytes;
                 fill_work_desc(hd[k], X, X, bc, bc, r.size, r.size, t->row_bytes, 1,
