@@ -110,11 +110,8 @@ hipError_t mfma_init_attributes();
 //               x owns items [qbeg[x], qbeg[x+1]) (qbeg NULL: equal shares)
 //   packed      nonzero: fp4 engine reading PI/PJ packed rows (no expanded
 //               operand), 128-tiles only
-//   words       direct operands: the longest row (packed words) of the
-//               launch's jobs, which picks the 256-tile schedule (0: read it
-//               from inline_desc when descs is null, else take the short one)
 hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              const WorkDesc *descs, const WorkDesc &inline_desc,
                              const WorkItem *items, int64_t nitems,
                              int persistent, uint32_t *queue, const int64_t *qbeg,
-                             hipStream_t s, int words = 0);
+                             hipStream_t s);

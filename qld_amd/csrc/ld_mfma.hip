@@ -632,10 +632,11 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     int g = 0;
     for (; g + 2 <= full; g += 2) {
-        // Split tiles (short rows): scheduling barriers keep each group's
-        // loads a whole compute ahead of their use (left alone, the
-        // scheduler sinks them next to the first use).  With long rows the
-        // compiler's own placement measured faster.
+        // Split 256-tiles: scheduling barriers keep each group's loads a
+        // whole compute ahead of their use (left alone, the scheduler sinks
+        // them next to the first use; measured 2.58 vs 2.79 ms on the
+        // 30,000-SNP triangle).  The 128-tile kernel keeps the compiler's
+        // placement.
         fetch(qa, qb, g + 1);
         if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
         compute(pa, pb);
@@ -1108,15 +1109,6 @@ typedef Geo<QLD_CFG_S128R4> GeoS4;
 // first context is created.
 static int g_l256_waves = 8;
 
-// Direct operands, 256-tiles: rows of at most this many packed words (about
-// 41,000 samples) use two workgroups of 4 waves per tile, two per CU, with
-// each group's loads held a whole compute ahead (L256W8H: the r^2 epilogue
-// is a large share of a short k loop, and one workgroup's epilogue overlaps
-// the other's MFMAs); longer rows use one workgroup of 8 waves and the
-// compiler's own load placement (L256W8).  Measured on MI355X: 16,622
-// samples 3.02 vs 3.21 ms (30,000-SNP triangle), 100,000 samples 4.84 vs
-// 4.10 ms (8,000 x 12,000 rectangle).
-#define QLD_SPLIT_MAX_WORDS 640
 // 128-tile ring: two 128-byte stages (default) or four 64-byte stages
 // (QLD_TILE128_RING=4).
 static int g_s128_ring = 2;
@@ -1144,9 +1136,7 @@ static hipError_t set_lds_limits_packed()
     if ((e = set_lds_limit<FP4, MODE, GeoS, 0, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoS, 1, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoL8H, 0, 2>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<FP4, MODE, GeoL8H, 1, 2>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<FP4, MODE, GeoL8, 0, 2>()) != hipSuccess) return e;
-    return set_lds_limit<FP4, MODE, GeoL8, 1, 2>();
+    return set_lds_limit<FP4, MODE, GeoL8H, 1, 2>();
 }
 
 // Raise the dynamic LDS limit of every kernel variant (done once per
@@ -1196,7 +1186,7 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              const WorkDesc *descs, const WorkDesc &inline_desc,
                              const WorkItem *items, int64_t nitems,
                              int persistent, uint32_t *queue, const int64_t *qbeg,
-                             hipStream_t s, int words)
+                             hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
     KArgs a;
@@ -1219,11 +1209,6 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
             return r2_mode == QLD_R2_EXACT
                        ? launch_mode<FP4, QLD_R2_EXACT, GeoS, 2>(a, persistent, s)
                        : launch_mode<FP4, QLD_R2_REFERENCE, GeoS, 2>(a, persistent, s);
-        if (words <= 0 && !descs) words = inline_desc.words;
-        if (words > QLD_SPLIT_MAX_WORDS)
-            return r2_mode == QLD_R2_EXACT
-                       ? launch_mode<FP4, QLD_R2_EXACT, GeoL8, 2>(a, persistent, s)
-                       : launch_mode<FP4, QLD_R2_REFERENCE, GeoL8, 2>(a, persistent, s);
         return r2_mode == QLD_R2_EXACT
                    ? launch_mode<FP4, QLD_R2_EXACT, GeoL8H, 2>(a, persistent, s)
                    : launch_mode<FP4, QLD_R2_REFERENCE, GeoL8H, 2>(a, persistent, s);

@@ -37,7 +37,6 @@ struct qld_plan {
     int direct = 0;                       // packed rows into registers (fp4)
     int engine = QLD_ENGINE_MFMA_FP4;
     int r2_mode = QLD_R2_REFERENCE;
-    int words = 0;                        // packed words per row of the table
     std::vector<PlanClass> classes;       // persistent mode
     std::vector<PlainLaunch> plain;       // plain mode
     WorkItem *d_items_all = nullptr;      // plain mode item array
@@ -211,8 +210,7 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
             if (!c.nitems) continue;
             e = launch_mfma_items(p->engine, p->r2_mode, c.tile, p->direct ? QLD_FP4_DIRECT : 0,
                                   c.d_descs, none, c.d_items,
-                                  (int64_t)c.nitems, c.grid, c.d_queue, nullptr, ctx->stream,
-                                  p->words);
+                                  (int64_t)c.nitems, c.grid, c.d_queue, nullptr, ctx->stream);
             if (e != hipSuccess) return e;
         }
     } else {
@@ -220,7 +218,7 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
             e = launch_mfma_items(p->engine, p->r2_mode, l.tile, p->direct ? QLD_FP4_DIRECT : 0,
                                   nullptr, l.desc,
                                   p->d_items_all + l.item_off, l.item_cnt, 0, nullptr, nullptr,
-                                  ctx->stream, p->words);
+                                  ctx->stream);
             if (e != hipSuccess) return e;
         }
     }
@@ -273,7 +271,6 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
     p->mode = mode;
     p->engine = t->engine;
     p->direct = t->direct;
-    p->words = t->words;
     p->r2_mode = ctx->cfg.r2_mode;
 
     // Windows in table order, so that each XCD's contiguous share of the

@@ -425,7 +425,7 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
     e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, pmode, nullptr, d,
                           (const WorkItem *)ctx->tiles.ptr, ntiles, (int)grid,
-                          (uint32_t *)ctx->queue.ptr, nullptr, ctx->stream, d.words);
+                          (uint32_t *)ctx->queue.ptr, nullptr, ctx->stream);
     if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "MFMA kernel launch");
 }
