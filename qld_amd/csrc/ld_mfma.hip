@@ -632,19 +632,18 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     int g = 0;
     for (; g + 2 <= full; g += 2) {
-        // Split 256-tiles: scheduling barriers keep each group's loads a
-        // whole compute ahead of their use (left alone, the scheduler sinks
-        // them next to the first use; measured 2.58 vs 2.79 ms on the
-        // 30,000-SNP triangle).  The 128-tile kernel keeps the compiler's
-        // placement.
+        // Scheduling barriers keep each group's loads a whole compute ahead
+        // of their use (left alone, the scheduler sinks them next to the
+        // first use; measured on the 30,000-SNP triangle: 2.58 vs 2.79 ms
+        // with split 256-tiles, 2.76 vs 3.06 ms with 128-tiles).
         fetch(qa, qb, g + 1);
-        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
         compute(pa, pb);
-        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
         fetch(pa, pb, g + 2);
-        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
         compute(qa, qb);
-        if constexpr (G::SPLIT > 1) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
     }
     if (g < full) compute(pa, pb);
 }

@@ -330,13 +330,16 @@ static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // Tile edge for one plain job.  Expanded operands: 128 x 128 tiles (two
 // workgroups per CU) measured faster than 256 x 256 (3.47 vs 4.03 ms on the
-// 30,000-SNP triangle).  Direct fp4 operands: 256-tiles once both sides
-// reach the tile cutoff (2.91 vs 3.22 ms on the same triangle with 8-wave
-// workgroups; the launcher splits them further for short rows), 128 below.
+// 30,000-SNP triangle).  Direct fp4 operands: 256-tiles (split over two
+// workgroups) once the job has at least 4 of them per CU, 128 below: smaller
+// jobs lose more to the coarser tiles' quantization than they gain per tile
+// (measured triangles at 16,622 samples: 8,000 SNPs 0.223 vs 0.231 ms,
+// 12,000 SNPs 0.465 vs 0.419 ms, 30,000 SNPs 2.77 vs 2.64 ms for 128 vs 256).
 static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job, int pmode)
 {
     if (ctx->plain_tile) return ctx->plain_tile;
-    if (pmode == QLD_FP4_DIRECT && job.m >= ctx->tile_cutoff && job.n >= ctx->tile_cutoff)
+    if (pmode == QLD_FP4_DIRECT &&
+        job_tile_count(job.n, job.m, job.tri, QLD_TILE_LARGE) >= (size_t)4 * ctx->num_cu)
         return QLD_TILE_LARGE;
     return QLD_TILE_SMALL;
 }
