@@ -99,15 +99,15 @@ struct Geo {
 // Kernel configurations: (tile, stage bytes, ring depth, waves down, waves
 // across[, workgroups per tile]).  256-tiles with operands in LDS: 8 waves
 // of 128 x 64 (two per SIMD, 0.75 KiB of LDS operand reads per MFMA) over a
-// 4-deep ring of 64-byte stages, or the earlier 4 waves of 128 x 128 (one
-// per SIMD, 2-deep ring of 128 bytes).  Direct operands: the same 8 waves
+// 4-deep ring of 64-byte stages (measured faster than 4 waves of 128 x 128
+// over a 2-deep ring of 128-byte stages).  Direct operands: the same 8 waves
 // of 128 x 64 as two workgroups of 4 (L256W8H; no ring, the stage fields
-// only size the unused ring geometry).
-#define QLD_CFG_L256 256, 128, 2, 2, 2
+// only size the unused ring geometry).  128-tiles: 4 waves of 64 x 64 over
+// a 2-deep ring of 128-byte stages (a 4-deep ring of 64-byte stages
+// measured slower).
 #define QLD_CFG_L256W8 256, 64, 4, 2, 4
 #define QLD_CFG_L256W8H 256, 64, 4, 2, 4, 2
 #define QLD_CFG_S128 128, 128, 2, 2, 2
-#define QLD_CFG_S128R4 128, 64, 4, 2, 2
 
 // v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (scale 2^0): on gfx950
 // it retires twice the flops per cycle of the 16x16x128 form.
@@ -1103,31 +1103,16 @@ static hipError_t set_lds_limit()
                                lds_bytes<G, PACKED>());
 }
 
-typedef Geo<QLD_CFG_L256> GeoL;
 typedef Geo<QLD_CFG_L256W8> GeoL8;
 typedef Geo<QLD_CFG_L256W8H> GeoL8H;
 typedef Geo<QLD_CFG_S128> GeoS;
-typedef Geo<QLD_CFG_S128R4> GeoS4;
-
-// 256-tile variant for operands in LDS: 8 waves of 128 x 64 (default) or 4
-// of 128 x 128 (QLD_TILE256_WAVES=4 in the environment), read once when the
-// first context is created.
-static int g_l256_waves = 8;
-
-// 128-tile ring: two 128-byte stages (default) or four 64-byte stages
-// (QLD_TILE128_RING=4).
-static int g_s128_ring = 2;
 
 template <class OP, int MODE>
 static hipError_t set_lds_limits_mode()
 {
     hipError_t e;
-    if ((e = set_lds_limit<OP, MODE, GeoL, 0>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<OP, MODE, GeoL, 1>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoL8, 0>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoL8, 1>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<OP, MODE, GeoS4, 0>()) != hipSuccess) return e;
-    if ((e = set_lds_limit<OP, MODE, GeoS4, 1>()) != hipSuccess) return e;
     if ((e = set_lds_limit<OP, MODE, GeoS, 0>()) != hipSuccess) return e;
     return set_lds_limit<OP, MODE, GeoS, 1>();
 }
@@ -1148,10 +1133,6 @@ static hipError_t set_lds_limits_packed()
 // process, before any launch -- and so before any graph capture).
 hipError_t mfma_init_attributes()
 {
-    const char *w = getenv("QLD_TILE256_WAVES");
-    if (w && atoi(w) == 4) g_l256_waves = 4;
-    const char *r = getenv("QLD_TILE128_RING");
-    if (r && atoi(r) == 4) g_s128_ring = 4;
     hipError_t e;
     if ((e = set_lds_limits_mode<FP4, QLD_R2_REFERENCE>()) != hipSuccess) return e;
     if ((e = set_lds_limits_mode<FP4, QLD_R2_EXACT>()) != hipSuccess) return e;
@@ -1180,10 +1161,7 @@ static hipError_t launch_mode(const KArgs &a, int persistent, hipStream_t s)
 template <class OP, int MODE>
 static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStream_t s)
 {
-    if (tile == QLD_TILE_SMALL)
-        return g_s128_ring == 4 ? launch_mode<OP, MODE, GeoS4>(a, persistent, s)
-                                : launch_mode<OP, MODE, GeoS>(a, persistent, s);
-    if (g_l256_waves == 4) return launch_mode<OP, MODE, GeoL>(a, persistent, s);
+    if (tile == QLD_TILE_SMALL) return launch_mode<OP, MODE, GeoS>(a, persistent, s);
     return launch_mode<OP, MODE, GeoL8>(a, persistent, s);
 }
 
