@@ -160,7 +160,10 @@ int qld_last_engine(const qld_ctx *ctx);
  * callers that correlate many windows of one chromosome. */
 typedef struct qld_table qld_table;
 
-/* Copy `rows` packed rows and their bit counts from host memory. */
+/* Copy `rows` packed rows and their bit counts from host memory into device
+ * memory, in the layout the context's engine reads: a row-block-tiled copy
+ * of the same size for the default fp4 direct operands, the packed rows plus
+ * an expanded fp4 / int8 copy otherwise.  The caller's rows are not kept. */
 int qld_table_create(qld_ctx *ctx, const uint64_t *rows_host,
                      const uint32_t *bc_host, int rows, int words,
                      int snp_size, qld_table **out);
