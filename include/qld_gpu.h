@@ -133,9 +133,10 @@ int qld_timer_start(qld_ctx *ctx);
 int qld_timer_stop(qld_ctx *ctx, float *ms);
 
 /* Operand source of the fp4 engine (plain calls, and tables created after
- * the call): DIRECT (default) reads the packed rows straight into registers
- * and expands bits to e2m1 there (no LDS ring, no workspace, no expanded
- * copy, 1 bit of memory traffic per sample); EXPANDED first writes an fp4
+ * the call): DIRECT (default) reads the packed rows, from a row-block-tiled
+ * copy made per call (or held by the table), straight into registers and
+ * splits bits into e2m1 planes there (no LDS ring, no expanded copy, 1 bit
+ * of memory traffic per sample); EXPANDED first writes an fp4
  * copy of the tables (4 bits per sample) that the kernel streams into LDS by
  * DMA; PACKED reads the packed rows into LDS and expands them there
  * (128-tiles). */
@@ -145,8 +146,8 @@ int qld_timer_stop(qld_ctx *ctx, float *ms);
 int qld_set_fp4_operands(qld_ctx *ctx, int mode);
 
 /* Tile edge of the MFMA engines for plain calls: 256, 128, or 0 for the
- * default (direct fp4: 256 when both sides reach the tile cutoff, else 128;
- * expanded operands: 128). */
+ * default (direct fp4: 256 once the job has at least 4 such tiles per CU,
+ * else 128; expanded operands: 128). */
 int qld_set_plain_tile(qld_ctx *ctx, int tile);
 
 /* Engine the last call used (QLD_ENGINE_*), after AUTO resolution. */
