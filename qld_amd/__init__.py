@@ -580,7 +580,8 @@ class Engine:
 
 
 class Table:
-    """A packed table resident in HBM with its expanded MFMA operand form."""
+    """A packed table resident in HBM, held in the layout the context's
+    engine reads (qld_table_create / qld_table_create_dev)."""
 
     def __init__(self, eng: Engine, rows: np.ndarray, bc: np.ndarray, snp_size=None):
         rows = np.ascontiguousarray(rows, dtype=np.uint64)
@@ -592,6 +593,21 @@ class Table:
         eng._check(hip_lib().qld_table_create(eng._ctx, _ptr(rows), _ptr(bc), self.rows,
                                               self.words, self.snp_size, C.byref(t)))
         self._t = t
+
+    @classmethod
+    def from_device(cls, eng: Engine, rows_buf: "DevBuffer", bc_buf: "DevBuffer",
+                    nrows: int, words: int, snp_size: int) -> "Table":
+        """A table copied from packed rows and bit counts already in device
+        buffers (qld_table_create_dev)."""
+        self = cls.__new__(cls)
+        self.eng = eng
+        self.rows, self.words, self.snp_size = int(nrows), int(words), int(snp_size)
+        t = C.c_void_p()
+        eng._check(hip_lib().qld_table_create_dev(eng._ctx, rows_buf.ptr, bc_buf.ptr,
+                                                  self.rows, self.words, self.snp_size,
+                                                  C.byref(t)))
+        self._t = t
+        return self
 
     @property
     def nbytes(self) -> int:

@@ -68,6 +68,34 @@ def test_window_plans(engine, mode, cutoff, engine_factory):
     out.free()
 
 
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x"])
+def test_table_from_device_rows(engine, engine_factory):
+    """A table built from device buffers (qld_table_create_dev) gives the
+    same windows as one built from host rows; rows not a multiple of 32 and
+    windows reaching the last row exercise the tiled copy's edges."""
+    N, rows = 100000, 1037
+    A, bc = qld_amd.synth_blocks(777, rows, N)
+    rng = np.random.default_rng(5)
+    wins = _windows(rng, rows, 12, 40, 600)
+    wins = qld_amd.window_layout(np.append(wins["start"], rows - 300),
+                                 np.append(wins["size"], 300))
+    total = int((wins["size"].astype(np.int64) ** 2).sum())
+    e = engine_factory(engine=engine, r2_mode="reference")
+    dA = e.alloc(A.nbytes).upload(A)
+    dbc = e.alloc(bc.nbytes).upload(bc)
+    t = qld_amd.Table.from_device(e, dA, dbc, rows, A.shape[1], N)
+    dA.fill(0)                              # the table holds its own copy
+    assert t.nbytes > 0
+    out = e.alloc(total * 4)
+    out.fill(0x7FC00000)
+    plan = e.plan_windows(t, wins, out)
+    plan.run()
+    e.sync()
+    _check_windows(out.download(total, np.float32), wins, A, bc, N)
+    for b in (plan, t, out, dA, dbc):
+        b.free()
+
+
 def test_window_plan_exact_mode_and_samples_100k(engine_factory):
     N, rows = 100000, 900
     A, bc = qld_amd.synth_blocks(99, rows, N)
