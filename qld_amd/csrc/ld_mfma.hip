@@ -518,19 +518,17 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
     if (s + 1 < stages) step(s + 1, R2);
 }
 
-#ifndef QLD_FP4_PLANES
-#define QLD_FP4_PLANES 1     // direct k loop: bit planes + MFMA scales (0: fp4_chunk)
-#endif
-
-// k loop straight from the packed rows (fp4, no LDS, no barrier).  For each
-// group of 4 k-steps (4 packed words = 256 samples of a row) lane l of a
-// 32 x 32 operand fetch loads 16 bytes -- words 2h, 2h+1 of the group, h =
-// l/32 -- of row l%32 with one buffer_load_dwordx4, and k-step ks uses
-// dword ks of them, expanded to e2m1 nibbles in registers (fp4_chunk).  Both
-// operands use the same sample -> K mapping, so the products pair samples
-// correctly.  The next group's words are in flight while this group's MFMAs
-// run.  Rows past the job end read 0 through the buffer bound; words past
-// the row end (W not a multiple of 4) are zeroed.
+// k loop straight from the packed rows (fp4, no LDS, no barrier), read from
+// the row-block-tiled copy (launch_tile_rows).  For each group of 4 k-steps
+// (4 packed words = 256 samples of a row) lane l of a 32 x 32 operand fetch
+// loads 16 bytes -- words 2h, 2h+1 of the group, h = l/32 -- of row l%32
+// with one buffer_load_dwordx4 (the wave's 64 loads are 1 KiB of
+// consecutive bytes), split into four bit planes in registers (fp4_plane).
+// Both operands use the same sample -> K mapping, so the products pair
+// samples correctly.  The next group's words are in flight while this
+// group's MFMAs run.  Rows past the table and words past the row end are
+// zero in the tiled copy; rows past the job but inside the table are read
+// and their products never stored.
 template <class G>
 __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, int wm, int wn,
                                            v16f (&acc)[G::NBM][G::NBN])
@@ -581,7 +579,6 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
         for (int b = 0; b < NBN; ++b)
             xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB[b] + k, 0, 0);
     };
-#if QLD_FP4_PLANES
     auto plane = [&](auto pc, const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
         constexpr int P = decltype(pc)::value;
         v4i bv[NBN];
@@ -601,22 +598,6 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
         plane(std::integral_constant<int, 2>(), xa, xb);
         plane(std::integral_constant<int, 3>(), xa, xb);
     };
-#else
-    auto compute = [&](const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            v4i bv[NBN];
-#pragma unroll
-            for (int b = 0; b < NBN; ++b) bv[b] = fp4_chunk((uint32_t)xb[b][ks]);
-#pragma unroll
-            for (int a = 0; a < NBM; ++a) {
-                const v4i av = fp4_chunk((uint32_t)xa[a][ks]);
-#pragma unroll
-                for (int b = 0; b < NBN; ++b) acc[a][b] = FP4::mma(av, bv[b], acc[a][b]);
-            }
-        }
-    };
-#endif
     // Two register sets: group g+1 in flight while group g computes.  The
     // loads in the loop are unconditional, so it carries no register copies
     // (the group read past the last is beyond the rows' blocks or the
