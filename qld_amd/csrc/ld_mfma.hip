@@ -14,18 +14,22 @@
 // The pairing of samples to K positions only has to be the same for every
 // row; the expansion keeps each 64-bit word's samples together.
 //
-// Work decomposition (both engines), tile edge T = 256 or 128 (the direct
-// k loop and the 256-tile configurations below refine this)
-//   workgroup : 4 wave64 (2 x 2), one T x T tile of pairs of one job
-//   wave      : T/2 x T/2 = (T/64)^2 MFMA blocks of 32 x 32 (256 or 64
-//               fp32/int32 accumulators per lane)
-//   k stage   : SB bytes of every tile row (SB/32 MFMA k-steps) in a ring
-//               of NSTAGE LDS buffers, filled with buffer_load_dwordx4 ... lds
-//               (no register staging; out-of-range rows read as 0 through
-//               the buffer descriptor's bound); one barrier per stage
-// LDS rows are stored as 16-byte chunks permuted per row (Geo::swz) so that
-// the ds_read_b128 of consecutive rows by an MFMA operand fetch spreads over
-// all banks.
+// Work decomposition, tile edge T = 256 or 128 (configurations below)
+//   direct fp4 (default): each wave reads its own 32-row operand blocks
+//               from the row-block-tiled table into registers (kloop_regs,
+//               no LDS ring, no barrier); 256-tiles are two workgroups of 4
+//               waves of 128 x 64, 128-tiles one workgroup of 4 waves of
+//               64 x 64; LDS only stages the r^2 epilogue
+//   operands in LDS (expanded fp4 / int8, or packed rows expanded in LDS):
+//     workgroup : one T x T tile (8 waves of 128 x 64 for 256-tiles, 4 of
+//                 64 x 64 for 128-tiles)
+//     k stage   : SB bytes of every tile row (SB/32 MFMA k-steps) in a ring
+//                 of NSTAGE LDS buffers, filled with buffer_load_dwordx4 ...
+//                 lds (no register staging; out-of-range rows read as 0
+//                 through the buffer descriptor's bound); one barrier per
+//                 stage.  LDS rows are stored as 16-byte chunks permuted per
+//                 row (Geo::swz) so that the ds_read_b128 of consecutive rows
+//                 by an MFMA operand fetch spreads over all banks.
 #include "qld_internal.h"
 #include "ld_epilogue.h"
 #include "ld_tiles.h"
