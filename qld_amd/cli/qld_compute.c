@@ -332,7 +332,11 @@ static task *list_tasks(options *o, int *ntask, FILE *rep)
             printf("ABORTING: Line %d ---posWmin2 greater than posWmax2---\n", line_no);
             continue;
         }
-        snprintf(t.output, sizeof t.output, "%s_%d.txt", o->output, line_no);
+        if ((size_t)snprintf(t.output, sizeof t.output, "%s_%d.txt", o->output, line_no) >=
+            sizeof t.output) {
+            printf("ABORTING: Line %d ---output path too long---\n", line_no);
+            continue;
+        }
         if (access(t.input, F_OK) == -1 || access(t.input2, F_OK) == -1) {
             printf("ABORTING: Line %d ---Input files do not exist---\n", line_no);
             continue;

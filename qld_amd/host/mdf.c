@@ -160,7 +160,8 @@ int qld_mdf_read_header(const char *dir, qld_mdf_header *h)
     int found = 0;
     while ((e = readdir(dp)) != NULL) {
         char path[QLD_PATH_MAX];
-        snprintf(path, sizeof path, "%s%s", d, e->d_name);
+        if ((size_t)snprintf(path, sizeof path, "%s%s", d, e->d_name) >= sizeof path)
+            continue;               /* longer than any path this reader opens */
         if (!is_regular(path) || !strstr(e->d_name, "header")) continue;
         if (found) continue;        /* the first header file found is used */
         found = 1;
@@ -269,7 +270,8 @@ int qld_find_chunks(const char *dir, const char *id, int snips_per_file, int tot
         struct dirent *e;
         while ((e = readdir(dp)) != NULL) {
             char path[QLD_PATH_MAX];
-            snprintf(path, sizeof path, "%s%s", d, e->d_name);
+            if ((size_t)snprintf(path, sizeof path, "%s%s", d, e->d_name) >= sizeof path)
+                continue;           /* longer than any path this reader opens */
             if (!is_regular(path)) continue;
             if (!strstr(e->d_name, id) || strstr(e->d_name, "header") || !strstr(e->d_name, suffix))
                 continue;
