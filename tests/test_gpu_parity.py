@@ -183,6 +183,38 @@ def test_full_size_sampled_rows(engine, engine_factory):
         b.free()
 
 
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_largest_sample_count(engine, mode, engine_factory):
+    """N = 2^24 - 1, the largest sample count the engines accept (counts near
+    2^24 must stay exact in fp32 sums): dense, complementary, sparse and
+    all-one rows, a triangle and a rectangle; N = 2^24 is rejected."""
+    N = (1 << 24) - 1
+    W = (N + 63) // 64
+    rng = np.random.default_rng(24)
+    bits = np.zeros((6, W * 64), dtype=np.uint8)
+    bits[0, :N] = 1                                     # all ones (bc = N -> NaN r2)
+    bits[1, :N] = rng.random(N) < 0.9                   # dense
+    bits[2, :N] = 1 - bits[1, :N]                       # its complement
+    bits[3, :N] = bits[1, :N]                           # a copy
+    bits[3, rng.integers(0, N, 1000)] ^= 1
+    bits[4, :N] = rng.random(N) < 0.001                 # sparse
+    bits[5, :N] = rng.random(N) < 0.5
+    A = np.packbits(bits, axis=1, bitorder="big").view(">u8").astype(np.uint64)
+    bc = bits.sum(axis=1).astype(np.uint32)
+    e = engine_factory(engine=engine, r2_mode=mode)
+    say("max N", engine, mode)
+    res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+    _check(res, cnt, A, bc, A, bc, N, True, mode)
+    res, cnt = e.correlate(A[:2], bc[:2], A[2:], bc[2:], snp_size=N, triangular=False,
+                           counts=True)
+    _check(res, cnt, A[:2], bc[:2], A[2:], bc[2:], N, False, mode)
+    with pytest.raises(qld_amd.QldError) as x:
+        e.correlate(np.zeros((2, W + 1), np.uint64), np.zeros(2, np.uint32),
+                    snp_size=1 << 24)
+    assert x.value.code == qld_amd.QLD_ERR_INVALID
+
+
 def test_errors_are_status_codes(engine_factory):
     e = engine_factory(engine="popcount")
     A, bc = qld_amd.synth_blocks(1, 10, 100)
