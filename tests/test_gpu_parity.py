@@ -215,6 +215,27 @@ def test_largest_sample_count(engine, mode, engine_factory):
     assert x.value.code == qld_amd.QLD_ERR_INVALID
 
 
+@pytest.mark.parametrize("engine", ENGINES)
+def test_empty_inputs(engine, engine_factory):
+    """Zero rows on either side: no work, no error, empty outputs; the
+    context stays usable."""
+    N = 16622
+    A, bc = qld_amd.synth_blocks(11, 40, N)
+    E = np.zeros((0, A.shape[1]), np.uint64)
+    bE = np.zeros(0, np.uint32)
+    e = engine_factory(engine=engine, r2_mode="reference")
+    res, cnt = e.correlate(E, bE, snp_size=N, triangular=True, counts=True)
+    assert res.shape == (0, 0) and cnt.shape == (0, 0)
+    res, _ = e.correlate(A, bc, E, bE, snp_size=N, triangular=False)
+    assert res.shape == (0, 40)
+    res, _ = e.correlate(E, bE, A, bc, snp_size=N, triangular=False)
+    assert res.shape == (40, 0)
+    offs, cols, r2 = e.correlate_hits(A, bc, E, bE, snp_size=N, upper=False)
+    assert len(cols) == 0 and len(r2) == 0 and int(offs[-1]) == 0
+    res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+    _check(res, cnt, A, bc, A, bc, N, True, "reference")
+
+
 def test_errors_are_status_codes(engine_factory):
     e = engine_factory(engine="popcount")
     A, bc = qld_amd.synth_blocks(1, 10, 100)
