@@ -236,6 +236,40 @@ def test_empty_inputs(engine, engine_factory):
     _check(res, cnt, A, bc, A, bc, N, True, "reference")
 
 
+def test_contexts_on_concurrent_threads(engine_factory):
+    """One context per calling thread, four threads at once (ctypes releases
+    the GIL during each call): every thread's results stay bit-exact."""
+    import threading
+    N = 16622
+    jobs = []
+    for k, (m, eng) in enumerate([(1200, "auto"), (700, "mfma_fp4"), (300, "popcount"),
+                                  (1100, "mfma_fp4x")]):
+        A, bc = qld_amd.synth_blocks(900 + k, m, N)
+        want_c = oracle.counts_naive(A)
+        jobs.append((A, bc, want_c, oracle.r2_reference(bc, bc, want_c, N),
+                     engine_factory(engine=eng, r2_mode="reference")))
+    errors = []
+
+    def work(A, bc, want_c, want, e):
+        try:
+            sel = np.triu(np.ones((A.shape[0],) * 2, dtype=bool), 1)
+            for _ in range(3):
+                res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+                assert (cnt[sel] == want_c[sel]).all(), "counts differ"
+                assert (res.view(np.uint32)[sel] == want.view(np.uint32)[sel]).all(), "r2 differs"
+        except Exception as x:          # reported on the main thread
+            errors.append(repr(x))
+
+    threads = [threading.Thread(target=work, args=j) for j in jobs]
+    say("threads", len(threads))
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=300)
+    assert not any(t.is_alive() for t in threads), "a thread did not finish"
+    assert not errors, errors
+
+
 def test_errors_are_status_codes(engine_factory):
     e = engine_factory(engine="popcount")
     A, bc = qld_amd.synth_blocks(1, 10, 100)
