@@ -9,7 +9,6 @@
 
 #include "qld_internal.h"
 
-#include <vector>
 
 // One all-pairs job on expanded operands.  Row i of the job reads expanded
 // row XI + i*row_bytes and bit count bcI[i]; column j reads XJ / bcJ.  Cell
@@ -88,11 +87,6 @@ int64_t mfma_row_bytes(int engine, int words);
 void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int nj,
                       int tri, int T);
 size_t job_tile_count(int ni, int nj, int tri, int T);
-
-// Tiles of one job for a persistent launch, grouped per XCD queue:
-// items = queue 0's tiles, then queue 1's, ...; qbeg[0..8] = queue bounds.
-void build_xcd_schedule(std::vector<WorkItem> &items, int64_t *qbeg, uint32_t desc,
-                        int ni, int nj, int tri, int T);
 
 // Expand packed rows to fp4 nibbles / int8 bytes (rows x row_bytes).
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,

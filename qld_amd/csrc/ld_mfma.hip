@@ -37,7 +37,6 @@
 
 #include <stdlib.h>
 #include <type_traits>
-#include <vector>
 
 namespace {
 
@@ -985,37 +984,6 @@ void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int n
                     if (!tri || ti <= tj) out[k++] = WorkItem{desc, (uint16_t)ti, (uint16_t)tj};
         }
     *count = k;
-}
-
-void build_xcd_schedule(std::vector<WorkItem> &items, int64_t *qbeg, uint32_t desc,
-                        int ni, int nj, int tri, int T)
-{
-    // The persistent grid keeps 2 workgroups per CU = 64 per XCD busy.  The
-    // tiles are cut into super-blocks of 16 x 32 tiles; inside one, XCD x
-    // owns the 8 x 8 sub-block (x / 4, x % 4).  Every queue walks the
-    // super-blocks in the same order, so at any time all XCDs share one
-    // super-block's 48 panels in the Infinity Cache while each XCD reuses
-    // its own 16 panels in its L2.
-    const int SI = 16, SJ = 32, SUB = 8;
-    const int tic = (ni + T - 1) / T, tjc = (nj + T - 1) / T;
-    std::vector<std::vector<WorkItem>> q(QLD_NUM_XCD);
-    for (int J = 0; J < (tjc + SJ - 1) / SJ; ++J)
-        for (int I = 0; I < (tic + SI - 1) / SI; ++I) {
-            if (tri && I * SI > (J + 1) * SJ - 1) continue;
-            for (int x = 0; x < QLD_NUM_XCD; ++x) {
-                const int bi = I * SI + (x / 4) * SUB, bj = J * SJ + (x % 4) * SUB;
-                for (int tj = bj; tj < bj + SUB && tj < tjc; ++tj)
-                    for (int ti = bi; ti < bi + SUB && ti < tic; ++ti)
-                        if (!tri || ti <= tj)
-                            q[x].push_back(WorkItem{desc, (uint16_t)ti, (uint16_t)tj});
-            }
-        }
-    items.clear();
-    for (int x = 0; x < QLD_NUM_XCD; ++x) {
-        qbeg[x] = (int64_t)items.size();
-        items.insert(items.end(), q[x].begin(), q[x].end());
-    }
-    qbeg[QLD_NUM_XCD] = (int64_t)items.size();
 }
 
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,

@@ -313,11 +313,11 @@ static int check_job(qld_ctx *ctx, const LdJob &j)
 
 int qld_check_job(qld_ctx *ctx, const LdJob &j) { return check_job(ctx, j); }
 
-// AUTO policy from measurements on MI355X (30,000 SNPs x 16,622 samples:
-// popcount 17.1 ms, int8 MFMA 11.2 ms, fp4 MFMA 5.0 ms per triangle): the fp4
-// engine wins once a launch has enough 256 x 256 tiles to keep the matrix
-// cores busy; tiny problems stay on the popcount engine, which needs no
-// expansion pass.
+// AUTO policy from measurements on MI355X (30,000 SNPs x 16,622 samples,
+// exact r^2: popcount 17.0 ms, int8 MFMA 7.7 ms, fp4 MFMA 2.16 ms per
+// triangle): the fp4 engine wins once a launch has more than a handful of
+// tiles; tiny problems stay on the popcount engine, which needs neither the
+// row-tiling pass nor a tile list.
 int qld_pick_engine(const qld_ctx *ctx, const LdJob &j)
 {
     if (ctx->cfg.engine != QLD_ENGINE_AUTO) return ctx->cfg.engine;
@@ -368,8 +368,9 @@ static int upload_plain_tiles(qld_ctx *ctx, const LdJob &job, int T)
 
 static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
 {
-    // The fp4 engine reads packed rows directly (expanding in the kernel,
-    // 128-tiles) unless the context asks for an expanded operand copy.
+    // The fp4 engine reads a row-block-tiled copy of the packed rows and
+    // splits it into bit planes in registers (DIRECT), unless the context
+    // asks for row-major packed operands (PACKED) or an expanded copy.
     const int pmode = engine == QLD_ENGINE_MFMA_FP4 ? ctx->fp4_packed : QLD_FP4_EXPANDED;
     const bool packed = pmode != QLD_FP4_EXPANDED;
     const int T = pmode == QLD_FP4_PACKED ? QLD_TILE_SMALL : plain_tile_edge(ctx, job, pmode);
@@ -419,8 +420,8 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     }
     // Persistent launch: two workgroups per CU (128-tiles) or one per CU
     // per tile share (256; the launcher doubles it for split tiles) drain
-    // per-XCD queues over equal shares of the tile list; measured 3.47 ms
-    // vs 4.11 ms for one workgroup per tile on the 30,000-SNP triangle.
+    // per-XCD queues over equal shares of the tile list (expanded operands,
+    // 30,000-SNP triangle: 3.47 ms vs 4.11 ms for one workgroup per tile).
     if ((rc = ctx->queue.reserve(ctx, 16 * sizeof(uint32_t)))) return rc;
     const int64_t ntiles = (int64_t)ctx->tiles_host.size();
     int64_t grid = (int64_t)ctx->num_cu * (T == QLD_TILE_SMALL ? 2 : 1);
