@@ -163,7 +163,9 @@ int qld_vcf_header_lines(const char *header_path, char **line1, char **chrom_lin
 
 /* makeValidList / getFilter / sample_isValid (correlate_IO.c:35-76,259-290):
  * sample columns of the #CHROM line named in the list file (all samples
- * without one); valid_count = selected samples x ploidy. */
+ * without one); valid_count = selected samples x ploidy.  A list whose names
+ * match more columns than it has names (a name the header repeats) is an
+ * error, where the reference fails its assertion (correlate_IO.c:287-288). */
 int qld_vcf_sample_mask(const char *chrom_line, const char *list_file, int ploidy,
                         unsigned char **mask, int *nsamples, int *valid_count)
 {
@@ -211,6 +213,12 @@ int qld_vcf_sample_mask(const char *chrom_line, const char *list_file, int ploid
     if (oom) {
         free(m);
         return vfail(QLD_ERR_NOMEM, "out of host memory");
+    }
+    if (nnames > 0 && (int64_t)valid > (int64_t)ploidy * nnames) {
+        free(m);
+        return vfail(QLD_ERR_INVALID, "sample list %s: %d names select %d sample columns "
+                     "(a listed name appears more than once in the header)",
+                     list_file, nnames, valid / (ploidy > 0 ? ploidy : 1));
     }
     *mask = m;
     *nsamples = n;

@@ -1,0 +1,101 @@
+"""Write the qLD-compute fixtures on the reference's own test data under
+tests/golden/covid/.
+
+The reference's smoke script (test_run:29-30,48-56,77-92) parses
+test_dataset/vcf/test_covid.vcf.gz with qLD-parse-VCF (-chrom 1 -size 1) and
+runs qLD-compute on the result with -r2limit 0.6 -ploidy haploid.  This
+script does the same with the reference's own builds (``make -C oracle
+ref-parser ref-cli``):
+
+1. the parser's chunk directory -> ``covid/data/covid_19/`` (22,554 haploid
+   samples, 6,328 sites in 18 chunks, as the parser wrote them);
+2. a sample list of every fourth name among the ``#CHROM`` line's names
+   that occur once (the data repeats some names, and a listed name that
+   matches two columns trips the reference's assertion in makeValidList,
+   correlate_IO.c:287) -> ``covid/data/samples_covid.txt``;
+3. for each case the report and ``input_report.txt`` of the reference
+   correlator's CPU build, run without -mdf -> ``covid/expected/<case>/``.
+
+Only this container has the reference checkout; everything written is data.
+Usage: python tests/golden/make_covid_golden.py
+"""
+from __future__ import annotations
+
+import collections
+import gzip
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+OUT = os.path.join(HERE, "covid")
+REF_VCF = "/root/reference/test_dataset/vcf/test_covid.vcf.gz"
+REF_PARSER = os.path.join(ROOT, "oracle", "_ref", "qLD-parse-VCF")
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "qLD-compute")
+
+CASES = {
+    # test_run's own compute flags
+    "test_run": ["-input", "data/covid_19", "-r2limit", "0.6", "-ploidy", "haploid"],
+    "r2limit_0.1": ["-input", "data/covid_19", "-r2limit", "0.1", "-ploidy", "haploid"],
+    "two_windows": ["-input", "data/covid_19", "-ploidy", "haploid",
+                    "-posWmin1", "3000", "-posWmax1", "12000",
+                    "-posWmin2", "9000", "-posWmax2", "25000", "-r2limit", "0.05"],
+    "samples": ["-input", "data/covid_19", "-ploidy", "haploid",
+                "-sampleList", "data/samples_covid.txt", "-r2limit", "0.3"],
+}
+
+
+def parse():
+    data = os.path.join(OUT, "data")
+    shutil.rmtree(data, ignore_errors=True)
+    os.makedirs(data)
+    r = subprocess.run([REF_PARSER, "-input", REF_VCF, "-output",
+                        os.path.join(data, "covid_19"), "-chrom", "1", "-size", "1"],
+                       capture_output=True, text=True, timeout=300)
+    # the parser ends with a non-zero status after writing every chunk
+    if not os.path.exists(os.path.join(data, "covid_19", "1_header.vcf.gz")):
+        raise SystemExit(f"parser failed ({r.returncode}):\n{r.stdout[-2000:]}{r.stderr[-2000:]}")
+    with gzip.open(os.path.join(data, "covid_19", "1_header.vcf.gz"), "rt") as f:
+        chrom = next(line for line in f if line.startswith("#CHROM"))
+    names = chrom.rstrip("\n").split("\t")[9:]
+    seen = collections.Counter(names)
+    once = [n for n in names if seen[n] == 1]
+    with open(os.path.join(data, "samples_covid.txt"), "w") as f:
+        f.write("\n".join(once[::4]) + "\n")
+
+
+def run_reference(name, args):
+    out = os.path.join(OUT, "expected", name)
+    shutil.rmtree(out, ignore_errors=True)
+    os.makedirs(out)
+    work = os.path.join(OUT, "_run")
+    shutil.rmtree(work, ignore_errors=True)
+    os.makedirs(os.path.join(work, "out"))
+    os.symlink(os.path.join(OUT, "data"), os.path.join(work, "data"))
+    cmd = [REF_CLI] + args + ["-output", "out/report.txt", "-threads", "1"]
+    r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise SystemExit(f"{name}: reference exited {r.returncode}\n{r.stderr[-2000:]}")
+    for fn in sorted(os.listdir(os.path.join(work, "out"))):
+        shutil.copy(os.path.join(work, "out", fn), os.path.join(out, fn))
+    shutil.copy(os.path.join(work, "input_report.txt"), os.path.join(out, "input_report.txt"))
+    shutil.rmtree(work)
+
+
+def main():
+    for b in (REF_PARSER, REF_CLI):
+        if not os.path.exists(b):
+            raise SystemExit("build the reference tools first: make -C oracle ref-parser ref-cli")
+    parse()
+    for name, args in CASES.items():
+        run_reference(name, args)
+        print(name, sorted(os.listdir(os.path.join(OUT, "expected", name))), flush=True)
+    with open(os.path.join(OUT, "cases.json"), "w") as f:
+        json.dump({"cases": CASES}, f, indent=1)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -369,3 +369,81 @@ def test_cli_vcf_reports_equal_reference_on_gpu(tmp_path):
         assert sorted(got) == sorted(exp), name
         for f in exp:
             assert got[f] == exp[f], (name, f)
+
+
+# ---- the reference's own test data (tests/golden/covid/, make_covid_golden.py) ----
+
+COVID_DIR = os.path.join(GOLDEN, "covid")
+COVID_MANIFEST = json.load(open(os.path.join(COVID_DIR, "cases.json")))
+COVID_CASES = sorted(COVID_MANIFEST["cases"])
+
+
+def _covid_workdir(tmp_path):
+    os.symlink(os.path.join(COVID_DIR, "data"), tmp_path / "data")
+    os.makedirs(tmp_path / "out")
+    return tmp_path
+
+
+def _covid_expected(name):
+    d = os.path.join(COVID_DIR, "expected", name)
+    return {f: open(os.path.join(d, f), "rb").read() for f in sorted(os.listdir(d))}
+
+
+@pytest.mark.parametrize("name", COVID_CASES)
+def test_cli_covid_tables_reproduce_reference_report(name, tmp_path):
+    """The reference's own test VCF (test_dataset/vcf/test_covid.vcf.gz:
+    22,554 haploid samples, 6,328 sites) as its parser chunks it, under the
+    compute flags of its smoke script (test_run:77-92) and three variations
+    (r2limit 0.1, two windows, a sample list): the CLI's tables through the
+    oracle give the reference's report byte for byte."""
+    args = COVID_MANIFEST["cases"][name]
+    wd = _covid_workdir(tmp_path)
+    r = subprocess.run([CLI_BIN] + args + ["-output", "out/report.txt", "-dump-tables"],
+                       cwd=wd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    exp = _covid_expected(name)
+    assert open(wd / "input_report.txt", "rb").read() == exp["input_report.txt"]
+    assert sorted(os.listdir(wd / "out")) == ["report.txt.tables"]
+    single, wset2, snp_size, A, B = _read_tables(wd / "out" / "report.txt.tables")
+    assert snp_size == 22554 and len(A["pos"]) > 1000
+    counts = oracle.counts_blocked(A["snp"], B["snp"], tri=False)
+    r2 = oracle.r2_reference(A["bc"], B["bc"], counts, snp_size)
+    text = oracle.write_results_text(str(wd / "out" / "report.txt.oracle"), A["pos"], A["id"],
+                                     A["bc"], B["pos"], B["id"], B["bc"], snp_size, r2,
+                                     _r2limit(args), wset2)
+    assert text.encode() == exp["report.txt"], name
+
+
+def test_cli_sample_list_matching_a_repeated_name_is_an_error(tmp_path):
+    """test_covid's header names one sample twice.  A sample list naming it
+    selects two columns for one name, which the reference's makeValidList
+    asserts against (correlate_IO.c:287-288); here the command fails with a
+    message instead."""
+    wd = _covid_workdir(tmp_path)
+    (wd / "dup.txt").write_text("HCOV-19_SPAIN_20200326_EPI_ISL_436284\n"
+                                "HCOV-19_SPAIN_20200324_EPI_ISL_436287\n")
+    r = subprocess.run([CLI_BIN, "-input", "data/covid_19", "-ploidy", "haploid",
+                        "-sampleList", "dup.txt", "-output", "out/report.txt",
+                        "-dump-tables"], cwd=wd, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "2 names select 3 sample columns" in r.stderr + r.stdout
+    assert os.listdir(wd / "out") == []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,engine",
+                         [(n, "auto") for n in COVID_CASES] +
+                         [("test_run", e) for e in ("popcount", "mfma_fp4", "mfma_i8")])
+def test_cli_covid_reports_equal_reference_on_gpu(name, engine, tmp_path):
+    wd = _covid_workdir(tmp_path)
+    print(f"[cli-covid] {name} engine={engine}", flush=True)
+    r = subprocess.run([CLI_BIN] + COVID_MANIFEST["cases"][name] +
+                       ["-output", "out/report.txt", "-engine", engine],
+                       cwd=wd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (name, r.stderr)
+    exp = _covid_expected(name)
+    got = {f: open(wd / "out" / f, "rb").read() for f in sorted(os.listdir(wd / "out"))}
+    got["input_report.txt"] = open(wd / "input_report.txt", "rb").read()
+    assert sorted(got) == sorted(exp), name
+    for f in exp:
+        assert got[f] == exp[f], (name, f, engine)
