@@ -86,7 +86,10 @@ const char *qld_gpu_last_error(const qld_ctx *ctx);
  * gemm_gpu.c:753-845: gpu_gemm + get_pairwise_ld_score_gpu).  Host
  * buffers, synchronous.  A: m x words, B: n x words (B may equal A);
  * triangular requires A == B content and m == n.  results (n*m floats) and
- * counts (n*m uint32) are each optional (NULL = not wanted). */
+ * counts (n*m uint32) are each optional (NULL = not wanted).  snp_size is
+ * the N of the frequencies, 1 .. 2^24 - 1: the header's sample count, as
+ * correlate_function_gpu passes it (correlate_function_gpu.c:240-275), which
+ * exceeds the rows' bits when a sample list selects fewer samples. */
 int qld_gpu_correlate(qld_ctx *ctx,
                       const uint64_t *A, const uint32_t *bcA, int m,
                       const uint64_t *B, const uint32_t *bcB, int n,

@@ -66,8 +66,7 @@ static int table_create(qld_ctx *ctx, const void *rows_src, const void *bc_src,
                         qld_table **out)
 {
     if (!ctx || !out || !rows_src || !bc_src || rows <= 0 || words <= 0 ||
-        snp_size <= 0 || snp_size >= QLD_MAX_SNP_SIZE ||
-        (int64_t)words * 64 < (int64_t)snp_size - 63) {
+        snp_size <= 0 || snp_size >= QLD_MAX_SNP_SIZE) {
         if (ctx) qld_set_error(ctx, "qld_table_create: bad arguments (rows=%d words=%d N=%d)",
                                rows, words, snp_size);
         return QLD_ERR_INVALID;

@@ -291,9 +291,11 @@ static int check_job(qld_ctx *ctx, const LdJob &j)
         qld_set_error(ctx, "bad shape m=%d n=%d words=%d", j.m, j.n, j.words);
         return QLD_ERR_INVALID;
     }
-    if (j.snp_size <= 0 || j.snp_size >= QLD_MAX_SNP_SIZE ||
-        (int64_t)j.words * 64 < (int64_t)j.snp_size - 63) {
-        qld_set_error(ctx, "bad snp_size %d for %d words", j.snp_size, j.words);
+    // snp_size is the N of the frequencies (the header's sample count, as
+    // the reference passes it: with a sample list the rows hold fewer bits
+    // than N).  It only has to keep the r^2 arithmetic exact.
+    if (j.snp_size <= 0 || j.snp_size >= QLD_MAX_SNP_SIZE) {
+        qld_set_error(ctx, "bad snp_size %d (1 .. %d)", j.snp_size, QLD_MAX_SNP_SIZE - 1);
         return QLD_ERR_INVALID;
     }
     if (j.tri && j.m != j.n) {

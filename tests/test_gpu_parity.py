@@ -216,6 +216,32 @@ def test_largest_sample_count(engine, mode, engine_factory):
 
 
 @pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_sample_count_above_row_bits(engine, mode, engine_factory):
+    """With a sample list the reference packs only the listed samples but
+    still divides by the header's sample count (snp_size = 22,554 for rows
+    of 5,638 bits, as in its own test data): triangle, rectangle and the
+    selection path against the oracle."""
+    N, bits = 22554, 5638
+    A, bc = qld_amd.synth_blocks(31, 300, bits)
+    B, bB = qld_amd.synth_blocks(32, 170, bits)
+    e = engine_factory(engine=engine, r2_mode=mode)
+    say("N above row bits", engine, mode)
+    res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+    _check(res, cnt, A, bc, A, bc, N, True, mode)
+    res, cnt = e.correlate(A, bc, B, bB, snp_size=N, triangular=False, counts=True)
+    _check(res, cnt, A, bc, B, bB, N, False, mode)
+    offs, cols, r2 = e.correlate_hits(A, bc, snp_size=N, upper=True, r2limit=0.05)
+    full, _ = e.correlate(A, bc, snp_size=N, triangular=True)
+    for i in range(A.shape[0]):
+        c = cols[offs[i]:offs[i + 1]]
+        j = np.arange(i + 1, A.shape[0])
+        want = j[(full[i, j] >= 0.05) & (full[i, j] < 100)]
+        assert (c == want).all()
+        assert (r2[offs[i]:offs[i + 1]].view(np.uint32) == full[i, want].view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
 def test_empty_inputs(engine, engine_factory):
     """Zero rows on either side: no work, no error, empty outputs; the
     context stays usable."""
@@ -277,9 +303,10 @@ def test_errors_are_status_codes(engine_factory):
     with pytest.raises(qld_amd.QldError) as x:
         e.correlate(A, bc, B, bB, snp_size=100, triangular=True)
     assert x.value.code == qld_amd.QLD_ERR_INVALID
-    with pytest.raises(qld_amd.QldError) as x:
-        e.correlate(A, bc, snp_size=64 * A.shape[1] + 64)
-    assert x.value.code == qld_amd.QLD_ERR_INVALID
+    for bad_n in (0, -5, 1 << 24):
+        with pytest.raises(qld_amd.QldError) as x:
+            e.correlate(A, bc, snp_size=bad_n)
+        assert x.value.code == qld_amd.QLD_ERR_INVALID
     small = engine_factory(engine="popcount", hbm_budget=1 << 10)
     with pytest.raises(qld_amd.QldError) as x:
         small.alloc(1 << 20)

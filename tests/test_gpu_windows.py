@@ -39,6 +39,30 @@ def _check_windows(out, wins, A, bc, N, mode="reference"):
 
 
 @pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x", "mfma_i8"])
+def test_window_plan_sample_count_above_row_bits(engine, engine_factory):
+    """A resident table whose N (the header's sample count, 22,554) exceeds
+    its rows' bits (5,638 listed samples), as qLD-compute -inputList builds
+    with a sample list: windows against the oracle with that N."""
+    N, bits, rows = 22554, 5638, 900
+    A, bc = qld_amd.synth_blocks(4343, rows, bits)
+    wins = qld_amd.window_layout(np.array([0, 37, 300, 512], np.int64),
+                                 np.array([300, 129, 388, 257], np.int64))
+    total = int((wins["size"].astype(np.int64) ** 2).sum())
+    e = engine_factory(engine=engine, r2_mode="reference")
+    t = e.table(A, bc, N)
+    out = e.alloc(total * 4)
+    plan = e.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
+    say("plan N above row bits", engine)
+    out.fill(0x7FC00000)
+    plan.run()
+    e.sync()
+    _check_windows(out.download(total, np.float32), wins, A, bc, N)
+    plan.free()
+    t.free()
+    out.free()
+
+
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x", "mfma_i8"])
 @pytest.mark.parametrize("mode", [qld_amd.BATCH_PERSISTENT, qld_amd.BATCH_PLAIN])
 @pytest.mark.parametrize("cutoff", [300, 0, 1 << 30])
 def test_window_plans(engine, mode, cutoff, engine_factory):
