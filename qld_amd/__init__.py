@@ -309,6 +309,87 @@ class MdfReader:
             pass
 
 
+def _table_dict(t: "_SnpTable") -> dict:
+    r, w = t.rows, t.words
+    snp = np.ctypeslib.as_array(C.cast(t.snp, C.POINTER(C.c_uint64)), (max(r, 1) * w,))
+    pos = np.ctypeslib.as_array(C.cast(t.pos, C.POINTER(C.c_uint32)), (max(r, 1),))
+    bc = np.ctypeslib.as_array(C.cast(t.bc, C.POINTER(C.c_uint32)), (max(r, 1),))
+    idp = C.cast(t.id, C.POINTER(C.c_char_p))
+    return {"snp": snp[:r * w].reshape(r, w).copy(), "pos": pos[:r].copy(),
+            "bc": bc[:r].copy(), "id": [idp[i].decode() for i in range(r)]}
+
+
+class VcfReader:
+    """qld_vcf_reader over the parser's VCF chunks of one header file: rows
+    as the correlator builds them without -mdf (writeToTable_x64), with the
+    header's first and #CHROM lines and the sample selection of `sample_list`
+    (None = every sample).  Reads draw unphased shuffles from the reader's own
+    rand() stream, in call order."""
+
+    def __init__(self, header_path: str, ploidy: int, sample_list: str | None = None):
+        l1, ch = C.c_void_p(), C.c_void_p()
+        rc = host_lib().qld_vcf_header_lines(header_path.encode(), C.byref(l1), C.byref(ch))
+        if rc != QLD_OK:
+            raise QldError(rc, f"header lines of {header_path}: "
+                               f"{(host_lib().qld_vcf_last_error() or b'').decode()}")
+        free = libc_free()
+        self.line1, self.chrom_line = C.string_at(l1.value), C.string_at(ch.value)
+        free(l1)
+        free(ch)
+        self._mask = C.c_void_p()
+        ns, vc = C.c_int(), C.c_int()
+        rc = host_lib().qld_vcf_sample_mask(self.chrom_line,
+                                            sample_list.encode() if sample_list else None,
+                                            ploidy, C.byref(self._mask), C.byref(ns),
+                                            C.byref(vc))
+        if rc != QLD_OK:
+            raise QldError(rc, f"sample selection: "
+                               f"{(host_lib().qld_vcf_last_error() or b'').decode()}")
+        self.nsamples, self.valid_count, self.ploidy = ns.value, vc.value, int(ploidy)
+        self._r = host_lib().qld_vcf_reader_create()
+        if not self._r:
+            raise QldError(QLD_ERR_NOMEM, "qld_vcf_reader_create")
+
+    def table(self, files: list[str], pos_wmin: int, pos_wmax: int, snip_size: int,
+              pos_wset: int = 1) -> dict:
+        """Rows of `files` in [pos_wmin, pos_wmax] -> dict(snp, pos, bc, id)."""
+        arr = (C.c_char_p * max(1, len(files)))(*[f.encode() for f in files])
+        t = _SnpTable()
+        rc = host_lib().qld_vcf_read_table(self._r, C.cast(arr, C.c_void_p), len(files),
+                                           pos_wmin, pos_wmax, pos_wset, self.line1,
+                                           self.chrom_line, self._mask, self.nsamples,
+                                           self.valid_count, self.ploidy, int(snip_size),
+                                           C.byref(t))
+        if rc != QLD_OK:
+            raise QldError(rc, f"reading VCF rows: "
+                               f"{(host_lib().qld_vcf_last_error() or b'').decode()}")
+        try:
+            return _table_dict(t)
+        finally:
+            host_lib().qld_snp_table_free(C.byref(t))
+
+    def close(self):
+        if self._r:
+            host_lib().qld_vcf_reader_free(self._r)
+            self._r = None
+        if self._mask:
+            libc_free()(self._mask)
+            self._mask = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def libc_free():
+    f = C.CDLL(None).free
+    f.argtypes = [C.c_void_p]
+    f.restype = None
+    return f
+
+
 def mdf_read_table(files: list[str], pos_wmin: int, pos_wmax: int, words: int,
                    pos_wset: int = 1, nthreads: int = 0, reader: "MdfReader | None" = None) -> dict:
     """Rows of `files` in [pos_wmin, pos_wmax] -> dict(snp, pos, bc, id)."""
@@ -324,16 +405,9 @@ def mdf_read_table(files: list[str], pos_wmin: int, pos_wmax: int, words: int,
                                              C.byref(t))
     _host_check(rc, "reading MDF rows")
     try:
-        r, w = t.rows, t.words
-        snp = np.ctypeslib.as_array(C.cast(t.snp, C.POINTER(C.c_uint64)), (max(r, 1) * w,))
-        pos = np.ctypeslib.as_array(C.cast(t.pos, C.POINTER(C.c_uint32)), (max(r, 1),))
-        bc = np.ctypeslib.as_array(C.cast(t.bc, C.POINTER(C.c_uint32)), (max(r, 1),))
-        idp = C.cast(t.id, C.POINTER(C.c_char_p))
-        out = {"snp": snp[:r * w].reshape(r, w).copy(), "pos": pos[:r].copy(),
-               "bc": bc[:r].copy(), "id": [idp[i].decode() for i in range(r)]}
+        return _table_dict(t)
     finally:
         host_lib().qld_snp_table_free(C.byref(t))
-    return out
 
 
 def mdf_write(path: str, chrom: str, snp: np.ndarray, pos, ids, bc) -> None:

@@ -450,7 +450,13 @@ static mdf_file *reader_file(qld_mdf_reader *r, const char *path)
 
 /* Parse data line k of f into its row slot: POS, ID, BC and the packed words
  * (strtoul per word, as writeMDF_x64 does, with a fast path for plain digit
- * strings). */
+ * strings).  A row may carry more than W words -- the MDF tool writes
+ * ploidy-2 widths unless told otherwise, and the reference's own smoke run
+ * then reads haploid rows from it: writeMDF_x64 (correlate_IOx64.c:102-115)
+ * stores every word from the row's first slot on, the surplus lands in the
+ * next row's slots, which that row overwrites, so the table keeps each row's
+ * first W words.  Fewer than W words leave slots the reference never writes:
+ * an error here. */
 static void parse_line(mdf_file *f, int k)
 {
     char *p = f->line[k];
@@ -470,13 +476,7 @@ static void parse_line(mdf_file *f, int k)
     f->bc[k] = (uint32_t)atoi_word(wbc);
     uint64_t *row = f->snp + (size_t)k * W;
     int n = 0;
-    while ((w = next_word(&p, &wl)) != NULL) {
-        if (n == W) {
-            n = W + 1;
-            break;
-        }
-        row[n++] = u64_word(w, wl);
-    }
+    while (n < W && (w = next_word(&p, &wl)) != NULL) row[n++] = u64_word(w, wl);
     if (n != W) {
         f->parsed[k] = 3;
         return;
@@ -601,8 +601,8 @@ int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int 
                           q, f->pos[k]);
             else if (f->parsed[k] == 3)
                 rc = fail(QLD_ERR_INVALID,
-                          "%s: row %ld (POS %d): the number of packed words differs from the "
-                          "%d the header's samples imply", f->path, q, f->pos[k], words);
+                          "%s: row %ld (POS %d): fewer packed words than the %d the "
+                          "header's samples imply", f->path, q, f->pos[k], words);
         }
     }
     /* assemble */

@@ -14,7 +14,14 @@ ref-parser ref-cli``):
    matches two columns trips the reference's assertion in makeValidList,
    correlate_IO.c:287) -> ``covid/data/samples_covid.txt``;
 3. for each case the report and ``input_report.txt`` of the reference
-   correlator's CPU build, run without -mdf -> ``covid/expected/<case>/``.
+   correlator's CPU build, run without -mdf -> ``covid/expected/<case>/``;
+4. the smoke script's MDF step: qLD-parse-2MDF with its default options
+   (``make -C oracle ref-2mdf``; rows of the ploidy-2 width, 705 words for
+   22,554 samples) into a scratch directory, the SHA-256 of every file it
+   writes -> ``cases.json`` (``mdf``), and the reference correlator's -mdf
+   reports on it -> ``covid/expected/<case>/`` for the ``mdf_cases``.  The
+   8 MB MDF directory itself is not kept: the tests rebuild it with the
+   VCF reader and MDF writer here and check it against these hashes.
 
 Only this container has the reference checkout; everything written is data.
 Usage: python tests/golden/make_covid_golden.py
@@ -23,6 +30,7 @@ from __future__ import annotations
 
 import collections
 import gzip
+import hashlib
 import json
 import os
 import shutil
@@ -35,6 +43,7 @@ OUT = os.path.join(HERE, "covid")
 REF_VCF = "/root/reference/test_dataset/vcf/test_covid.vcf.gz"
 REF_PARSER = os.path.join(ROOT, "oracle", "_ref", "qLD-parse-VCF")
 REF_CLI = os.path.join(ROOT, "oracle", "_ref", "qLD-compute")
+REF_2MDF = os.path.join(ROOT, "oracle", "_ref", "qLD-parse-2MDF")
 
 CASES = {
     # test_run's own compute flags
@@ -45,6 +54,15 @@ CASES = {
                     "-posWmin2", "9000", "-posWmax2", "25000", "-r2limit", "0.05"],
     "samples": ["-input", "data/covid_19", "-ploidy", "haploid",
                 "-sampleList", "data/samples_covid.txt", "-r2limit", "0.3"],
+}
+
+
+MDF_CASES = {
+    # test_run's compute step on its own MDF directory
+    "test_run_mdf": ["-input", "mdf/covid_19", "-r2limit", "0.6", "-ploidy", "haploid", "-mdf"],
+    "two_windows_mdf": ["-input", "mdf/covid_19", "-ploidy", "haploid", "-mdf",
+                        "-posWmin1", "3000", "-posWmax1", "12000",
+                        "-posWmin2", "9000", "-posWmax2", "25000", "-r2limit", "0.05"],
 }
 
 
@@ -67,7 +85,20 @@ def parse():
         f.write("\n".join(once[::4]) + "\n")
 
 
-def run_reference(name, args):
+def make_mdf(scratch):
+    """qLD-parse-2MDF as test_run:59-65 runs it; returns {file: sha256}."""
+    shutil.rmtree(scratch, ignore_errors=True)
+    os.makedirs(scratch)
+    mdf = os.path.join(scratch, "covid_19")
+    r = subprocess.run([REF_2MDF, "-input", os.path.join(OUT, "data", "covid_19"),
+                        "-output", mdf], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise SystemExit(f"qLD-parse-2MDF exited {r.returncode}\n{r.stderr[-2000:]}")
+    return {fn: hashlib.sha256(open(os.path.join(mdf, fn), "rb").read()).hexdigest()
+            for fn in sorted(os.listdir(mdf))}
+
+
+def run_reference(name, args, mdf_scratch=None):
     out = os.path.join(OUT, "expected", name)
     shutil.rmtree(out, ignore_errors=True)
     os.makedirs(out)
@@ -75,6 +106,8 @@ def run_reference(name, args):
     shutil.rmtree(work, ignore_errors=True)
     os.makedirs(os.path.join(work, "out"))
     os.symlink(os.path.join(OUT, "data"), os.path.join(work, "data"))
+    if mdf_scratch:
+        os.symlink(mdf_scratch, os.path.join(work, "mdf"))
     cmd = [REF_CLI] + args + ["-output", "out/report.txt", "-threads", "1"]
     r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
@@ -86,15 +119,23 @@ def run_reference(name, args):
 
 
 def main():
-    for b in (REF_PARSER, REF_CLI):
+    for b in (REF_PARSER, REF_CLI, REF_2MDF):
         if not os.path.exists(b):
-            raise SystemExit("build the reference tools first: make -C oracle ref-parser ref-cli")
+            raise SystemExit("build the reference tools first: "
+                             "make -C oracle ref-parser ref-cli ref-2mdf")
     parse()
     for name, args in CASES.items():
         run_reference(name, args)
         print(name, sorted(os.listdir(os.path.join(OUT, "expected", name))), flush=True)
+    scratch = os.path.join(OUT, "_mdf")
+    hashes = make_mdf(scratch)
+    for name, args in MDF_CASES.items():
+        run_reference(name, args, scratch)
+        print(name, sorted(os.listdir(os.path.join(OUT, "expected", name))), flush=True)
+    shutil.rmtree(scratch)
     with open(os.path.join(OUT, "cases.json"), "w") as f:
-        json.dump({"cases": CASES}, f, indent=1)
+        json.dump({"cases": CASES, "mdf_cases": MDF_CASES,
+                   "mdf": {"dir": "covid_19", "words": 705, "sha256": hashes}}, f, indent=1)
 
 
 if __name__ == "__main__":

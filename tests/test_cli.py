@@ -199,18 +199,41 @@ def test_mdf_find_files_window_past_last_chunk_is_an_error():
         qld_amd.mdf_find_files(d, "chrA", 25, 96, 1000, 10**6)
 
 
-def test_mdf_row_with_wrong_word_count_is_an_error(tmp_path):
+def _chrB_chunk():
     src = os.path.join(CLI_DIR, "data", "chrB", "chrB_61_70_6181_6393.mdf")
     if not os.path.exists(src):
         src = sorted(os.path.join(CLI_DIR, "data", "chrB", f)
                      for f in os.listdir(os.path.join(CLI_DIR, "data", "chrB"))
                      if f.endswith(".mdf"))[-1]
-    text = open(src).read().split("\n")
+    return src
+
+
+def test_mdf_row_with_too_few_words_is_an_error(tmp_path):
+    text = open(_chrB_chunk()).read().split("\n")
     text[4] = text[4].rsplit("\t", 1)[0]          # drop one word of the second row
     bad = tmp_path / "bad.mdf"
     bad.write_text("\n".join(text))
-    with pytest.raises(qld_amd.QldError, match="packed words"):
+    with pytest.raises(qld_amd.QldError, match="fewer packed words"):
         qld_amd.mdf_read_table([str(bad)], 1, 2**31 - 1, 3)
+
+
+def test_mdf_rows_wider_than_the_ploidy_keep_their_first_words(tmp_path):
+    """The MDF tool writes ploidy-2 rows unless told otherwise, and the
+    reference's smoke run reads them as haploid (test_run:59-92): its row
+    loop keeps each row's first words (correlate_IOx64.c:102-115), so wider
+    rows -- zero or not past the header's width -- load as their first
+    words."""
+    src = _chrB_chunk()
+    want = qld_amd.mdf_read_table([src], 1, 2**31 - 1, 3)
+    text = open(src).read().split("\n")
+    for k in range(3, len(text)):
+        text[k] += "\t0\t0\t0" if k % 2 else "\t18446744073709551615\t7"
+    wide = tmp_path / "wide.mdf"
+    wide.write_text("\n".join(text))
+    got = qld_amd.mdf_read_table([str(wide)], 1, 2**31 - 1, 3)
+    for key in ("snp", "pos", "bc"):
+        assert (got[key] == want[key]).all(), key
+    assert got["id"] == want["id"]
 
 
 @pytest.mark.gpu
@@ -289,8 +312,11 @@ def test_mdf_reader_cache_matches_one_shot_reads():
             t2 = reader.table(files, int(lo), int(hi), 3, pos_wset=wset)
             assert (t1["snp"] == t2["snp"]).all() and t1["id"] == t2["id"]
             assert (t1["pos"] == t2["pos"]).all() and (t1["bc"] == t2["bc"]).all()
-    with pytest.raises(qld_amd.QldError, match="packed words"):
-        reader.table(files, h.pos_min, h.pos_max, 2)     # other packing: re-parsed, rejected
+    with pytest.raises(qld_amd.QldError, match="fewer packed words"):
+        reader.table(files, h.pos_min, h.pos_max, 4)     # wider packing: re-parsed, rejected
+    narrow = reader.table(files, h.pos_min, h.pos_max, 2)   # narrower: each row's first words
+    full = qld_amd.mdf_read_table(files, h.pos_min, h.pos_max, 3)
+    assert (narrow["snp"] == full["snp"][:, :2]).all()
     t = reader.table(files, int(lo), int(hi), 3)
     assert len(t["pos"]) == len(qld_amd.mdf_read_table(files, int(lo), int(hi), 3)["pos"])
     reader.close()
@@ -428,6 +454,83 @@ def test_cli_sample_list_matching_a_repeated_name_is_an_error(tmp_path):
     assert r.returncode != 0
     assert "2 names select 3 sample columns" in r.stderr + r.stdout
     assert os.listdir(wd / "out") == []
+
+
+COVID_MDF_CASES = sorted(COVID_MANIFEST["mdf_cases"])
+
+
+@pytest.fixture(scope="module")
+def covid_mdf(tmp_path_factory):
+    """The smoke script's MDF directory (test_run:59-65, qLD-parse-2MDF with
+    its default ploidy-2 width), rebuilt here by the VCF reader and the MDF
+    writer: every file must hash as the reference converter's did."""
+    import hashlib
+    m = COVID_MANIFEST["mdf"]
+    root = tmp_path_factory.mktemp("covid_mdf")
+    src = os.path.join(COVID_DIR, "data", m["dir"])
+    dst = root / "mdf" / m["dir"]
+    os.makedirs(dst)
+    hdr = next(f for f in sorted(os.listdir(src)) if "header" in f)
+    shutil.copy(os.path.join(src, hdr), dst / hdr)
+    rd = qld_amd.VcfReader(os.path.join(src, hdr), 1)
+    for fn in sorted(os.listdir(src)):
+        if fn == hdr:
+            continue
+        t = rd.table([os.path.join(src, fn)], 0, 2**31 - 1, 22554)
+        snp = np.zeros((len(t["pos"]), m["words"]), np.uint64)
+        snp[:, :t["snp"].shape[1]] = t["snp"]
+        qld_amd.mdf_write(str(dst / (fn[:-len(".vcf.gz")] + ".mdf")), "1", snp, t["pos"],
+                          t["id"], t["bc"])
+    rd.close()
+    got = {fn: hashlib.sha256(open(dst / fn, "rb").read()).hexdigest()
+           for fn in sorted(os.listdir(dst))}
+    assert got == m["sha256"]
+    return root
+
+
+def _covid_mdf_workdir(tmp_path, covid_mdf):
+    os.symlink(covid_mdf / "mdf", tmp_path / "mdf")
+    os.makedirs(tmp_path / "out")
+    return tmp_path
+
+
+@pytest.mark.parametrize("name", COVID_MDF_CASES)
+def test_cli_covid_mdf_tables_reproduce_reference_report(name, covid_mdf, tmp_path):
+    """The smoke script's compute step on its MDF directory (-mdf -ploidy
+    haploid over 705-word rows): the CLI's tables through the oracle give
+    the reference's report byte for byte."""
+    args = COVID_MANIFEST["mdf_cases"][name]
+    wd = _covid_mdf_workdir(tmp_path, covid_mdf)
+    r = subprocess.run([CLI_BIN] + args + ["-output", "out/report.txt", "-dump-tables"],
+                       cwd=wd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    exp = _covid_expected(name)
+    assert open(wd / "input_report.txt", "rb").read() == exp["input_report.txt"]
+    single, wset2, snp_size, A, B = _read_tables(wd / "out" / "report.txt.tables")
+    assert A["words"] == 353
+    counts = oracle.counts_blocked(A["snp"], B["snp"], tri=False)
+    r2 = oracle.r2_reference(A["bc"], B["bc"], counts, snp_size)
+    text = oracle.write_results_text(str(wd / "out" / "report.txt.oracle"), A["pos"], A["id"],
+                                     A["bc"], B["pos"], B["id"], B["bc"], snp_size, r2,
+                                     _r2limit(args), wset2)
+    assert text.encode() == exp["report.txt"], name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", COVID_MDF_CASES)
+def test_cli_covid_mdf_reports_equal_reference_on_gpu(name, covid_mdf, tmp_path):
+    wd = _covid_mdf_workdir(tmp_path, covid_mdf)
+    print(f"[cli-covid-mdf] {name}", flush=True)
+    r = subprocess.run([CLI_BIN] + COVID_MANIFEST["mdf_cases"][name] +
+                       ["-output", "out/report.txt"],
+                       cwd=wd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (name, r.stderr)
+    exp = _covid_expected(name)
+    got = {f: open(wd / "out" / f, "rb").read() for f in sorted(os.listdir(wd / "out"))}
+    got["input_report.txt"] = open(wd / "input_report.txt", "rb").read()
+    assert sorted(got) == sorted(exp), name
+    for f in exp:
+        assert got[f] == exp[f], (name, f)
 
 
 @pytest.mark.gpu
