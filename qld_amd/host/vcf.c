@@ -63,6 +63,11 @@ static int vrand(qld_vcf_reader *r)
     return (int)v;
 }
 
+/* Returned by the line parser run without a rand() stream (reader NULL) for
+ * a line that would draw from it: that line is parsed again, in order, with
+ * the reader's stream. */
+#define VDEFER (-1000)
+
 /* ---- text helpers (read_file.c semantics) ----------------------------- */
 
 typedef struct {
@@ -395,15 +400,17 @@ static int gt_field(char *s, int pos)
 }
 
 /* dataShuffleKnuth (:337-353) with the reader's rand() stream */
-static void shuffle(qld_vcf_reader *r, char *d, int start, int end)
+static int shuffle(qld_vcf_reader *r, char *d, int start, int end)
 {
-    if (start == end) return;
+    if (start == end) return QLD_OK;
+    if (!r) return VDEFER;
     for (int i = end; i > start; i--) {
         int idx = start + (vrand(r) % (i - start + 1));
         char t = d[idx];
         d[idx] = d[i];
         d[i] = t;
     }
+    return QLD_OK;
 }
 
 /* getGTdata (:355-393) */
@@ -424,14 +431,14 @@ static int gt_data(qld_vcf_reader *r, const char *s, const char *sv, int total, 
             if (s[i] == '/') end++;
             if (s[i] == '|') {
                 if (end >= 4) return vfail(QLD_ERR_INVALID, "genotype %s too long", s);
-                shuffle(r, out, start, end);
+                if (shuffle(r, out, start, end)) return VDEFER;
                 start = j;
                 end = j;
             }
         }
     }
     if (end >= 4) return vfail(QLD_ERR_INVALID, "genotype %s too long", s);
-    shuffle(r, out, start, end);
+    if (shuffle(r, out, start, end)) return VDEFER;
     return j;
 }
 
@@ -508,6 +515,7 @@ static int binary_deduction(qld_vcf_reader *r, int type, int n, char *line)
         }
     }
     if (st[2] + st[3] != 0) {
+        if (!r) return VDEFER;
         char rep[4];
         int total = st[0] + st[1];
         for (int i = 0; i < 2; ++i) {
@@ -546,7 +554,20 @@ typedef struct {
     char **id;
 } vtab;
 
-static int vtab_push(vtab *t, int pos, const char *id, const char *snip, int n)
+/* compressSnip_x64 (:783-815): '1' -> bit 63 - i%64 of word i/64; returns BC */
+static uint32_t pack_row(const char *snip, int n, uint64_t *w, int words)
+{
+    memset(w, 0, (size_t)words * 8);
+    uint32_t bc = 0;
+    for (int i = 0; i < n; i++) {
+        uint64_t bit = (uint64_t)((unsigned char)snip[i] & 1u);
+        w[i / 64] |= bit << (63 - i % 64);
+        bc += (uint32_t)bit;
+    }
+    return bc;
+}
+
+static int vtab_push(vtab *t, int pos, const char *id, const uint64_t *row, uint32_t bc)
 {
     if (t->rows == t->cap) {
         int nc = t->cap ? t->cap * 2 : 1024;
@@ -561,14 +582,7 @@ static int vtab_push(vtab *t, int pos, const char *id, const char *snip, int n)
         if (!ns || !np || !nb || !ni) return vfail(QLD_ERR_NOMEM, "out of host memory");
         t->cap = nc;
     }
-    uint64_t *w = t->snp + (size_t)t->rows * t->words;
-    memset(w, 0, (size_t)t->words * 8);
-    uint32_t bc = 0;
-    for (int i = 0; i < n; i++) {
-        uint64_t bit = (uint64_t)((unsigned char)snip[i] & 1u);
-        w[i / 64] |= bit << (63 - i % 64);
-        bc += (uint32_t)bit;
-    }
+    memcpy(t->snp + (size_t)t->rows * t->words, row, (size_t)t->words * 8);
     char *idc = strdup(id);
     if (!idc) return vfail(QLD_ERR_NOMEM, "out of host memory");
     t->pos[t->rows] = (uint32_t)pos;
@@ -578,21 +592,23 @@ static int vtab_push(vtab *t, int pos, const char *id, const char *snip, int n)
     return QLD_OK;
 }
 
-/* One data line: 1 row kept, 0 skipped, <0 error. */
+/* One data line: 1 row (the site's valid_count coded bases in snip, POS in
+ * *pos_out, ID in id[256]), 0 skipped, <0 error; VDEFER when r is NULL and
+ * the line would draw from the rand() stream.  The line text is only read. */
 static int vcf_line(qld_vcf_reader *r, char *line, const unsigned char *mask, int nmask,
-                    int valid_count, int ploidy, int snip_size, vtab *t, char *word,
-                    size_t wcap, char *snip)
+                    int valid_count, int ploidy, int snip_size, char *word, size_t wcap,
+                    char *snip, int *pos_out, char *id)
 {
     char *p = line;
     int eol = 0, el = -1, skipped = 0, position = -1, gtpos = -1, sref = 0, salt = 0;
     char sv[MAXST];
-    char id[256] = "";
+    id[0] = '\0';
     for (int j = 0; j < MAXST; j++) sv[j] = 'X';
     while (vword(&p, word, wcap, &eol)) {
         el++;
         switch (el) {
         case 1: position = atoi(word); break;
-        case 2: snprintf(id, sizeof id, "%s", word); break;
+        case 2: snprintf(id, 256, "%s", word); break;
         case 3: {
             if (strlen(word) > 1) { skipped = 1; break; }
             int s = get_states(word, sv, 0, position);
@@ -671,8 +687,8 @@ static int vcf_line(qld_vcf_reader *r, char *line, const unsigned char *mask, in
     if (non_polymorphic(snip, n, type)) return 0;
     int rc = binary_deduction(r, type, n, snip);
     if (rc) return rc;
-    rc = vtab_push(t, position, id, snip, n);
-    return rc ? rc : 1;
+    *pos_out = position;
+    return 1;
 }
 
 int qld_vcf_read_table(qld_vcf_reader *r, char *const *files, int nfiles, int pos_wmin,
@@ -685,18 +701,39 @@ int qld_vcf_read_table(qld_vcf_reader *r, char *const *files, int nfiles, int po
         return vfail(QLD_ERR_INVALID, "invalid argument");
     memset(out, 0, sizeof *out);
     vtab t = {0, 0, (valid_count + 63) / 64, NULL, NULL, NULL, NULL};
+    const int W = t.words;
     char *snip = (char *)malloc((size_t)valid_count + 8);
-    int rc = snip ? QLD_OK : vfail(QLD_ERR_NOMEM, "out of host memory");
+    uint64_t *row = (uint64_t *)malloc((size_t)W * 8);
+    char id[256];
+    int rc = snip && row ? QLD_OK : vfail(QLD_ERR_NOMEM, "out of host memory");
+    /* files are inflated VBATCH at a time on separate threads, then parsed
+     * in order; a file that fails to load is loaded again here so that the
+     * error (and its message) is the calling thread's */
+    enum { VBATCH = 8 };
+    char *bufs[VBATCH] = {0};
+    size_t lens[VBATCH] = {0};
+    int brc[VBATCH] = {0};
     for (int f = 0; f < nfiles && !rc; f++) {
-        char *buf;
-        size_t len;
-        if ((rc = vslurp(files[f], &buf, &len))) break;
+        const int slot = f % VBATCH;
+        if (slot == 0) {
+            const int nb = nfiles - f < VBATCH ? nfiles - f : VBATCH;
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int b = 0; b < nb; b++) brc[b] = vslurp(files[f + b], &bufs[b], &lens[b]);
+        }
+        char *buf = bufs[slot];
+        size_t len = lens[slot];
+        bufs[slot] = NULL;
+        if (brc[slot] && (rc = vslurp(files[f], &buf, &len))) break;
         size_t wcap = len + 1;
         char *word = (char *)malloc(wcap);
         if (!word) { free(buf); rc = vfail(QLD_ERR_NOMEM, "out of host memory"); break; }
         vcursor c = {buf, buf + len};
         char *line;
         int eof;
+        /* the lines readTable_x64's loop hands to writeToTable_x64, in order */
+        char **sel = NULL;
+        int nsel = 0, cap = 0;
+        size_t maxlen = 0;
         if (!vnext_line(&c, &line, &eof)) {
             rc = vfail(QLD_ERR_INVALID, "File %s empty", files[f]);
         } else if (strcmp(line, line1)) {
@@ -717,9 +754,16 @@ int qld_vcf_read_table(qld_vcf_reader *r, char *const *files, int nfiles, int po
                         }
                         pos = atoi(word);
                         if (pos >= pos_wmin && pos <= pos_wmax) {
-                            int k = vcf_line(r, line, mask, nsamples, valid_count, ploidy,
-                                             snip_size, &t, word, wcap, snip);
-                            if (k < 0) { rc = k; break; }
+                            if (nsel == cap) {
+                                int nc = cap ? 2 * cap : 1024;
+                                char **ns = (char **)realloc(sel, (size_t)nc * sizeof *sel);
+                                if (!ns) { rc = vfail(QLD_ERR_NOMEM, "out of host memory"); break; }
+                                sel = ns;
+                                cap = nc;
+                            }
+                            sel[nsel++] = line;
+                            size_t l = strlen(line);
+                            if (l > maxlen) maxlen = l;
                         }
                     } else {
                         pos = 0;
@@ -727,9 +771,69 @@ int qld_vcf_read_table(qld_vcf_reader *r, char *const *files, int nfiles, int po
                 } while (!eof && pos_wset && pos < pos_wmax);
             }
         }
+        /* Lines are parsed on every core without the rand() stream; a line
+         * that would draw from it (unphased genotypes, sites with more than
+         * two bases) or that fails is parsed again below, in file order, with
+         * the reader's stream, so the draws, the rows and the first error are
+         * those of a sequential read. */
+        signed char *st = NULL;
+        uint64_t *rows = NULL;
+        uint32_t *rbc = NULL, *rpos = NULL;
+        char *rid = NULL;
+        if (!rc && nsel) {
+            st = (signed char *)malloc((size_t)nsel);
+            rows = (uint64_t *)malloc((size_t)nsel * W * 8);
+            rbc = (uint32_t *)malloc((size_t)nsel * 4);
+            rpos = (uint32_t *)malloc((size_t)nsel * 4);
+            rid = (char *)malloc((size_t)nsel * 256);
+            if (!st || !rows || !rbc || !rpos || !rid)
+                rc = vfail(QLD_ERR_NOMEM, "out of host memory");
+        }
+        if (!rc && nsel) {
+#pragma omp parallel
+            {
+                char *wb = (char *)malloc(maxlen + 1);
+                char *sb = (char *)malloc((size_t)valid_count + 8);
+#pragma omp for schedule(dynamic, 8)
+                for (int k = 0; k < nsel; k++) {
+                    int pk = 0, s = VDEFER;
+                    if (wb && sb)
+                        s = vcf_line(NULL, sel[k], mask, nsamples, valid_count, ploidy,
+                                     snip_size, wb, maxlen + 1, sb, &pk, rid + (size_t)k * 256);
+                    if (s == 1) {
+                        rbc[k] = pack_row(sb, valid_count, rows + (size_t)k * W, W);
+                        rpos[k] = (uint32_t)pk;
+                    }
+                    st[k] = (signed char)(s == 1 ? 1 : s == 0 ? 0 : 2);
+                }
+                free(wb);
+                free(sb);
+            }
+            for (int k = 0; k < nsel && !rc; k++) {
+                if (st[k] == 1) {
+                    rc = vtab_push(&t, (int)rpos[k], rid + (size_t)k * 256, rows + (size_t)k * W,
+                                   rbc[k]);
+                } else if (st[k] == 2) {
+                    int pk = 0;
+                    int s = vcf_line(r, sel[k], mask, nsamples, valid_count, ploidy, snip_size,
+                                     word, wcap, snip, &pk, id);
+                    if (s < 0) rc = s;
+                    else if (s == 1)
+                        rc = vtab_push(&t, pk, id, row, pack_row(snip, valid_count, row, W));
+                }
+            }
+        }
+        free(st);
+        free(rows);
+        free(rbc);
+        free(rpos);
+        free(rid);
+        free(sel);
         free(word);
         free(buf);
     }
+    for (int b = 0; b < VBATCH; b++) free(bufs[b]);
+    free(row);
     free(snip);
     if (!rc) {
         size_t nr = t.rows ? (size_t)t.rows : 1;
