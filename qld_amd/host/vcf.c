@@ -63,10 +63,42 @@ static int vrand(qld_vcf_reader *r)
     return (int)v;
 }
 
-/* Returned by the line parser run without a rand() stream (reader NULL) for
- * a line that would draw from it: that line is parsed again, in order, with
- * the reader's stream. */
+/* Returned by the line parser when a line cannot be handled in the current
+ * draw mode (below): it is parsed again, in order, with the reader's
+ * stream. */
 #define VDEFER (-1000)
+
+/* rand() values drawn ahead of the lines that use them, at most */
+#ifndef QLD_VCF_AHEAD
+#define QLD_VCF_AHEAD (1 << 24)
+#endif
+
+/* Where a line's rand() values come from.  STREAM: the reader's stream (a
+ * sequential read).  COUNT: nothing is drawn or shuffled, the draws the line
+ * would make are counted (its row is exact only when it makes none).
+ * REPLAY: the line's own values, drawn from the stream beforehand in line
+ * order. */
+enum { VR_STREAM, VR_COUNT, VR_REPLAY };
+typedef struct {
+    int mode;
+    qld_vcf_reader *r;
+    const int32_t *vals;
+    long n, used;
+} vrng;
+
+static int vdraw(vrng *g, int *v)
+{
+    if (g->mode == VR_STREAM) {
+        *v = vrand(g->r);
+        g->used++;
+        return QLD_OK;
+    }
+    if (g->mode == VR_REPLAY && g->used < g->n) {
+        *v = g->vals[g->used++];
+        return QLD_OK;
+    }
+    return VDEFER;
+}
 
 /* ---- text helpers (read_file.c semantics) ----------------------------- */
 
@@ -399,13 +431,18 @@ static int gt_field(char *s, int pos)
     return counter + 1;
 }
 
-/* dataShuffleKnuth (:337-353) with the reader's rand() stream */
-static int shuffle(qld_vcf_reader *r, char *d, int start, int end)
+/* dataShuffleKnuth (:337-353): end - start draws */
+static int shuffle(vrng *g, char *d, int start, int end)
 {
     if (start == end) return QLD_OK;
-    if (!r) return VDEFER;
+    if (g->mode == VR_COUNT) {
+        g->used += end - start;
+        return QLD_OK;
+    }
     for (int i = end; i > start; i--) {
-        int idx = start + (vrand(r) % (i - start + 1));
+        int v;
+        if (vdraw(g, &v)) return VDEFER;
+        int idx = start + (v % (i - start + 1));
         char t = d[idx];
         d[idx] = d[i];
         d[i] = t;
@@ -414,7 +451,7 @@ static int shuffle(qld_vcf_reader *r, char *d, int start, int end)
 }
 
 /* getGTdata (:355-393) */
-static int gt_data(qld_vcf_reader *r, const char *s, const char *sv, int total, char *out)
+static int gt_data(vrng *g, const char *s, const char *sv, int total, char *out)
 {
     int j = 0, start = 0, end = 0, len = (int)strlen(s);
     for (int i = 0; i < len; i++) {
@@ -431,14 +468,14 @@ static int gt_data(qld_vcf_reader *r, const char *s, const char *sv, int total, 
             if (s[i] == '/') end++;
             if (s[i] == '|') {
                 if (end >= 4) return vfail(QLD_ERR_INVALID, "genotype %s too long", s);
-                if (shuffle(r, out, start, end)) return VDEFER;
+                if (shuffle(g, out, start, end)) return VDEFER;
                 start = j;
                 end = j;
             }
         }
     }
     if (end >= 4) return vfail(QLD_ERR_INVALID, "genotype %s too long", s);
-    if (shuffle(r, out, start, end)) return VDEFER;
+    if (shuffle(g, out, start, end)) return VDEFER;
     return j;
 }
 
@@ -491,7 +528,7 @@ static int non_polymorphic(const char *line, int n, int type)
 }
 
 /* binaryDeduction (:602-747) */
-static int binary_deduction(qld_vcf_reader *r, int type, int n, char *line)
+static int binary_deduction(vrng *g, int type, int n, char *line)
 {
     if (type == 2 || type == 3) return QLD_OK;
     int st[STATESALL_] = {0};
@@ -515,11 +552,16 @@ static int binary_deduction(qld_vcf_reader *r, int type, int n, char *line)
         }
     }
     if (st[2] + st[3] != 0) {
-        if (!r) return VDEFER;
+        if (g->mode == VR_COUNT) {
+            g->used += 2;
+            return QLD_OK;
+        }
         char rep[4];
         int total = st[0] + st[1];
         for (int i = 0; i < 2; ++i) {
-            int rv = vrand(r) % total + 1;
+            int v;
+            if (vdraw(g, &v)) return VDEFER;
+            int rv = v % total + 1;
             char major = rv <= st[0] ? alpha[0] : alpha[1];
             rep[i] = alpha[i + 2];
             rep[i + 2] = major;
@@ -593,9 +635,9 @@ static int vtab_push(vtab *t, int pos, const char *id, const uint64_t *row, uint
 }
 
 /* One data line: 1 row (the site's valid_count coded bases in snip, POS in
- * *pos_out, ID in id[256]), 0 skipped, <0 error; VDEFER when r is NULL and
- * the line would draw from the rand() stream.  The line text is only read. */
-static int vcf_line(qld_vcf_reader *r, char *line, const unsigned char *mask, int nmask,
+ * *pos_out, ID in id[256]), 0 skipped, <0 error; VDEFER when the draw mode
+ * cannot handle it.  The line text is only read. */
+static int vcf_line(vrng *g, char *line, const unsigned char *mask, int nmask,
                     int valid_count, int ploidy, int snip_size, char *word, size_t wcap,
                     char *snip, int *pos_out, char *id)
 {
@@ -662,7 +704,7 @@ static int vcf_line(qld_vcf_reader *r, char *line, const unsigned char *mask, in
             if (size > ploidy)
                 return vfail(QLD_ERR_INVALID, "Detected data bigger than the ploidy of input "
                              "(position %d)", position);
-            int got = gt_data(r, word, sv, salt, part);
+            int got = gt_data(g, word, sv, salt, part);
             if (got < 0) return got;
             if (got != size)
                 return vfail(QLD_ERR_INVALID, "genotype %s in position %d is malformed", word,
@@ -684,8 +726,14 @@ static int vcf_line(qld_vcf_reader *r, char *line, const unsigned char *mask, in
     for (int i = 0; i < n; i++) st[map_char(snip[i])]++;
     int type = data_type(st);
     if (type == -1) return vfail(QLD_ERR_INVALID, "Empty alignment (no states found) in position %d", position);
+    /* With an N among three or more bases, whether removeNonPolymorphicSite
+     * keeps the site (and binaryDeduction then draws twice) depends on the
+     * shuffled order: such a line is not counted ahead. */
+    if (g->mode == VR_COUNT && type == 5 &&
+        (st[3] != 0) + (st[4] != 0) + (st[5] != 0) + (st[6] != 0) > 2)
+        return VDEFER;
     if (non_polymorphic(snip, n, type)) return 0;
-    int rc = binary_deduction(r, type, n, snip);
+    int rc = binary_deduction(g, type, n, snip);
     if (rc) return rc;
     *pos_out = position;
     return 1;
@@ -771,58 +819,137 @@ int qld_vcf_read_table(qld_vcf_reader *r, char *const *files, int nfiles, int po
                 } while (!eof && pos_wset && pos < pos_wmax);
             }
         }
-        /* Lines are parsed on every core without the rand() stream; a line
-         * that would draw from it (unphased genotypes, sites with more than
-         * two bases) or that fails is parsed again below, in file order, with
-         * the reader's stream, so the draws, the rows and the first error are
-         * those of a sequential read. */
+        /* Lines are parsed on every core.  A first pass counts the rand()
+         * draws of each line (its row is already exact when it draws none:
+         * haploid and phased genotypes at most two bases); the draws of the
+         * other lines are then taken from the reader's stream in line order
+         * and a second pass replays each line with its own values.  A line
+         * the count cannot settle (an N among three or more bases) or that
+         * fails stops the batch there and is parsed in order with the stream
+         * itself, so the draws, the rows and the first error are those of a
+         * sequential read. */
         signed char *st = NULL;
+        long *cnt = NULL, *off = NULL;
         uint64_t *rows = NULL;
         uint32_t *rbc = NULL, *rpos = NULL;
         char *rid = NULL;
         if (!rc && nsel) {
             st = (signed char *)malloc((size_t)nsel);
+            cnt = (long *)malloc((size_t)nsel * sizeof(long));
+            off = (long *)malloc((size_t)nsel * sizeof(long));
             rows = (uint64_t *)malloc((size_t)nsel * W * 8);
             rbc = (uint32_t *)malloc((size_t)nsel * 4);
             rpos = (uint32_t *)malloc((size_t)nsel * 4);
             rid = (char *)malloc((size_t)nsel * 256);
-            if (!st || !rows || !rbc || !rpos || !rid)
+            if (!st || !cnt || !off || !rows || !rbc || !rpos || !rid)
                 rc = vfail(QLD_ERR_NOMEM, "out of host memory");
         }
-        if (!rc && nsel) {
+        /* lines per count pass: at most QLD_VCF_AHEAD values drawn ahead */
+        const int BL = valid_count >= QLD_VCF_AHEAD / 16 ? 16 : QLD_VCF_AHEAD / valid_count;
+        int counted = 0;
+        for (int k0 = 0; !rc && k0 < nsel;) {
+            if (counted == k0) {
+                const int k_end = nsel - k0 < BL ? nsel : k0 + BL;
+                /* pass 1: status 0 skipped, 1 row, 2 replay (cnt draws), 3 in order */
 #pragma omp parallel
-            {
-                char *wb = (char *)malloc(maxlen + 1);
-                char *sb = (char *)malloc((size_t)valid_count + 8);
+                {
+                    char *wb = (char *)malloc(maxlen + 1);
+                    char *sb = (char *)malloc((size_t)valid_count + 8);
 #pragma omp for schedule(dynamic, 8)
-                for (int k = 0; k < nsel; k++) {
-                    int pk = 0, s = VDEFER;
-                    if (wb && sb)
-                        s = vcf_line(NULL, sel[k], mask, nsamples, valid_count, ploidy,
-                                     snip_size, wb, maxlen + 1, sb, &pk, rid + (size_t)k * 256);
-                    if (s == 1) {
-                        rbc[k] = pack_row(sb, valid_count, rows + (size_t)k * W, W);
-                        rpos[k] = (uint32_t)pk;
+                    for (int k = k0; k < k_end; k++) {
+                        vrng g = {VR_COUNT, NULL, NULL, 0, 0};
+                        int pk = 0, v = VDEFER;
+                        if (wb && sb)
+                            v = vcf_line(&g, sel[k], mask, nsamples, valid_count, ploidy,
+                                         snip_size, wb, maxlen + 1, sb, &pk,
+                                         rid + (size_t)k * 256);
+                        cnt[k] = g.used;
+                        if (v < 0) {
+                            st[k] = 3;
+                        } else if (g.used > 0) {
+                            st[k] = 2;
+                        } else {
+                            st[k] = (signed char)v;
+                            if (v == 1) {
+                                rbc[k] = pack_row(sb, valid_count, rows + (size_t)k * W, W);
+                                rpos[k] = (uint32_t)pk;
+                            }
+                        }
                     }
-                    st[k] = (signed char)(s == 1 ? 1 : s == 0 ? 0 : 2);
+                    free(wb);
+                    free(sb);
                 }
-                free(wb);
-                free(sb);
+                counted = k_end;
             }
-            for (int k = 0; k < nsel && !rc; k++) {
-                if (st[k] == 1) {
-                    rc = vtab_push(&t, (int)rpos[k], rid + (size_t)k * 256, rows + (size_t)k * W,
-                                   rbc[k]);
-                } else if (st[k] == 2) {
-                    int pk = 0;
-                    int s = vcf_line(r, sel[k], mask, nsamples, valid_count, ploidy, snip_size,
-                                     word, wcap, snip, &pk, id);
-                    if (s < 0) rc = s;
-                    else if (s == 1)
-                        rc = vtab_push(&t, pk, id, row, pack_row(snip, valid_count, row, W));
+            /* lines k0 .. k1-1 need no in-order parse */
+            int k1 = k0;
+            long total = 0;
+            while (k1 < counted && st[k1] != 3) {
+                off[k1] = total;
+                if (st[k1] == 2) total += cnt[k1];
+                k1++;
+            }
+            if (total > 0) {
+                int32_t *vals = (int32_t *)malloc((size_t)total * sizeof(int32_t));
+                if (!vals) { rc = vfail(QLD_ERR_NOMEM, "out of host memory"); break; }
+                for (long q = 0; q < total; q++) vals[q] = (int32_t)vrand(r);
+                int bad = 0;
+                /* pass 2: replay the lines that draw */
+#pragma omp parallel
+                {
+                    char *wb = (char *)malloc(maxlen + 1);
+                    char *sb = (char *)malloc((size_t)valid_count + 8);
+#pragma omp for schedule(dynamic, 8)
+                    for (int k = k0; k < k1; k++) {
+                        if (st[k] != 2) continue;
+                        vrng g = {VR_REPLAY, NULL, vals + off[k], cnt[k], 0};
+                        int pk = 0, v = VDEFER;
+                        if (wb && sb)
+                            v = vcf_line(&g, sel[k], mask, nsamples, valid_count, ploidy,
+                                         snip_size, wb, maxlen + 1, sb, &pk,
+                                         rid + (size_t)k * 256);
+                        if (v < 0 || g.used != g.n) {
+                            const int why = (wb && sb) ? 2 : 1;
+#pragma omp atomic write
+                            bad = why;
+                        } else {
+                            st[k] = (signed char)v;
+                            if (v == 1) {
+                                rbc[k] = pack_row(sb, valid_count, rows + (size_t)k * W, W);
+                                rpos[k] = (uint32_t)pk;
+                            }
+                        }
+                    }
+                    free(wb);
+                    free(sb);
+                }
+                free(vals);
+                if (bad) {
+                    rc = bad == 1 ? vfail(QLD_ERR_NOMEM, "out of host memory")
+                                  : vfail(QLD_ERR_INVALID, "%s: a line drew other rand() values "
+                                          "than counted", files[f]);
+                    break;
                 }
             }
+            for (int k = k0; k < k1 && !rc; k++)
+                if (st[k] == 1)
+                    rc = vtab_push(&t, (int)rpos[k], rid + (size_t)k * 256,
+                                   rows + (size_t)k * W, rbc[k]);
+            if (!rc && k1 < counted) {
+                /* the line the count could not settle, with the stream */
+                vrng g = {VR_STREAM, r, NULL, 0, 0};
+                int pk = 0;
+                int v = vcf_line(&g, sel[k1], mask, nsamples, valid_count, ploidy, snip_size,
+                                 word, wcap, snip, &pk, id);
+                if (v < 0) rc = v;
+                else if (v == 1)
+                    rc = vtab_push(&t, pk, id, row, pack_row(snip, valid_count, row, W));
+                k1++;
+            }
+            k0 = k1;
         }
+        free(cnt);
+        free(off);
         free(st);
         free(rows);
         free(rbc);
