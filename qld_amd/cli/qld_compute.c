@@ -38,6 +38,7 @@ typedef struct {
     int in2set, wset1, wset2, listset, outset, pathset;
     int wmin1, wmax1, wmin2, wmax2;
     int ploidy, threads, mdf, gpu, device, dump, plain_tasks;
+    int64_t batch_cells;      /* r^2 cells of one -inputList batch */
     float r2limit;
     int engine, r2mode;
 } options;
@@ -87,7 +88,9 @@ static void usage(void)
            "  -device D           GPU ordinal (default 0)\n"
            "  -dump-tables        write the tables each task would correlate to\n"
            "                      <output>.tables instead of computing (no GPU)\n"
-           "  -plain-tasks        run -inputList tasks one by one (no batching)\n");
+           "  -plain-tasks        run -inputList tasks one by one (no batching)\n"
+           "  -batch-cells C      r2 cells of one -inputList batch (default 2^29; a\n"
+           "                      task above C/4 runs alone through the streamed call)\n");
 }
 
 static int need_arg(int i, int argc) { return i != argc - 1; }
@@ -97,6 +100,7 @@ static void parse_args(int argc, char **argv, options *o)
     memset(o, 0, sizeof *o);
     o->r2limit = 0.2f;
     o->threads = 1;
+    o->batch_cells = (int64_t)1 << 29;
     o->engine = QLD_ENGINE_AUTO;
     o->r2mode = QLD_R2_REFERENCE;
     if (argc < 2) die("qLD-compute needs some arguments to run. Run \"qLD-compute -help\" for more info%s\n", "");
@@ -172,6 +176,14 @@ static void parse_args(int argc, char **argv, options *o)
         }
         if (!strcmp(a, "-device")) {
             if (need_arg(i, argc)) o->device = atoi(argv[++i]);
+            continue;
+        }
+        if (!strcmp(a, "-batch-cells")) {
+            if (!need_arg(i, argc)) die("\nERROR: -batch-cells needs a value%s\n\n", "");
+            long long v = atoll(argv[++i]);
+            if (v < 4 || v > ((long long)1 << 40))
+                die("\nERROR: -batch-cells %s is out of range (4 .. 2^40)\n\n", argv[i]);
+            o->batch_cells = (int64_t)v;
             continue;
         }
         if (!strcmp(a, "-seed")) {
@@ -483,26 +495,32 @@ static void gpu_check(qld_ctx *ctx, int rc, const char *what)
     }
 }
 
-/* One task through the drop-in correlate call (correlate_function.c:239-290). */
-static void run_task(qld_ctx *ctx, const options *o, task *t)
+/* One loaded task through the drop-in correlate call
+ * (correlate_function.c:239-290); frees its tables. */
+static void run_loaded(qld_ctx *ctx, const options *o, task *t, loaded *L)
 {
-    FILE *fp = o->dump ? NULL : fopen(t->output, "wb");
-    if (!o->dump && !fp) die("\n ERROR: cannot create output file %s\n\n", t->output);
-    loaded L;
-    load_task(o, t, &L);
-    const qld_snp_table *b = table_b(&L);
+    const qld_snp_table *b = table_b(L);
     if (o->dump) {
-        dump_tables(t, L.single, &L.A, b);
-        free_loaded(&L);
+        dump_tables(t, L->single, &L->A, b);
+        free_loaded(L);
         return;
     }
-    gpu_check(ctx, qld_correlate_report(ctx, fp, L.A.snp, L.A.pos, L.A.id, L.A.bc, L.A.rows,
-                                        b->snp, b->pos, b->id, b->bc, b->rows, L.A.words,
-                                        t->h1.snip_size, t->wset2, o->r2limit),
+    FILE *fp = fopen(t->output, "wb");
+    if (!fp) die("\n ERROR: cannot create output file %s\n\n", t->output);
+    gpu_check(ctx, qld_correlate_report(ctx, fp, L->A.snp, L->A.pos, L->A.id, L->A.bc,
+                                        L->A.rows, b->snp, b->pos, b->id, b->bc, b->rows,
+                                        L->A.words, t->h1.snip_size, t->wset2, o->r2limit),
               "GPU correlate");
     if (fclose(fp) != 0) die("\n ERROR: write error on %s\n\n", t->output);
-    printf("%s: %d x %d SNPs\n", t->output, L.A.rows, b->rows);
-    free_loaded(&L);
+    printf("%s: %d x %d SNPs\n", t->output, L->A.rows, b->rows);
+    free_loaded(L);
+}
+
+static void run_task(qld_ctx *ctx, const options *o, task *t)
+{
+    loaded L;
+    load_task(o, t, &L);
+    run_loaded(ctx, o, t, &L);
 }
 
 /* ---- -inputList tasks as one batched launch ------------------------------
@@ -513,7 +531,6 @@ static void run_task(qld_ctx *ctx, const options *o, task *t)
  * launch per tile shape for the whole batch), and each task's r^2 block is
  * compacted on the device before its report is written. */
 
-enum { BATCH_CELLS = 1 << 29 };     /* r^2 cells of one batch (2 GiB) */
 
 static void *xmalloc(size_t n)
 {
@@ -629,8 +646,8 @@ static void run_batch(qld_ctx *ctx, const options *o, task *const *bt, loaded *L
 }
 
 /* Tasks in list order; consecutive tasks with the same packing go into one
- * batch until BATCH_CELLS; a task above a quarter of that runs alone through
- * the streamed drop-in call. */
+ * batch until -batch-cells r^2 cells (2^29 = 2 GiB by default); a task above
+ * a quarter of that runs alone through the streamed drop-in call. */
 static void run_list(qld_ctx *ctx, const options *o, task *tasks, int ntask)
 {
     task **bt = (task **)xmalloc((size_t)ntask * sizeof(task *));
@@ -643,20 +660,22 @@ static void run_list(qld_ctx *ctx, const options *o, task *tasks, int ntask)
         int flush = i == ntask;
         if (!flush) {
             load_task(o, &tasks[i], &cur);
-            if (task_cells(&cur) > BATCH_CELLS / 4) {
-                free_loaded(&cur);
+            if (task_cells(&cur) > o->batch_cells / 4) {
+                /* every task's tables are read once and in list order: a
+                 * VCF read draws the unphased shuffles from the run's
+                 * rand() stream */
                 if (nb) {
                     run_batch(ctx, o, bt, L, nb);
                     for (int k = 0; k < nb; k++) free_loaded(&L[k]);
                     nb = 0;
                     cells = 0;
                 }
-                run_task(ctx, o, &tasks[i]);
+                run_loaded(ctx, o, &tasks[i], &cur);
                 continue;
             }
             flush = nb && (cur.A.words != L[0].A.words ||
                            tasks[i].h1.snip_size != bt[0]->h1.snip_size ||
-                           cells + task_cells(&cur) > BATCH_CELLS);
+                           cells + task_cells(&cur) > o->batch_cells);
         }
         if (flush && nb) {
             run_batch(ctx, o, bt, L, nb);

@@ -364,6 +364,28 @@ def test_cli_vcf_tables_reproduce_reference_report(name, tmp_path):
         assert text.encode() == exp[rep], (name, rep)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("cells", [4, 3000, 1 << 29])
+def test_cli_vcf_input_list_batch_limits(cells, tmp_path):
+    """-inputList over unphased VCF chunks with batch limits that send every
+    task alone, mix lone tasks and batches, or batch them all: each task's
+    tables are read once, in list order (the unphased shuffles draw from one
+    rand() stream), so the reports equal the reference's whatever the split."""
+    name = "unphased_list"
+    wd = _vcf_workdir(tmp_path)
+    print(f"[cli-vcf] {name} batch-cells={cells}", flush=True)
+    r = subprocess.run([CLI_BIN] + VCF_MANIFEST["cases"][name] +
+                       ["-output", "out/report.txt", "-batch-cells", str(cells)],
+                       cwd=wd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    exp = _vcf_expected(name)
+    got = {f: open(wd / "out" / f, "rb").read() for f in sorted(os.listdir(wd / "out"))}
+    got["input_report.txt"] = open(wd / "input_report.txt", "rb").read()
+    assert sorted(got) == sorted(exp)
+    for f in exp:
+        assert got[f] == exp[f], (f, cells)
+
+
 def test_vcf_reader_rand_stream_is_glibc_default():
     """The reader's private generator reproduces the C library's unseeded
     rand() sequence (which the reference's shuffles draw from)."""
