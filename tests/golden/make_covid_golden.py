@@ -57,9 +57,21 @@ CASES = {
 }
 
 
+# -inputList over the MDF directory: same-window triangles, a disjoint and an
+# overlapping rectangle, and a line the reference rejects (posWmin > posWmax)
+LIST_MDF = "\n".join([
+    "mdf/covid_19,1000,9000,mdf/covid_19,1000,9000",
+    "mdf/covid_19,2000,6000,mdf/covid_19,15000,22000",
+    "mdf/covid_19,9000,8000,mdf/covid_19,100,200",
+    "mdf/covid_19,20000,27000,mdf/covid_19,20000,27000",
+    "mdf/covid_19,500,3000,mdf/covid_19,2500,8000",
+]) + "\n"
+
 MDF_CASES = {
     # test_run's compute step on its own MDF directory
     "test_run_mdf": ["-input", "mdf/covid_19", "-r2limit", "0.6", "-ploidy", "haploid", "-mdf"],
+    "input_list_mdf": ["-inputList", "list_mdf.csv", "-ploidy", "haploid", "-mdf",
+                       "-r2limit", "0.2"],
     "two_windows_mdf": ["-input", "mdf/covid_19", "-ploidy", "haploid", "-mdf",
                         "-posWmin1", "3000", "-posWmax1", "12000",
                         "-posWmin2", "9000", "-posWmax2", "25000", "-r2limit", "0.05"],
@@ -108,6 +120,8 @@ def run_reference(name, args, mdf_scratch=None):
     os.symlink(os.path.join(OUT, "data"), os.path.join(work, "data"))
     if mdf_scratch:
         os.symlink(mdf_scratch, os.path.join(work, "mdf"))
+    if "-inputList" in args:
+        shutil.copy(os.path.join(OUT, "list_mdf.csv"), os.path.join(work, "list_mdf.csv"))
     cmd = [REF_CLI] + args + ["-output", "out/report.txt", "-threads", "1"]
     r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
@@ -129,6 +143,8 @@ def main():
         print(name, sorted(os.listdir(os.path.join(OUT, "expected", name))), flush=True)
     scratch = os.path.join(OUT, "_mdf")
     hashes = make_mdf(scratch)
+    with open(os.path.join(OUT, "list_mdf.csv"), "w") as f:
+        f.write(LIST_MDF)
     for name, args in MDF_CASES.items():
         run_reference(name, args, scratch)
         print(name, sorted(os.listdir(os.path.join(OUT, "expected", name))), flush=True)

@@ -513,14 +513,16 @@ def covid_mdf(tmp_path_factory):
 def _covid_mdf_workdir(tmp_path, covid_mdf):
     os.symlink(covid_mdf / "mdf", tmp_path / "mdf")
     os.makedirs(tmp_path / "out")
+    shutil.copy(os.path.join(COVID_DIR, "list_mdf.csv"), tmp_path / "list_mdf.csv")
     return tmp_path
 
 
 @pytest.mark.parametrize("name", COVID_MDF_CASES)
 def test_cli_covid_mdf_tables_reproduce_reference_report(name, covid_mdf, tmp_path):
     """The smoke script's compute step on its MDF directory (-mdf -ploidy
-    haploid over 705-word rows): the CLI's tables through the oracle give
-    the reference's report byte for byte."""
+    haploid over 705-word rows), two windows, and an -inputList of five
+    lines (triangles, rectangles, a rejected line): the CLI's tables through
+    the oracle give the reference's reports byte for byte."""
     args = COVID_MANIFEST["mdf_cases"][name]
     wd = _covid_mdf_workdir(tmp_path, covid_mdf)
     r = subprocess.run([CLI_BIN] + args + ["-output", "out/report.txt", "-dump-tables"],
@@ -528,14 +530,18 @@ def test_cli_covid_mdf_tables_reproduce_reference_report(name, covid_mdf, tmp_pa
     assert r.returncode == 0, r.stderr
     exp = _covid_expected(name)
     assert open(wd / "input_report.txt", "rb").read() == exp["input_report.txt"]
-    single, wset2, snp_size, A, B = _read_tables(wd / "out" / "report.txt.tables")
-    assert A["words"] == 353
-    counts = oracle.counts_blocked(A["snp"], B["snp"], tri=False)
-    r2 = oracle.r2_reference(A["bc"], B["bc"], counts, snp_size)
-    text = oracle.write_results_text(str(wd / "out" / "report.txt.oracle"), A["pos"], A["id"],
-                                     A["bc"], B["pos"], B["id"], B["bc"], snp_size, r2,
-                                     _r2limit(args), wset2)
-    assert text.encode() == exp["report.txt"], name
+    dumps = sorted(f for f in os.listdir(wd / "out") if f.endswith(".tables"))
+    reports = sorted(f for f in exp if f != "input_report.txt")
+    assert [d[:-len(".tables")] for d in dumps] == reports
+    for d, rep in zip(dumps, reports):
+        single, wset2, snp_size, A, B = _read_tables(wd / "out" / d)
+        assert A["words"] == 353
+        counts = oracle.counts_blocked(A["snp"], B["snp"], tri=False)
+        r2 = oracle.r2_reference(A["bc"], B["bc"], counts, snp_size)
+        text = oracle.write_results_text(str(wd / "out" / (rep + ".oracle")), A["pos"],
+                                         A["id"], A["bc"], B["pos"], B["id"], B["bc"],
+                                         snp_size, r2, _r2limit(args), wset2)
+        assert text.encode() == exp[rep], (name, rep)
 
 
 @pytest.mark.gpu
