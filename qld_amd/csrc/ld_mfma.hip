@@ -743,11 +743,14 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
     const int i0 = ti * TILE, j0 = tj * TILE;
     const int lane = threadIdx.x & 63;
     // Direct operands: a wave whose block of a triangle's diagonal tile lies
-    // wholly below the diagonal (every column < every row: no j > i) has
-    // nothing to compute or write; its waves do not share a k loop with
-    // the others, so it simply skips the tile.
+    // wholly below the diagonal (every column < every row: no j > i), or
+    // whose block has no row or no column inside the job (ragged edges of
+    // short windows), has nothing to compute or write; its waves do not
+    // share a k loop with the others, so it simply skips the tile.
     if constexpr (PACKED == 2)
-        if (d.tri && j0 + wn * WTN + WTN <= i0 + wm * WTM + 1) return;
+        if ((d.tri && j0 + wn * WTN + WTN <= i0 + wm * WTM + 1) || i0 + wm * WTM >= d.ni ||
+            j0 + wn * WTN >= d.nj)
+            return;
 
     // Bit counts of this lane's epilogue rows/columns, fetched now so their
     // latency hides under the k loop.
