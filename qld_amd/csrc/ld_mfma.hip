@@ -532,7 +532,10 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 // group's MFMAs run.  Rows past the table and words past the row end are
 // zero in the tiled copy; rows past the job but inside the table are read
 // and their products never stored.
-template <class G>
+// DIAG: the wave's block is square and centred on a triangle's diagonal, so
+// its 32 x 32 blocks left of the block diagonal (column block < row block)
+// hold no pair j > i and are not computed.
+template <class G, bool DIAG = false>
 __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, int wm, int wn,
                                            v16f (&acc)[G::NBM][G::NBN])
 {
@@ -592,7 +595,8 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
             const v4i av = fp4_plane<P>(xa[a]);
 #pragma unroll
             for (int b = 0; b < NBN; ++b)
-                acc[a][b] = FP4::mma_scaled<PlaneScale<P>::EA>(av, bv[b], acc[a][b]);
+                if (!DIAG || b >= a)
+                    acc[a][b] = FP4::mma_scaled<PlaneScale<P>::EA>(av, bv[b], acc[a][b]);
         }
     };
     auto compute = [&](const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
@@ -775,7 +779,13 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
 
     static_assert(G::SPLIT == 1 || PACKED == 2, "split tiles need direct operands");
     if constexpr (PACKED == 2) {
-        kloop_regs<G>(d, i0, j0, wm, wn, acc);   // no LDS ring: epilogue regions are per wave
+        // no LDS ring: epilogue regions are per wave
+        if constexpr (WTM == WTN) {
+            if (d.tri && i0 + wm * WTM == j0 + wn * WTN) kloop_regs<G, true>(d, i0, j0, wm, wn, acc);
+            else kloop_regs<G>(d, i0, j0, wm, wn, acc);
+        } else {
+            kloop_regs<G>(d, i0, j0, wm, wn, acc);
+        }
     } else {
         if constexpr (PACKED == 1) kloop_packed<G>(d, i0, j0, acc, smem);
         else kloop_dma<OP, G>(d, i0, j0, acc, smem);
