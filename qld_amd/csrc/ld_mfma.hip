@@ -535,7 +535,10 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 // DIAG: the wave's block is square and centred on a triangle's diagonal, so
 // its 32 x 32 blocks left of the block diagonal (column block < row block)
 // hold no pair j > i and are not computed.
-template <class G, bool DIAG = false>
+// MA, NA: only the first MA row and NA column blocks of the wave's block
+// hold rows / columns of the job (a ragged edge): the others are neither
+// loaded nor computed.
+template <class G, bool DIAG = false, int MA = G::NBM, int NA = G::NBN>
 __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, int wm, int wn,
                                            v16f (&acc)[G::NBM][G::NBN])
 {
@@ -579,22 +582,22 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     auto fetch = [&](v4i (&xa)[NBM], v4i (&xb)[NBN], int g) {
         const uint32_t k = (uint32_t)g * 1024u;
 #pragma unroll
-        for (int a = 0; a < NBM; ++a)
+        for (int a = 0; a < MA; ++a)
             xa[a] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsA, offA[a] + k, 0, 0);
 #pragma unroll
-        for (int b = 0; b < NBN; ++b)
+        for (int b = 0; b < NA; ++b)
             xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB[b] + k, 0, 0);
     };
     auto plane = [&](auto pc, const v4i (&xa)[NBM], const v4i (&xb)[NBN]) {
         constexpr int P = decltype(pc)::value;
         v4i bv[NBN];
 #pragma unroll
-        for (int b = 0; b < NBN; ++b) bv[b] = fp4_plane<P>(xb[b]);
+        for (int b = 0; b < NA; ++b) bv[b] = fp4_plane<P>(xb[b]);
 #pragma unroll
-        for (int a = 0; a < NBM; ++a) {
+        for (int a = 0; a < MA; ++a) {
             const v4i av = fp4_plane<P>(xa[a]);
 #pragma unroll
-            for (int b = 0; b < NBN; ++b)
+            for (int b = 0; b < NA; ++b)
                 if (!DIAG || b >= a)
                     acc[a][b] = FP4::mma_scaled<PlaneScale<P>::EA>(av, bv[b], acc[a][b]);
         }
@@ -780,9 +783,23 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
     static_assert(G::SPLIT == 1 || PACKED == 2, "split tiles need direct operands");
     if constexpr (PACKED == 2) {
         // no LDS ring: epilogue regions are per wave
-        if constexpr (WTM == WTN) {
-            if (d.tri && i0 + wm * WTM == j0 + wn * WTN) kloop_regs<G, true>(d, i0, j0, wm, wn, acc);
-            else kloop_regs<G>(d, i0, j0, wm, wn, acc);
+        if constexpr (WTM == WTN && NBM == 2 && NBN == 2) {
+            // 128-tiles (the short windows of plans): a block that reaches
+            // past the job's last row or column computes only its first
+            // 32-row / 32-column half there
+            const bool mfull = i0 + wm * WTM + 32 < d.ni, nfull = j0 + wn * WTN + 32 < d.nj;
+            if (mfull && nfull) {
+                if (d.tri && i0 + wm * WTM == j0 + wn * WTN)
+                    kloop_regs<G, true>(d, i0, j0, wm, wn, acc);
+                else
+                    kloop_regs<G>(d, i0, j0, wm, wn, acc);
+            } else if (mfull) {
+                kloop_regs<G, false, 2, 1>(d, i0, j0, wm, wn, acc);
+            } else if (nfull) {
+                kloop_regs<G, false, 1, 2>(d, i0, j0, wm, wn, acc);
+            } else {
+                kloop_regs<G, false, 1, 1>(d, i0, j0, wm, wn, acc);
+            }
         } else {
             kloop_regs<G>(d, i0, j0, wm, wn, acc);
         }
