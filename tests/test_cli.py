@@ -137,6 +137,18 @@ def test_cli_error_cases_exit_instead_of_hanging(name, tmp_path):
     assert "ERROR" in r.stderr or "ploidy" in r.stderr
 
 
+def test_cli_batch_cells_argument_is_checked(tmp_path):
+    """-batch-cells takes 4 .. 2^40 r^2 cells; anything else stops the
+    command before any input is read."""
+    wd = _workdir(tmp_path, "single_all")
+    for bad in (["-batch-cells", "2"], ["-batch-cells", str(1 << 41)], ["-batch-cells"]):
+        r = subprocess.run([CLI_BIN, "-input", "data/chrA", "-ploidy", "haploid", "-mdf",
+                            "-output", "out/report.txt"] + bad,
+                           cwd=wd, capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and "-batch-cells" in r.stderr, bad
+        assert os.listdir(wd / "out") == []
+
+
 def test_cli_vcf_mode_on_an_mdf_directory_and_unknown_flags(tmp_path):
     """Without -mdf the chunks are .vcf.gz files: an MDF directory has none,
     which is an error here (the reference rescans forever)."""
