@@ -183,6 +183,9 @@ void qld_mdf_reader_free(qld_mdf_reader *r);
 typedef struct qld_vcf_reader qld_vcf_reader;
 qld_vcf_reader *qld_vcf_reader_create(void);
 void qld_vcf_reader_free(qld_vcf_reader *r);
+/* Restart the reader's stream as srand(seed) restarts rand() (the
+ * reference's -seed, correlate_our_kernel.c:387-392). */
+void qld_vcf_reader_seed(qld_vcf_reader *r, unsigned int seed);
 const char *qld_vcf_last_error(void);
 
 /* First line and last #CHROM line of a header file (malloc'd); each chunk

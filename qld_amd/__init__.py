@@ -152,6 +152,7 @@ HOST_SYMBOLS = [
                                   C.c_char_p, _vp, _vp]),
     ("qld_vcf_reader_create", _vp, []),
     ("qld_vcf_reader_free", None, [_vp]),
+    ("qld_vcf_reader_seed", None, [_vp, C.c_uint]),
     ("qld_vcf_last_error", C.c_char_p, []),
     ("qld_vcf_header_lines", C.c_int, [C.c_char_p, _vp, _vp]),
     ("qld_vcf_sample_mask", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, _vp, _vp, _vp]),
@@ -326,7 +327,8 @@ class VcfReader:
     (None = every sample).  Reads draw unphased shuffles from the reader's own
     rand() stream, in call order."""
 
-    def __init__(self, header_path: str, ploidy: int, sample_list: str | None = None):
+    def __init__(self, header_path: str, ploidy: int, sample_list: str | None = None,
+                 seed: int | None = None):
         l1, ch = C.c_void_p(), C.c_void_p()
         rc = host_lib().qld_vcf_header_lines(header_path.encode(), C.byref(l1), C.byref(ch))
         if rc != QLD_OK:
@@ -349,6 +351,8 @@ class VcfReader:
         self._r = host_lib().qld_vcf_reader_create()
         if not self._r:
             raise QldError(QLD_ERR_NOMEM, "qld_vcf_reader_create")
+        if seed is not None:
+            host_lib().qld_vcf_reader_seed(self._r, int(seed))
 
     def table(self, files: list[str], pos_wmin: int, pos_wmax: int, snip_size: int,
               pos_wset: int = 1) -> dict:

@@ -39,6 +39,8 @@ typedef struct {
     int wmin1, wmax1, wmin2, wmax2;
     int ploidy, threads, mdf, gpu, device, dump, plain_tasks;
     int64_t batch_cells;      /* r^2 cells of one -inputList batch */
+    unsigned seed;            /* -seed: srand() seed of the unphased shuffles */
+    int seedset;
     float r2limit;
     int engine, r2mode;
 } options;
@@ -79,7 +81,8 @@ static void usage(void)
            "  -sampleList FILE    samples of the first input (VCF input)\n"
            "  -sampleList2 FILE   samples of the second input (default: -sampleList)\n"
            "  -gpu                accepted; the engine always runs on the GPU\n"
-           "  -blis, -sorted, -competing, -rTable, -threads T, -seed S\n"
+           "  -seed S             srand() seed of the unphased-genotype shuffles\n"
+           "  -blis, -sorted, -competing, -rTable, -threads T\n"
            "                      accepted, no effect (MDF files are read on all\n"
            "                      host cores, up to 32)\n"
            "  -engine E           auto | popcount | mfma_fp4 | mfma_i8 (default auto)\n"
@@ -187,7 +190,13 @@ static void parse_args(int argc, char **argv, options *o)
             continue;
         }
         if (!strcmp(a, "-seed")) {
-            if (need_arg(i, argc)) ++i;
+            /* the reference seeds rand() with it (and then stops at its own
+             * "not a valid command line parameter", correlate_our_kernel.c:
+             * 387-392,433-435); here the seed restarts the VCF reader's
+             * stream of unphased shuffles */
+            if (!need_arg(i, argc)) die("\nERROR: -seed needs a value%s\n\n", "");
+            o->seed = (unsigned)atoi(argv[++i]);
+            o->seedset = 1;
             continue;
         }
         if (!strcmp(a, "-engine") && need_arg(i, argc)) {
@@ -701,6 +710,7 @@ int main(int argc, char **argv)
     g_reader = qld_mdf_reader_create();
     g_vcf = qld_vcf_reader_create();
     if (!g_reader || !g_vcf) die("ERROR: out of host memory%s\n", "");
+    if (o.seedset) qld_vcf_reader_seed(g_vcf, o.seed);
     if (o.gpu) printf("GPU kernel active\n");
     if (o.mdf) printf("MDF input active\n");
 

@@ -56,6 +56,11 @@ qld_vcf_reader *qld_vcf_reader_create(void)
 
 void qld_vcf_reader_free(qld_vcf_reader *r) { free(r); }
 
+void qld_vcf_reader_seed(qld_vcf_reader *r, unsigned int seed)
+{
+    if (r) srandom_r(seed, &r->rd);
+}
+
 static int vrand(qld_vcf_reader *r)
 {
     int32_t v;

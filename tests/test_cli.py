@@ -398,6 +398,33 @@ def test_cli_vcf_input_list_batch_limits(cells, tmp_path):
         assert got[f] == exp[f], (f, cells)
 
 
+def test_cli_seed_restarts_the_unphased_shuffles(tmp_path):
+    """-seed S restarts the reader's rand() stream as srand(S) restarts the
+    reference's: S = 1 is glibc's default seed (the golden tables), another
+    seed shuffles the unphased genotypes differently, the same way on every
+    run.  (The reference's own parser seeds rand() and then rejects the
+    flag, correlate_our_kernel.c:387-392,433-435, so no reference report
+    pins a seeded run.)"""
+    name = "unphased"
+    args = VCF_MANIFEST["cases"][name]
+
+    def tables(seed, tag):
+        wd = tmp_path / tag
+        os.makedirs(wd)
+        _vcf_workdir(wd)
+        extra = [] if seed is None else ["-seed", str(seed)]
+        r = subprocess.run([CLI_BIN] + args + ["-output", "out/report.txt", "-dump-tables"] +
+                           extra, cwd=wd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        return open(wd / "out" / "report.txt.tables", "rb").read()
+
+    default = tables(None, "default")
+    assert tables(1, "seed1") == default
+    seven = tables(7, "seed7")
+    assert seven != default
+    assert tables(7, "seed7b") == seven
+
+
 def test_vcf_reader_rand_stream_is_glibc_default():
     """The reader's private generator reproduces the C library's unseeded
     rand() sequence (which the reference's shuffles draw from)."""
