@@ -41,6 +41,7 @@ typedef struct {
     int64_t batch_cells;      /* r^2 cells of one -inputList batch */
     unsigned seed;            /* -seed: srand() seed of the unphased shuffles */
     int seedset;
+    int sorted, competing;    /* -sorted -competing: -inputList tasks largest first */
     float r2limit;
     int engine, r2mode;
 } options;
@@ -82,7 +83,9 @@ static void usage(void)
            "  -sampleList2 FILE   samples of the second input (default: -sampleList)\n"
            "  -gpu                accepted; the engine always runs on the GPU\n"
            "  -seed S             srand() seed of the unphased-genotype shuffles\n"
-           "  -blis, -sorted, -competing, -rTable, -threads T\n"
+           "  -sorted -competing  run -inputList tasks largest first (their reports\n"
+           "                      keep their line numbers); either alone: no effect\n"
+           "  -blis, -rTable, -threads T\n"
            "                      accepted, no effect (MDF files are read on all\n"
            "                      host cores, up to 32)\n"
            "  -engine E           auto | popcount | mfma_fp4 | mfma_i8 (default auto)\n"
@@ -219,9 +222,9 @@ static void parse_args(int argc, char **argv, options *o)
         if (!strcmp(a, "-dump-tables")) { o->dump = 1; continue; }
         if (!strcmp(a, "-plain-tasks")) { o->plain_tasks = 1; continue; }
         if (!strcmp(a, "-gpu")) { o->gpu = 1; continue; }
-        if (!strcmp(a, "-blis") || !strcmp(a, "-sorted") || !strcmp(a, "-competing") ||
-            !strcmp(a, "-rTable"))
-            continue;
+        if (!strcmp(a, "-sorted")) { o->sorted = 1; continue; }
+        if (!strcmp(a, "-competing")) { o->competing = 1; continue; }
+        if (!strcmp(a, "-blis") || !strcmp(a, "-rTable")) continue;
         die("\nERROR: %s is not a valid command line parameter\n\n", a);
     }
     if (!o->ploidy) die("Please insert supported ploidy (see --help for more info)%s\n", "");
@@ -375,6 +378,38 @@ static task *list_tasks(options *o, int *ntask, FILE *rep)
     if (!n) die("ERROR: All files in the list are invalid!%s\n", "");
     *ntask = n;
     return ts;
+}
+
+/* -sorted with -inputList sorts the task queue (correlate_our_kernel.c:
+ * 589-590, MergeSort and SortedMerge, load_balancer.c:22-120): decreasing
+ * chunk-file count over both regions, then decreasing span of the first
+ * window, ties in list order -- a stable sort, which the reference's list
+ * merge sort is.  Its worker takes tasks from the queue head only with
+ * -competing; otherwise it fetches them by id, in list order
+ * (correlate_function.c:52-80).  The order matters beyond scheduling: VCF
+ * reads draw the unphased shuffles from one rand() stream, task by task. */
+static int task_before(const task *a, const task *b)
+{
+    const int fa = a->nfiles1 + a->nfiles2, fb = b->nfiles1 + b->nfiles2;
+    if (fa != fb) return fa > fb;
+    return (a->wmax1 - a->wmin1) > (b->wmax1 - b->wmin1);
+}
+
+static void sort_tasks(task *ts, int n)
+{
+    task *tmp = (task *)malloc((size_t)(n > 0 ? n : 1) * sizeof(task));
+    if (!tmp) die("ERROR: out of host memory%s\n", "");
+    for (int w = 1; w < n; w *= 2) {
+        for (int lo = 0; lo < n; lo += 2 * w) {
+            int mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) tmp[k++] = task_before(&ts[j], &ts[i]) ? ts[j++] : ts[i++];
+            while (i < mid) tmp[k++] = ts[i++];
+            while (j < hi) tmp[k++] = ts[j++];
+        }
+        memcpy(ts, tmp, (size_t)n * sizeof(task));
+    }
+    free(tmp);
 }
 
 static void dump_table(FILE *f, const qld_snp_table *t)
@@ -719,6 +754,7 @@ int main(int argc, char **argv)
     int ntask = 0;
     if (o.listset) {
         tasks = list_tasks(&o, &ntask, rep);
+        if (o.sorted && o.competing) sort_tasks(tasks, ntask);
     } else {
         tasks = (task *)calloc(1, sizeof(task));
         if (!tasks) die("ERROR: out of host memory%s\n", "");

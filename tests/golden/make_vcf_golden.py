@@ -163,6 +163,14 @@ def cases(P):
                                 "-posWmax2", str(P["vu2"][49]), "-r2limit", "0.03"],
         "unphased_list": ["-inputList", "list.csv", "-ploidy", "unphased_diploid",
                           "-r2limit", "0.05"],
+        # the same list run largest task first: the shuffles follow the order
+        # the list smallest task first: -sorted alone leaves the order (tasks
+        # are fetched by id), -sorted -competing runs the largest first
+        "unphased_list_sorted": ["-inputList", "list_sorted.csv", "-ploidy",
+                                 "unphased_diploid", "-r2limit", "0.05", "-sorted"],
+        "unphased_list_sorted_competing": ["-inputList", "list_sorted.csv", "-ploidy",
+                                           "unphased_diploid", "-r2limit", "0.05", "-sorted",
+                                           "-competing"],
     }
 
 
@@ -173,6 +181,12 @@ def list_text(P):
         f"data/vu,{u[10]},{u[70]},data/vu2,{u2[3]},{u2[40]}",
         f"data/vu,{u[0]},{u[30]},data/vu,{u[40]},{u[79]}",
     ]) + "\n"
+
+
+def sorted_list_text(lst):
+    """The -inputList lines in reverse: smallest task first, so that -sorted
+    reorders them."""
+    return "\n".join(reversed(lst.strip().split("\n"))) + "\n"
 
 
 def run_reference(name, args, lst):
@@ -186,6 +200,8 @@ def run_reference(name, args, lst):
     if "-inputList" in args:
         with open(os.path.join(work, "list.csv"), "w") as f:
             f.write(lst)
+        with open(os.path.join(work, "list_sorted.csv"), "w") as f:
+            f.write(sorted_list_text(lst))
     cmd = [REF_BIN] + args + ["-output", "out/report.txt", "-threads", "1"]
     r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=120)
     if r.returncode != 0:
@@ -203,6 +219,8 @@ def main():
     lst = list_text(P)
     with open(os.path.join(OUT, "list.csv"), "w") as f:
         f.write(lst)
+    with open(os.path.join(OUT, "list_sorted.csv"), "w") as f:
+        f.write(sorted_list_text(lst))
     cs = cases(P)
     for name, args in cs.items():
         run_reference(name, args, lst)

@@ -345,6 +345,7 @@ def _vcf_workdir(tmp_path):
     os.symlink(os.path.join(VCF_DIR, "data"), tmp_path / "data")
     os.makedirs(tmp_path / "out")
     shutil.copy(os.path.join(VCF_DIR, "list.csv"), tmp_path / "list.csv")
+    shutil.copy(os.path.join(VCF_DIR, "list_sorted.csv"), tmp_path / "list_sorted.csv")
     return tmp_path
 
 
