@@ -184,9 +184,17 @@ def list_text(P):
 
 
 def sorted_list_text(lst):
-    """The -inputList lines in reverse: smallest task first, so that -sorted
-    reorders them."""
-    return "\n".join(reversed(lst.strip().split("\n"))) + "\n"
+    """The -inputList lines in reverse (smallest task first, so that -sorted
+    reorders them), then a copy of one line (an exact tie: list order), a
+    short window over the same chunks as a longer one (a tie in chunk count:
+    the longer first) and a task on the second input alone."""
+    lines = list(reversed(lst.strip().split("\n")))
+    first = lines[0].split(",")
+    short = ",".join([first[0], first[1], str(int(first[1]) + 40), first[3], first[4],
+                      str(int(first[4]) + 40)])
+    vu2 = [l for l in lst.strip().split("\n") if "vu2" in l][0].split(",")
+    lines += [lines[-1], short, ",".join([vu2[3], vu2[4], vu2[5], vu2[3], vu2[4], vu2[5]])]
+    return "\n".join(lines) + "\n"
 
 
 def run_reference(name, args, lst):
