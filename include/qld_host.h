@@ -156,16 +156,23 @@ int qld_mdf_read_table(char *const *files, int nfiles, int pos_wmin, int pos_wma
                        int pos_wset, int words, int nthreads, qld_snp_table *t);
 void qld_snp_table_free(qld_snp_table *t);
 
-/* A reader that keeps every chunk file it has read: the text, the line
+/* A reader that caches the chunk files it has read: the text, the line
  * index (each data line's POS) and the rows parsed so far, so that many
  * windows over the same chunks (the reference's -inputList tasks) read and
  * parse each file and row once.  qld_mdf_reader_table has the semantics of
- * qld_mdf_read_table.  Not thread-safe: one caller at a time. */
+ * qld_mdf_read_table.  After each call the cache is trimmed to its byte
+ * limit (default 4 GiB) by unloading the least recently used files that the
+ * call did not read; an unloaded file is read again when needed.  Not
+ * thread-safe: one caller at a time. */
 typedef struct qld_mdf_reader qld_mdf_reader;
 qld_mdf_reader *qld_mdf_reader_create(void);
 int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int pos_wmin,
                          int pos_wmax, int pos_wset, int words, int nthreads,
                          qld_snp_table *t);
+/* Cache limit in bytes (0: keep only the files of the latest call). */
+void qld_mdf_reader_set_limit(qld_mdf_reader *r, size_t bytes);
+/* Bytes the cache holds now (file text, line index and parsed rows). */
+size_t qld_mdf_reader_bytes(const qld_mdf_reader *r);
 void qld_mdf_reader_free(qld_mdf_reader *r);
 
 /* ---- VCF chunk input ------------------------------------------------------ */

@@ -165,6 +165,8 @@ HOST_SYMBOLS = [
     ("qld_mdf_reader_table", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, _vp]),
     ("qld_mdf_reader_free", None, [_vp]),
+    ("qld_mdf_reader_set_limit", None, [_vp, C.c_size_t]),
+    ("qld_mdf_reader_bytes", C.c_size_t, [_vp]),
     ("qld_mdf_write", C.c_int, [C.c_char_p, C.c_char_p, _vp, _vp, _vp, _vp, C.c_int,
                                 C.c_int]),
 ]
@@ -292,11 +294,17 @@ def mdf_find_files(directory: str, hid: str, snips_per_file: int, total_snips: i
 class MdfReader:
     """qld_mdf_reader: chunk files read and rows parsed once across calls."""
 
-    def __init__(self):
+    def __init__(self, limit: int | None = None):
         self._r = host_lib().qld_mdf_reader_create()
+        if limit is not None:
+            host_lib().qld_mdf_reader_set_limit(self._r, limit)
 
     def table(self, files, pos_wmin, pos_wmax, words, pos_wset=1, nthreads=0) -> dict:
         return mdf_read_table(files, pos_wmin, pos_wmax, words, pos_wset, nthreads, reader=self)
+
+    @property
+    def cached_bytes(self) -> int:
+        return int(host_lib().qld_mdf_reader_bytes(self._r))
 
     def close(self):
         if self._r:

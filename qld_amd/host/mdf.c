@@ -342,16 +342,33 @@ typedef struct {
     char **id;
     unsigned short *idlen;
     unsigned char *parsed; /* 0 no, 1 ok, 2 too few fields, 3 wrong word count */
+    unsigned long long used;   /* reader call that last touched the file */
 } mdf_file;
 
+/* Files are allocated one by one and the registry holds pointers to them, so
+ * a file's address stays valid while later calls register more files.  The
+ * cache keeps at most `limit` bytes of file text, line index and parsed rows
+ * after each call: least recently used files go first, never one the call
+ * itself read (a single window larger than the limit is still served). */
 struct qld_mdf_reader {
-    mdf_file *f;
+    mdf_file **f;
     int n, cap;
+    unsigned long long calls;
+    size_t limit;
 };
+
+#define QLD_MDF_CACHE_DEFAULT ((size_t)4 << 30)
 
 qld_mdf_reader *qld_mdf_reader_create(void)
 {
-    return (qld_mdf_reader *)calloc(1, sizeof(qld_mdf_reader));
+    qld_mdf_reader *r = (qld_mdf_reader *)calloc(1, sizeof(qld_mdf_reader));
+    if (r) r->limit = QLD_MDF_CACHE_DEFAULT;
+    return r;
+}
+
+void qld_mdf_reader_set_limit(qld_mdf_reader *r, size_t bytes)
+{
+    if (r) r->limit = bytes;
 }
 
 static void file_free_rows(mdf_file *f)
@@ -370,17 +387,69 @@ static void file_free_rows(mdf_file *f)
     f->words = 0;
 }
 
+/* Drop a file's text, index and rows; it stays registered and is read again
+ * when a later call needs it. */
+static void file_unload(mdf_file *f)
+{
+    free(f->buf);
+    free(f->line);
+    free(f->pos);
+    free(f->kind);
+    file_free_rows(f);
+    f->buf = NULL;
+    f->len = 0;
+    f->line = NULL;
+    f->pos = NULL;
+    f->kind = NULL;
+    f->nlines = 0;
+    f->has_data = 0;
+    f->rc = -1;
+}
+
+static size_t file_bytes(const mdf_file *f)
+{
+    if (f->rc != 0) return 0;
+    size_t n = (size_t)f->nlines;
+    size_t b = f->len + 1 + n * (sizeof(char *) + sizeof(int) + 1);
+    if (f->words)
+        b += n * ((size_t)f->words * 8 + 8 + sizeof(char *) + sizeof(unsigned short) + 1);
+    return b;
+}
+
+size_t qld_mdf_reader_bytes(const qld_mdf_reader *r)
+{
+    size_t b = 0;
+    if (r)
+        for (int i = 0; i < r->n; i++) b += file_bytes(r->f[i]);
+    return b;
+}
+
+/* Unload least recently used files, none touched by call `now`, until the
+ * cache holds at most r->limit bytes. */
+static void reader_trim(qld_mdf_reader *r, unsigned long long now)
+{
+    size_t total = qld_mdf_reader_bytes(r);
+    while (total > r->limit) {
+        mdf_file *victim = NULL;
+        for (int i = 0; i < r->n; i++) {
+            mdf_file *f = r->f[i];
+            if (f->used == now || !file_bytes(f)) continue;
+            if (!victim || f->used < victim->used) victim = f;
+        }
+        if (!victim) break;
+        total -= file_bytes(victim);
+        file_unload(victim);
+    }
+}
+
 void qld_mdf_reader_free(qld_mdf_reader *r)
 {
     if (!r) return;
     for (int i = 0; i < r->n; i++) {
-        mdf_file *f = &r->f[i];
+        mdf_file *f = r->f[i];
+        file_unload(f);
         free(f->path);
-        free(f->buf);
-        free(f->line);
-        free(f->pos);
-        free(f->kind);
-        file_free_rows(f);
+        free(f);
     }
     free(r->f);
     free(r);
@@ -431,20 +500,23 @@ static int load_file(mdf_file *f)
 static mdf_file *reader_file(qld_mdf_reader *r, const char *path)
 {
     for (int i = 0; i < r->n; i++)
-        if (!strcmp(r->f[i].path, path)) return &r->f[i];
+        if (!strcmp(r->f[i]->path, path)) return r->f[i];
     if (r->n == r->cap) {
         int nc = r->cap ? r->cap * 2 : 64;
-        mdf_file *nf = (mdf_file *)realloc(r->f, (size_t)nc * sizeof(mdf_file));
+        mdf_file **nf = (mdf_file **)realloc(r->f, (size_t)nc * sizeof(mdf_file *));
         if (!nf) return NULL;
         r->f = nf;
         r->cap = nc;
     }
-    mdf_file *f = &r->f[r->n];
-    memset(f, 0, sizeof *f);
+    mdf_file *f = (mdf_file *)calloc(1, sizeof *f);
+    if (!f) return NULL;
     f->path = strdup(path);
-    if (!f->path) return NULL;
+    if (!f->path) {
+        free(f);
+        return NULL;
+    }
     f->rc = -1;             /* not loaded yet */
-    r->n++;
+    r->f[r->n++] = f;
     return f;
 }
 
@@ -521,8 +593,11 @@ int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int 
     mdf_file **fs = (mdf_file **)malloc((size_t)(nfiles ? nfiles : 1) * sizeof(mdf_file *));
     if (!fs) return fail(QLD_ERR_NOMEM, "out of host memory");
     int rc = QLD_OK;
-    for (int i = 0; i < nfiles && !rc; i++)
+    const unsigned long long now = ++r->calls;
+    for (int i = 0; i < nfiles && !rc; i++) {
         if (!(fs[i] = reader_file(r, files[i]))) rc = fail(QLD_ERR_NOMEM, "out of host memory");
+        else fs[i]->used = now;
+    }
     /* load the files this reader has not seen (each listed once) */
     char errs[1024] = "";
     if (!rc) {
@@ -533,6 +608,8 @@ int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int 
             for (int j = 0; j < i; j++)
                 if (fs[j] == f) first = 0;
             if (!first || f->rc != -1) continue;
+            /* g_err is thread-local: a failing load leaves its message in
+             * this worker's own copy, taken into errs under the lock */
             f->rc = load_file(f);
             if (f->rc) {
 #pragma omp critical
@@ -642,6 +719,7 @@ int qld_mdf_reader_table(qld_mdf_reader *r, char *const *files, int nfiles, int 
     free(sel);
     free(fs);
     if (rc) qld_snp_table_free(t);
+    reader_trim(r, now);
     return rc;
 }
 

@@ -618,3 +618,78 @@ def test_cli_covid_reports_equal_reference_on_gpu(name, engine, tmp_path):
     assert sorted(got) == sorted(exp), name
     for f in exp:
         assert got[f] == exp[f], (name, f, engine)
+
+
+def _write_many_mdf(tmp_path, nfiles, rows_per_file=4, words=3, seed=11):
+    """nfiles chunk files of consecutive positions; returns their paths."""
+    rng = np.random.default_rng(seed)
+    d = tmp_path / "many"
+    d.mkdir()
+    paths = []
+    for f in range(nfiles):
+        snp = rng.integers(0, 2**63, (rows_per_file, words), dtype=np.uint64)
+        pos = np.arange(rows_per_file, dtype=np.uint32) + 1000 + f * rows_per_file
+        bc = np.array([sum(bin(int(w)).count("1") for w in r) for r in snp], dtype=np.uint32)
+        p = str(d / f"c_{f * rows_per_file + 1}_{(f + 1) * rows_per_file}_{pos[0]}_{pos[-1]}.mdf")
+        qld_amd.mdf_write(p, "c", snp, pos, [f"r{f}_{k}" for k in range(rows_per_file)], bc)
+        paths.append(p)
+    return paths
+
+
+def test_mdf_reader_many_files_and_bounded_cache(tmp_path):
+    """More than 64 and more than 128 distinct files through one reader, in
+    one call and across calls (the registry grows while earlier files are in
+    use), equal one-shot reads; under a byte limit the cache stays bounded
+    by the limit plus the files of the latest call."""
+    paths = _write_many_mdf(tmp_path, 200)
+    one = qld_amd.mdf_read_table(paths, 0, 10**9, 3)
+    reader = qld_amd.MdfReader()
+    got = reader.table(paths, 0, 10**9, 3)
+    assert (got["snp"] == one["snp"]).all() and got["id"] == one["id"]
+    reader.close()
+    per_file = None
+    reader = qld_amd.MdfReader(limit=0)
+    for a in list(range(0, 200, 35)) + list(range(0, 200, 35)):
+        group = paths[a:a + 70]
+        t = reader.table(group, 0, 10**9, 3)
+        want = qld_amd.mdf_read_table(group, 0, 10**9, 3)
+        assert (t["snp"] == want["snp"]).all() and (t["pos"] == want["pos"]).all()
+        per_file = per_file or reader.cached_bytes // len(group)
+        # limit 0: only the latest call's files stay
+        assert reader.cached_bytes <= len(group) * per_file * 1.01
+    reader.close()
+    reader = qld_amd.MdfReader(limit=40 * per_file)
+    for a in range(0, 200, 10):
+        reader.table(paths[a:a + 10], 0, 10**9, 3)
+        assert reader.cached_bytes <= 41 * per_file
+    reader.close()
+
+
+def test_mdf_reader_many_files_asan(tmp_path):
+    """The same access pattern through the C API in a build of the MDF reader
+    with AddressSanitizer: no invalid access, and the tables match a build
+    without it."""
+    import shutil
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    paths = _write_many_mdf(tmp_path, 150)
+    src = [os.path.join(ROOT, "tests", "mdf_reader_many.c"),
+           os.path.join(ROOT, "qld_amd", "host", "mdf.c")]
+    outs = {}
+    for name, flags in (("plain", ["-O1"]), ("asan", ["-O1", "-g", "-fsanitize=address",
+                                                      "-fno-omit-frame-pointer"])):
+        exe = str(tmp_path / f"mdf_many_{name}")
+        r = subprocess.run([gcc, "-std=gnu11", "-fopenmp", *flags,
+                            f"-I{os.path.join(ROOT, 'include')}", "-o", exe, *src, "-lz"],
+                           capture_output=True, text=True)
+        if r.returncode and name == "asan":
+            pytest.skip("no AddressSanitizer runtime: " + r.stderr[-200:])
+        assert r.returncode == 0, r.stderr
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0")
+        r = subprocess.run([exe, "3", "20000", *paths], capture_output=True, text=True,
+                           env=env, timeout=120)
+        assert r.returncode == 0 and "ERROR: AddressSanitizer" not in r.stderr, r.stderr[-3000:]
+        outs[name] = r.stdout.splitlines()
+    assert outs["plain"] == outs["asan"]
+    assert outs["plain"][0].startswith("all 600 ")
