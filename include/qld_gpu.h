@@ -120,6 +120,27 @@ int qld_correlate_dev(qld_ctx *ctx,
                       float *dresults, int64_t ldr,
                       uint32_t *dcounts, int64_t ldc);
 
+/* Allocation-free variant of qld_correlate_dev: same arguments, same results
+ * bit for bit, and it allocates nothing (no hipMalloc/hipFree, no host
+ * allocation, no memset node).  No counterpart in the reference: it is for
+ * callers that cannot allocate -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation.  Its scratch (queue heads, tile
+ * list, row-block-tiled operand copies) is a static 64 MiB device array per
+ * device, held from library load to process end; shapes whose scratch would
+ * exceed it return QLD_ERR_BUDGET (at 16,622 samples: a triangle of up to
+ * about 32,000 rows, or about 32,000 rows of A and B together; the scratch
+ * holds 2,080 bytes per row there).  Runs the popcount
+ * engine, or the fp4 engine with direct operands (the defaults); other
+ * engine / operand settings return QLD_ERR_INVALID.  All calls on one device
+ * share its scratch: calls that overlap on a device (several contexts or
+ * threads) race and must be serialized by the caller. */
+int qld_correlate_dev_noalloc(qld_ctx *ctx,
+                              const uint64_t *dA, const uint32_t *dbcA, int m,
+                              const uint64_t *dB, const uint32_t *dbcB, int n,
+                              int words, int snp_size, int triangular,
+                              float *dresults, int64_t ldr,
+                              uint32_t *dcounts, int64_t ldc);
+
 /* Milliseconds of device time of the kernels queued by the last
  * qld_correlate_dev / qld_gpu_correlate call (needs cfg.timing); waits for
  * them.  Mirrors the reference's per-phase GEMM timer (gemm_gpu.c:823-829). */

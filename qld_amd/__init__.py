@@ -66,6 +66,9 @@ GPU_SYMBOLS = [
     ("qld_correlate_dev", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int,
                                     C.c_int, C.c_int, C.c_int, _vp, C.c_int64,
                                     _vp, C.c_int64]),
+    ("qld_correlate_dev_noalloc", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int,
+                                            C.c_int, C.c_int, C.c_int, _vp, C.c_int64,
+                                            _vp, C.c_int64]),
     ("qld_last_kernel_ms", C.c_int, [_vp, _f32p]),
     ("qld_last_contract_ms", C.c_int, [_vp, _f32p]),
     ("qld_timer_start", C.c_int, [_vp]),
@@ -605,6 +608,15 @@ class Engine:
         def p(x):
             return None if x is None else (x.ptr if isinstance(x, DevBuffer) else int(x))
         self._check(hip_lib().qld_correlate_dev(
+            self._ctx, p(dA), p(dbcA), m, p(dB), p(dbcB), n, words, snp_size,
+            1 if triangular else 0, p(dres), ldr or m, p(dcnt), ldc or m))
+
+    def correlate_dev_noalloc(self, dA, dbcA, m, dB, dbcB, n, words, snp_size,
+                              triangular, dres=None, ldr=0, dcnt=None, ldc=0):
+        """qld_correlate_dev_noalloc: the same call without any allocation."""
+        def p(x):
+            return None if x is None else (x.ptr if isinstance(x, DevBuffer) else int(x))
+        self._check(hip_lib().qld_correlate_dev_noalloc(
             self._ctx, p(dA), p(dbcA), m, p(dB), p(dbcB), n, words, snp_size,
             1 if triangular else 0, p(dres), ldr or m, p(dcnt), ldc or m))
 

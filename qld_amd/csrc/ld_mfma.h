@@ -8,6 +8,7 @@
 #pragma once
 
 #include "qld_internal.h"
+#include "tile_order.h"
 
 
 // One all-pairs job on expanded operands.  Row i of the job reads expanded
@@ -43,11 +44,6 @@ struct WorkDesc {
     int pad2_;
 };
 
-// One tile: rows [ti*T, ti*T + T) x columns [tj*T, tj*T + T) of job `desc`.
-struct WorkItem {
-    uint32_t desc;
-    uint16_t ti, tj;
-};
 
 void fill_work_desc(WorkDesc &d, const uint8_t *XI, const uint8_t *XJ,
                     const uint32_t *bcI, const uint32_t *bcJ, int ni, int nj,
@@ -82,11 +78,10 @@ int64_t mfma_row_bytes(int engine, int words);
 #define QLD_TILE_LARGE 256
 #define QLD_TILE_SMALL 128
 
-// Append the tiles of one job (tile edge T) to `out`, ordered in 8 x 8
-// super-tiles so that concurrently running tiles share operand panels.
-void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int nj,
-                      int tri, int T);
-size_t job_tile_count(int ni, int nj, int tri, int T);
+// Write the tile list of one job (tile edge T, desc 0) into device memory
+// `out` (job_tile_count entries) in append_job_tiles' order, one thread per
+// tile (job_tile_index), on stream s.
+hipError_t launch_job_tiles(WorkItem *out, int ni, int nj, int tri, int T, hipStream_t s);
 
 // Expand packed rows to fp4 nibbles / int8 bytes (rows x row_bytes).
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,

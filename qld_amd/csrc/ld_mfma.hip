@@ -993,27 +993,26 @@ int64_t tiled_bytes(int rows, int words)
     return (int64_t)((rows + 31) / 32) * ((words + 3) / 4) * 1024;
 }
 
-size_t job_tile_count(int ni, int nj, int tri, int T)
+// One thread per cell of the job's tile grid; a triangle's cells below the
+// diagonal write nothing.
+__global__ void job_tiles_kernel(WorkItem *out, int ti_count, int tj_count, int tri)
 {
-    size_t a = (size_t)((ni + T - 1) / T), b = (size_t)((nj + T - 1) / T);
-    return tri ? a * (a + 1) / 2 : a * b;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ti_count * tj_count) return;
+    const int ti = (int)(t % ti_count), tj = (int)(t / ti_count);
+    if (tri && ti > tj) return;
+    out[job_tile_index(ti, tj, ti_count, tj_count, tri)] =
+        WorkItem{0u, (uint16_t)ti, (uint16_t)tj};
 }
 
-void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int nj,
-                      int tri, int T)
+hipError_t launch_job_tiles(WorkItem *out, int ni, int nj, int tri, int T, hipStream_t s)
 {
-    const int S = 8;
     const int ti_count = (ni + T - 1) / T, tj_count = (nj + T - 1) / T;
-    const int si = (ti_count + S - 1) / S, sj = (tj_count + S - 1) / S;
-    size_t k = *count;
-    for (int J = 0; J < sj; ++J)
-        for (int I = 0; I < si; ++I) {
-            if (tri && I > J) continue;
-            for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj)
-                for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti)
-                    if (!tri || ti <= tj) out[k++] = WorkItem{desc, (uint16_t)ti, (uint16_t)tj};
-        }
-    *count = k;
+    const int64_t cells = (int64_t)ti_count * tj_count;
+    if (cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(job_tiles_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s,
+                       out, ti_count, tj_count, tri);
+    return hipGetLastError();
 }
 
 hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,

@@ -42,6 +42,7 @@ struct qld_ctx {
     int tile_cutoff = 1024;   // windows of at least this many SNPs use 256-tiles
     int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape
     int fp4_packed = QLD_FP4_DIRECT;   // fp4 operands: QLD_FP4_DIRECT, _PACKED or _EXPANDED
+    void *scratch = nullptr;  // device address of this device's static scratch array
     char err[512] = {0};
 };
 

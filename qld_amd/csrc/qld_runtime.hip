@@ -14,6 +14,36 @@
 
 static char g_init_error[256];
 
+// ---- scratch of the allocation-free entry point ----------------------------
+// qld_correlate_dev_noalloc takes its scratch (queue heads, tile list and the
+// row-block-tiled operand copies) from this array instead of allocating.  It
+// lives in the code object: each device the library is loaded on holds its
+// own copy, at its own address, until the process ends.  In host code the
+// array's name is not a device address; qld_gpu_init looks the address up
+// once per device (scratch_setup) and only that address is used.
+#define QLD_SCRATCH_BYTES ((size_t)64 << 20)
+#define QLD_SCRATCH_DEVICES 64
+static __device__ __align__(256) uint8_t scratch[QLD_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[QLD_SCRATCH_DEVICES];
+
+// With `device` current: the device address of its scratch array.
+static hipError_t scratch_setup(int device, void **ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= QLD_SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 void qld_set_error(qld_ctx *ctx, const char *fmt, ...)
 {
     char *dst = ctx ? ctx->err : g_init_error;
@@ -97,6 +127,12 @@ extern "C" int qld_gpu_init(qld_ctx **out, const qld_cfg *cfg)
     static hipError_t attr_status = mfma_init_attributes();
     if (attr_status != hipSuccess) {
         qld_set_error(nullptr, "kernel attributes: %s", hipGetErrorString(attr_status));
+        qld_gpu_release(ctx);
+        return QLD_ERR_DEVICE;
+    }
+    if ((e = scratch_setup(c.device, &ctx->scratch)) != hipSuccess) {
+        qld_set_error(nullptr, "scratch address of device %d: %s", c.device,
+                      hipGetErrorString(e));
         qld_gpu_release(ctx);
         return QLD_ERR_DEVICE;
     }
@@ -477,6 +513,97 @@ extern "C" int qld_correlate_dev(qld_ctx *ctx,
     LdJob j{dA, dbcA, m, dB, dbcB, n, words, snp_size, triangular ? 1 : 0,
             dresults, ldr, dcounts, ldc};
     return qld_run_job(ctx, j);
+}
+
+// ---- allocation-free variant of qld_correlate_dev -----------------------------
+//
+// The same computation with the same kernels, for callers that may not
+// allocate device memory (code under HIP graph capture, tight serving loops):
+// the popcount engine needs no scratch; the fp4 engine with direct operands
+// (the default) takes the queue heads, the tile list -- written on the device
+// in closed form instead of uploaded -- and the row-block-tiled operand
+// copies from the static scratch array.  No hipMalloc, hipFree, host
+// allocation or memset: the queue heads are cleared by the kernel launcher.
+// Every call on a device shares that device's scratch, so calls that overlap
+// on one device (two contexts, two threads) race.
+extern "C" int qld_correlate_dev_noalloc(qld_ctx *ctx,
+                                         const uint64_t *dA, const uint32_t *dbcA, int m,
+                                         const uint64_t *dB, const uint32_t *dbcB, int n,
+                                         int words, int snp_size, int triangular,
+                                         float *dresults, int64_t ldr,
+                                         uint32_t *dcounts, int64_t ldc)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    const LdJob job{dA, dbcA, m, dB, dbcB, n, words, snp_size, triangular ? 1 : 0,
+                    dresults, ldr, dcounts, ldc};
+    int rc = check_job(ctx, job);
+    if (rc) return rc;
+    if (job.m == 0 || job.n == 0 || (!job.res && !job.cnt)) return QLD_OK;
+    const int engine = qld_pick_engine(ctx, job);
+    hipError_t e = hipSuccess;
+    if (ctx->cfg.timing) hipEventRecord(ctx->ev0, ctx->stream);
+    if (engine == QLD_ENGINE_POPCOUNT) {
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
+        e = launch_popcount(job, ctx->cfg.r2_mode, ctx->stream);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "popcount kernel launch");
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
+    } else {
+        if (engine != QLD_ENGINE_MFMA_FP4 || ctx->fp4_packed != QLD_FP4_DIRECT) {
+            qld_set_error(ctx, "qld_correlate_dev_noalloc: engine %d with fp4 operand mode %d "
+                          "needs an allocated workspace (only popcount and fp4 direct "
+                          "operands run without one)", engine, ctx->fp4_packed);
+            return QLD_ERR_INVALID;
+        }
+        if (!ctx->scratch) {
+            qld_set_error(ctx, "qld_correlate_dev_noalloc: no scratch on device %d",
+                          ctx->device);
+            return QLD_ERR_DEVICE;
+        }
+        const int T = plain_tile_edge(ctx, job, QLD_FP4_DIRECT);
+        if ((job.n + T - 1) / T > 65535 || (job.m + T - 1) / T > 65535) {
+            qld_set_error(ctx, "too many rows for one launch (m=%d n=%d)", job.m, job.n);
+            return QLD_ERR_INVALID;
+        }
+        // scratch: queue heads | tile list | tiled A | tiled B (256-aligned)
+        const bool same = job.A == job.B && job.m == job.n;
+        const size_t ntiles = job_tile_count(job.n, job.m, job.tri, T);
+        const size_t off_items = 256;
+        const size_t off_a = off_items + align256(ntiles * sizeof(WorkItem));
+        const size_t off_b = off_a + align256((size_t)tiled_bytes(job.m, job.words));
+        const size_t need = off_b + (same ? 0 : align256((size_t)tiled_bytes(job.n, job.words)));
+        if (need > QLD_SCRATCH_BYTES) {
+            qld_set_error(ctx, "qld_correlate_dev_noalloc: %d x %d rows of %d words need %zu "
+                          "bytes of scratch, more than its %zu", job.n, job.m, job.words, need,
+                          (size_t)QLD_SCRATCH_BYTES);
+            return QLD_ERR_BUDGET;
+        }
+        uint8_t *base = (uint8_t *)ctx->scratch;
+        uint32_t *queue = (uint32_t *)base;
+        WorkItem *items = (WorkItem *)(base + off_items);
+        uint64_t *TA = (uint64_t *)(base + off_a);
+        uint64_t *TB = same ? TA : (uint64_t *)(base + off_b);
+        e = launch_job_tiles(items, job.n, job.m, job.tri, T, ctx->stream);
+        if (e == hipSuccess) e = launch_tile_rows(job.A, job.m, job.words, TA, ctx->stream);
+        if (e == hipSuccess && !same)
+            e = launch_tile_rows(job.B, job.n, job.words, TB, ctx->stream);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "tile list / row tiling kernel");
+        WorkDesc d;
+        fill_work_desc(d, nullptr, nullptr, job.bcB, job.bcA, job.n, job.m, 0, job.tri,
+                       job.snp_size, job.res, job.ldr, job.cnt, job.ldc);
+        set_tiled_operands(d, TB, 0, job.n, TA, 0, job.m, job.words);
+        int64_t grid = (int64_t)ctx->num_cu * (T == QLD_TILE_SMALL ? 2 : 1);
+        if (grid > (int64_t)ntiles) grid = (int64_t)ntiles;
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
+        e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, QLD_FP4_DIRECT, nullptr, d, items,
+                              (int64_t)ntiles, (int)grid, queue, nullptr, ctx->stream);
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
+        if (e != hipSuccess) return qld_hip_fail(ctx, e, "MFMA kernel launch");
+    }
+    if (ctx->cfg.timing) hipEventRecord(ctx->ev1, ctx->stream);
+    ctx->last_engine = engine;
+    ctx->timed = ctx->cfg.timing != 0;
+    return QLD_OK;
 }
 
 extern "C" int qld_gpu_correlate(qld_ctx *ctx,

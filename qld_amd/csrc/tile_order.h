@@ -1,0 +1,71 @@
+// Order of a job's tiles in a work list.
+//
+// A job's output tiles are listed in 8 x 8 super-tiles, column of
+// super-tiles by column (super-tile (I, J) covers row tiles I*8 .. I*8+7 and
+// column tiles J*8 .. J*8+7), each super-tile column by column, so that
+// tiles that run at the same time share operand panels.  A single region
+// (tri) lists only the tiles with ti <= tj: every cell j > i that the
+// reference prints (correlate_IO.c:403-406) lies in one of them.
+//
+// append_job_tiles() writes that list on the host; job_tile_index() gives the
+// position of one tile in it in closed form, so the list can also be written
+// on the device, one thread per tile, with no host buffer and no copy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+// One tile: rows [ti*T, ti*T + T) x columns [tj*T, tj*T + T) of job `desc`.
+struct WorkItem {
+    uint32_t desc;
+    uint16_t ti, tj;
+};
+
+#define QLD_SUPER_TILE 8
+
+inline size_t job_tile_count(int ni, int nj, int tri, int T)
+{
+    size_t a = (size_t)((ni + T - 1) / T), b = (size_t)((nj + T - 1) / T);
+    return tri ? a * (a + 1) / 2 : a * b;
+}
+
+inline void append_job_tiles(WorkItem *out, size_t *count, uint32_t desc, int ni, int nj,
+                             int tri, int T)
+{
+    const int S = QLD_SUPER_TILE;
+    const int ti_count = (ni + T - 1) / T, tj_count = (nj + T - 1) / T;
+    const int si = (ti_count + S - 1) / S, sj = (tj_count + S - 1) / S;
+    size_t k = *count;
+    for (int J = 0; J < sj; ++J)
+        for (int I = 0; I < si; ++I) {
+            if (tri && I > J) continue;
+            for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj)
+                for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti)
+                    if (!tri || ti <= tj) out[k++] = WorkItem{desc, (uint16_t)ti, (uint16_t)tj};
+        }
+    *count = k;
+}
+
+// Position of tile (ti, tj) in append_job_tiles' list (ti <= tj when tri;
+// for tri, ti_count == tj_count).  With S the super-tile edge,
+// (I, J) = (ti / S, tj / S), (a, b) the offsets inside it and rows(I) /
+// cols(J) the tile counts of super-tile row I / column J (S except at the
+// last one):
+//   rectangle: every super-tile column before J is full,
+//     J*S*ti_count + I*S*cols(J) + b*rows(I) + a;
+//   triangle: super-tile column J' < J holds J' full blocks and a diagonal
+//     block of S(S+1)/2 tiles, so
+//     S*S*J(J-1)/2 + J*S(S+1)/2 + I*S*cols(J) + (I < J ? b*S + a
+//                                                      : b(b+1)/2 + a).
+__host__ __device__ inline int64_t job_tile_index(int ti, int tj, int ti_count, int tj_count,
+                                                  int tri)
+{
+    const int64_t S = QLD_SUPER_TILE;
+    const int64_t I = ti / S, J = tj / S, a = ti - I * S, b = tj - J * S;
+    const int64_t rows = ti_count - I * S < S ? ti_count - I * S : S;
+    const int64_t cols = tj_count - J * S < S ? tj_count - J * S : S;
+    if (!tri) return J * S * ti_count + I * S * cols + b * rows + a;
+    const int64_t before = S * S * (J * (J - 1) / 2) + J * (S * (S + 1) / 2) + I * S * cols;
+    return before + (I < J ? b * S + a : b * (b + 1) / 2 + a);
+}

@@ -121,3 +121,22 @@ def test_engine_init_rejects_bad_config():
     with pytest.raises(qld_amd.QldError) as e:
         qld_amd.Engine(engine=9)
     assert e.value.code == qld_amd.QLD_ERR_INVALID
+
+
+def test_tile_index_matches_tile_list(tmp_path):
+    """The closed-form position of a tile in a job's work list
+    (job_tile_index, which the device tile-list kernel of the allocation-free
+    entry point uses) equals its position in append_job_tiles' list for every
+    tile grid up to 70 x 70, rectangles and triangles (a host program built
+    from the same header)."""
+    import subprocess
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    exe = str(tmp_path / "tile_order_check")
+    r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O1", "-o", exe,
+                        os.path.join(ROOT, "tests", "tile_order_check.cpp")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.startswith("ok 4970"), r.stdout + r.stderr
