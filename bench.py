@@ -133,11 +133,19 @@ def windows_bench(args, steps):
     t = eng.table(A, bc, N)
     out = eng.alloc(total * 4)
     del A
-    log("[bench] windows: autotuning tile shapes")
-    cutoff, _ = eng.autotune_windows(t, wins, out, reps=2)
+    if args.win_cutoff is None:
+        log("[bench] windows: autotuning tile shapes")
+        cutoff, _ = eng.autotune_windows(t, wins, out, reps=3)
+    else:
+        cutoff = args.win_cutoff
+        eng.set_tile_cutoff(cutoff)
+        eng.set_plan_schedule(args.win_sched)
+    if args.no_graph:
+        eng.set_plan_graphs(False)
+    sched = eng.plan_schedule
     res = {"workload": f"{nwin} windows, sizes LogUniform(50, 2000), over {rows} SNPs x "
                        f"{N} samples, per-window lower-triangle r2",
-           "windows": nwin, "pairs": pairs, "tile_cutoff": cutoff}
+           "windows": nwin, "pairs": pairs, "tile_cutoff": cutoff, "schedule": sched}
 
     def timed(plan, use_graph, k):
         plan.use_graph(use_graph)
@@ -148,12 +156,16 @@ def windows_bench(args, steps):
             plan.run()
         return eng.timer_stop() / k
 
+    graphs = not args.no_graph
     batched = eng.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
-    ms_b = timed(batched, True, steps)
+    ms_b = timed(batched, graphs, steps)
     st = batched.stats()
     batched.free()
+    # The baseline, one launch per window, on 128-tiles: a single window's
+    # launch needs the finer tiles to fill the GPU (its fastest setting).
+    eng.set_tile_cutoff(1 << 30)
     plain = eng.plan_windows(t, wins, out, qld_amd.BATCH_PLAIN)
-    ms_pg = timed(plain, True, max(2, steps // 3))
+    ms_pg = timed(plain, True, max(2, steps // 3)) if graphs else None
     ms_pd = timed(plain, False, max(2, steps // 3))
     plain.free()
     t.free()
@@ -161,14 +173,16 @@ def windows_bench(args, steps):
     eng.close()
     res.update({"batched_ms": round(ms_b, 3), "batched_gpairs_s": round(pairs / ms_b / 1e6, 2),
                 "batched_launches": st["launches"], "tiles": st["tiles"],
-                "plain_graph_ms": round(ms_pg, 3),
-                "plain_graph_gpairs_s": round(pairs / ms_pg / 1e6, 2),
+                "graphs": graphs,
                 "plain_direct_ms": round(ms_pd, 3),
                 "plain_direct_gpairs_s": round(pairs / ms_pd / 1e6, 2),
-                "speedup_vs_plain": round(ms_pd / ms_b, 2),
-                "speedup_vs_plain_graph": round(ms_pg / ms_b, 2)})
-    log(f"[bench] windows: batched {ms_b:.3f} ms, plain graph {ms_pg:.3f} ms, "
-        f"plain direct {ms_pd:.3f} ms (cutoff {cutoff})")
+                "speedup_vs_plain": round(ms_pd / ms_b, 2)})
+    if graphs:
+        res.update({"plain_graph_ms": round(ms_pg, 3),
+                    "plain_graph_gpairs_s": round(pairs / ms_pg / 1e6, 2),
+                    "speedup_vs_plain_graph": round(ms_pg / ms_b, 2)})
+    log(f"[bench] windows: batched {ms_b:.3f} ms, plain graph {ms_pg} ms, "
+        f"plain direct {ms_pd:.3f} ms (cutoff {cutoff}, schedule {sched}, graphs {graphs})")
     return res
 
 
@@ -491,6 +505,12 @@ def main():
     ap.add_argument("--serving-rows200", type=int, default=20000)
     ap.add_argument("--windows", type=int, default=4096)
     ap.add_argument("--win-rows", type=int, default=500000)
+    ap.add_argument("--win-cutoff", type=int, default=None,
+                    help="window plans: tile cutoff to use instead of autotuning")
+    ap.add_argument("--win-sched", default="waves", choices=["waves", "tiles"],
+                    help="window plans: work schedule with --win-cutoff")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="window plans: direct launches, no HIP graph capture")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -247,6 +247,9 @@ int qld_plan_pairs(qld_ctx *ctx, const qld_table *t, const qld_pair_window *wins
 int qld_plan_run(qld_ctx *ctx, qld_plan *plan);
 /* Replay through the captured HIP graph (nonzero) or by direct launches. */
 int qld_plan_use_graph(qld_plan *plan, int use_graph);
+/* Whether plans built afterwards capture their launches in a HIP graph
+ * (default 1); without one, qld_plan_run launches the kernels directly. */
+int qld_set_plan_graphs(qld_ctx *ctx, int capture);
 /* Tiles and pairs the plan computes; kernel launches per run. */
 int qld_plan_stats(const qld_plan *plan, int64_t *tiles, int64_t *pairs,
                    int *launches);
@@ -301,13 +304,27 @@ int qld_server_stats(const qld_server *s, int64_t *completed, int64_t *batches,
 int qld_host_alloc(qld_ctx *ctx, size_t bytes, void **ptr);
 int qld_host_free(qld_ctx *ctx, void *ptr);
 
+/* Work scheduling of persistent plans with direct fp4 operands.  WAVES
+ * (default): the work list holds wave blocks (64 x 64 in 128-tiles,
+ * 128 x 64 in 256-tiles) with something to compute, and every wave claims
+ * its own, so no wave waits for the others of its workgroup on a diagonal
+ * or ragged-edge tile.  TILES: a workgroup claims a whole tile.  Plans built
+ * afterwards use the setting. */
+#define QLD_SCHED_TILES 0
+#define QLD_SCHED_WAVES 1
+int qld_set_plan_schedule(qld_ctx *ctx, int sched);
+int qld_get_plan_schedule(const qld_ctx *ctx);
+
 /* Tile-shape policy of window plans: windows of at least `cutoff` SNPs use
  * 256 x 256 tiles, smaller ones 128 x 128 (0 = all large). */
 int qld_set_tile_cutoff(qld_ctx *ctx, int cutoff);
 int qld_get_tile_cutoff(const qld_ctx *ctx);
 /* Autotuner: time the given window set as persistent plans under each
- * candidate cutoff (`reps` runs each after a warm-up), keep the fastest in
- * the context and report it.  Writes r^2 into `out` as a plan run does. */
+ * candidate cutoff -- and, for tables with direct fp4 operands, under both
+ * work schedules -- (`reps` runs each after a warm-up), keep the fastest
+ * cutoff and schedule in the context and report the cutoff
+ * (qld_get_plan_schedule gives the schedule).  Writes r^2 into `out` as a
+ * plan run does. */
 int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
                          int nwin, float *out, int reps, int *best_cutoff,
                          float *best_ms);

@@ -70,6 +70,9 @@ GPU_SYMBOLS = [
                                             C.c_int, C.c_int, C.c_int, _vp, C.c_int64,
                                             _vp, C.c_int64]),
     ("qld_last_kernel_ms", C.c_int, [_vp, _f32p]),
+    ("qld_set_plan_schedule", C.c_int, [_vp, C.c_int]),
+    ("qld_get_plan_schedule", C.c_int, [_vp]),
+    ("qld_set_plan_graphs", C.c_int, [_vp, C.c_int]),
     ("qld_last_contract_ms", C.c_int, [_vp, _f32p]),
     ("qld_timer_start", C.c_int, [_vp]),
     ("qld_timer_stop", C.c_int, [_vp, _f32p]),
@@ -466,6 +469,14 @@ class DevBuffer:
         self.eng._check(hip_lib().qld_d2h(self.eng._ctx, _ptr(out), self.ptr, out.nbytes))
         return out
 
+    def download_at(self, offset: int, shape, dtype) -> np.ndarray:
+        """Copy out `shape` elements starting `offset` bytes into the buffer."""
+        out = np.empty(shape, dtype=dtype)
+        assert offset >= 0 and offset + out.nbytes <= self.nbytes
+        self.eng._check(hip_lib().qld_d2h(self.eng._ctx, _ptr(out), self.ptr + int(offset),
+                                          out.nbytes))
+        return out
+
     def fill(self, value_u32: int = 0):
         self.eng._check(hip_lib().qld_dev_fill(self.eng._ctx, self.ptr, value_u32,
                                                self.nbytes // 4))
@@ -657,6 +668,19 @@ class Engine:
 
     def set_tile_cutoff(self, cutoff: int):
         self._check(hip_lib().qld_set_tile_cutoff(self._ctx, int(cutoff)))
+
+    def set_plan_schedule(self, sched: str):
+        """'waves': every wave claims wave blocks of its own (default);
+        'tiles': a workgroup claims a whole tile."""
+        self._check(hip_lib().qld_set_plan_schedule(self._ctx, {"tiles": 0, "waves": 1}[sched]))
+
+    def set_plan_graphs(self, capture: bool):
+        """Whether plans built afterwards are captured in a HIP graph."""
+        self._check(hip_lib().qld_set_plan_graphs(self._ctx, 1 if capture else 0))
+
+    @property
+    def plan_schedule(self) -> str:
+        return {0: "tiles", 1: "waves"}[hip_lib().qld_get_plan_schedule(self._ctx)]
 
     @property
     def tile_cutoff(self) -> int:

@@ -90,6 +90,18 @@ hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
 // Allow every MFMA kernel variant its dynamic LDS (call once before use).
 hipError_t mfma_init_attributes();
 
+// Per-wave units of the direct fp4 kernel: `units` lists wave blocks
+// (desc, bi, bj) -- rows [bi*WTM, +WTM) x columns [bj*WTN, +WTN) of the job,
+// WTM x WTN = 64 x 64 for tile 128, 128 x 64 for tile 256 (wave_block_shape)
+// -- and `grid` workgroups of 4 waves drain them from eight per-XCD queues
+// (queue = 16 device uint32, zeroed on the stream first), each wave claiming
+// its own units.  A list built by append_job_wave_units leaves out blocks
+// with nothing to compute.
+hipError_t launch_mfma_wave_units(int r2_mode, int tile, const WorkDesc *descs,
+                                  const WorkDesc &inline_desc, const WorkItem *units,
+                                  int64_t nunits, int grid, uint32_t *queue, hipStream_t s);
+void wave_block_shape(int tile, int *wtm, int *wtn);
+
 // Launch the MFMA kernel over `nitems` items.
 //   descs       device array of job descriptors, or NULL to use `inline_desc`
 //               for every item (single-job launches need no upload)

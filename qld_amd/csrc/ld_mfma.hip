@@ -286,10 +286,12 @@ struct Epi<QLD_R2_REFERENCE> {
         t.omp = b.s.d;
         return t;
     }
-    static __device__ __forceinline__ float r2(float c, const term &col, const term &row,
-                                               const WorkDesc &d)
+    static __device__ __forceinline__ void r2x4(const float (&c)[4], const term (&col)[4],
+                                                const term &row, const WorkDesc &d,
+                                                float (&out)[4])
     {
-        return r2_reference(c, col, row, d.invN);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = r2_reference(c[q], col[q], row, d.invN);
     }
 };
 
@@ -319,10 +321,12 @@ struct Epi<QLD_R2_EXACT> {
         t.rv = b.s.d;
         return t;
     }
-    static __device__ __forceinline__ float r2(float c, const term &col, const term &row,
-                                               const WorkDesc &d)
+    static __device__ __forceinline__ void r2x4(const float (&c)[4], const term (&col)[4],
+                                                const term &row, const WorkDesc &d,
+                                                float (&out)[4])
     {
-        return r2_exact(c, col, row, d.dN);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = r2_exact(c[q], col[q], row, d.dN);
     }
 };
 
@@ -532,13 +536,15 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 // group's MFMAs run.  Rows past the table and words past the row end are
 // zero in the tiled copy; rows past the job but inside the table are read
 // and their products never stored.
-// DIAG: the wave's block is square and centred on a triangle's diagonal, so
-// its 32 x 32 blocks left of the block diagonal (column block < row block)
-// hold no pair j > i and are not computed.
+// DOFF: the wave's block straddles a triangle's diagonal, its first column
+// DOFF 32-blocks right of its first row: a 32 x 32 block (a, b) with
+// a - b > DOFF lies wholly below the diagonal (no pair j > i) and is not
+// computed (DOFF = QLD_NO_DIAG: every block is).
 // MA, NA: only the first MA row and NA column blocks of the wave's block
 // hold rows / columns of the job (a ragged edge): the others are neither
 // loaded nor computed.
-template <class G, bool DIAG = false, int MA = G::NBM, int NA = G::NBN>
+#define QLD_NO_DIAG 99
+template <class G, int DOFF = QLD_NO_DIAG, int MA = G::NBM, int NA = G::NBN>
 __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, int wm, int wn,
                                            v16f (&acc)[G::NBM][G::NBN])
 {
@@ -583,7 +589,8 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
         const uint32_t k = (uint32_t)g * 1024u;
 #pragma unroll
         for (int a = 0; a < MA; ++a)
-            xa[a] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsA, offA[a] + k, 0, 0);
+            if (a - DOFF <= NA - 1)     // row block a meets some computed column block
+                xa[a] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsA, offA[a] + k, 0, 0);
 #pragma unroll
         for (int b = 0; b < NA; ++b)
             xb[b] = (v4i)__builtin_amdgcn_raw_buffer_load_b128(rsB, offB[b] + k, 0, 0);
@@ -595,10 +602,11 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
         for (int b = 0; b < NA; ++b) bv[b] = fp4_plane<P>(xb[b]);
 #pragma unroll
         for (int a = 0; a < MA; ++a) {
+            if (a - DOFF > NA - 1) continue;
             const v4i av = fp4_plane<P>(xa[a]);
 #pragma unroll
             for (int b = 0; b < NA; ++b)
-                if (!DIAG || b >= a)
+                if (a - b <= DOFF)
                     acc[a][b] = FP4::mma_scaled<PlaneScale<P>::EA>(av, bv[b], acc[a][b]);
         }
     };
@@ -680,8 +688,7 @@ __device__ __forceinline__ void tile_epilogue_impl(const WorkDesc &d, int i0, in
             const float cv[4] = {OP::count(c4.x), OP::count(c4.y), OP::count(c4.z),
                                  OP::count(c4.w)};
             float rv[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) rv[q] = E::r2(cv[q], col[q], rowt, d);
+            E::r2x4(cv, col, rowt, d, rv);
             if constexpr (INTERIOR) {
                 // Every column of the tile is inside the job (and above the
                 // diagonal for a triangle) and rows are 16-byte aligned.
@@ -726,8 +733,10 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
                                               const uint32_t *bc_row, const uint32_t *bc_col,
                                               uint8_t *smem)
 {
-    constexpr int TILE = G::EDGE;
-    const bool inside = i0 + TILE <= d.ni && j0 + TILE <= d.nj && (!d.tri || j0 >= i0 + TILE);
+    // the wave's own block decides: the direct kernels' waves work alone
+    const int a0 = i0 + wm * G::WTM, b0 = j0 + wn * G::WTN;
+    const bool inside = a0 + G::WTM <= d.ni && b0 + G::WTN <= d.nj &&
+                        (!d.tri || b0 >= a0 + G::WTM);
     const bool aligned =
         (!d.res || ((d.ldr & 3) == 0 && (((uintptr_t)d.res) & 15) == 0)) &&
         (!d.cnt || ((d.ldc & 3) == 0 && (((uintptr_t)d.cnt) & 15) == 0));
@@ -741,7 +750,7 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
 // expanded in the kernel), then the fused epilogue.
 // wm, wn: the wave's block of the tile; wave: the wave's LDS epilogue
 // region.  Cooperative k loops (LDS ring) need wm, wn = this wave's own.
-template <class OP, int MODE, class G, int PACKED>
+template <class OP, int MODE, class G, int PACKED, bool WAVE_UNITS = false>
 __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int wm, int wn,
                                           int wave, uint8_t *smem)
 {
@@ -783,23 +792,38 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
     static_assert(G::SPLIT == 1 || PACKED == 2, "split tiles need direct operands");
     if constexpr (PACKED == 2) {
         // no LDS ring: epilogue regions are per wave
+        const int a0 = i0 + wm * WTM, b0 = j0 + wn * WTN;
         if constexpr (WTM == WTN && NBM == 2 && NBN == 2) {
             // 128-tiles (the short windows of plans): a block that reaches
             // past the job's last row or column computes only its first
             // 32-row / 32-column half there
-            const bool mfull = i0 + wm * WTM + 32 < d.ni, nfull = j0 + wn * WTN + 32 < d.nj;
+            const bool mfull = a0 + 32 < d.ni, nfull = b0 + 32 < d.nj;
             if (mfull && nfull) {
-                if (d.tri && i0 + wm * WTM == j0 + wn * WTN)
-                    kloop_regs<G, true>(d, i0, j0, wm, wn, acc);
+                if (d.tri && a0 == b0)
+                    kloop_regs<G, 0>(d, i0, j0, wm, wn, acc);
                 else
                     kloop_regs<G>(d, i0, j0, wm, wn, acc);
             } else if (mfull) {
-                kloop_regs<G, false, 2, 1>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, QLD_NO_DIAG, 2, 1>(d, i0, j0, wm, wn, acc);
             } else if (nfull) {
-                kloop_regs<G, false, 1, 2>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, QLD_NO_DIAG, 1, 2>(d, i0, j0, wm, wn, acc);
             } else {
-                kloop_regs<G, false, 1, 1>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, QLD_NO_DIAG, 1, 1>(d, i0, j0, wm, wn, acc);
             }
+        } else if constexpr (WAVE_UNITS && WTM == 128 && WTN == 64) {
+            // 128 x 64 wave blocks of 256-tiles: a block whose columns start
+            // on its first row (a triangle's diagonal) has pairs j > i only
+            // in its first 64 rows, and only in their 32-blocks on or right
+            // of the block diagonal: 3 of its 8 MFMA blocks.  (The block
+            // starting 64 columns right of its rows lacks only its last
+            // 32-block; it runs in full: another variant would push the
+            // kernel past its register budget.)  Only the per-wave kernels
+            // of plans carry the variant: in a single region diagonal blocks
+            // are rare and the extra code would only grow the kernel.
+            if (d.tri && b0 == a0)
+                kloop_regs<G, 0, 2, 2>(d, i0, j0, wm, wn, acc);
+            else
+                kloop_regs<G>(d, i0, j0, wm, wn, acc);
         } else {
             kloop_regs<G>(d, i0, j0, wm, wn, acc);
         }
@@ -830,12 +854,52 @@ __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, int wm, in
 // the item list, then steals from the other queues.  Every claim advances a
 // head, so the loop ends once all eight ranges are exhausted.  A unit is
 // item u / SPLIT, wave rows (u % SPLIT) * WAVES_M onwards.
+// PERSIST == 2 (direct operands only): every wave claims units of its own
+// from the queues -- a unit is one wave block, item (desc, bi, bj) naming
+// rows [bi*WTM, bi*WTM + WTM) and columns [bj*WTN, bj*WTN + WTN) of the job --
+// so no wave waits for the others of its workgroup: a wave whose block is
+// short (a triangle's diagonal, a window's ragged edge) moves on at once,
+// and blocks with nothing to compute are not in the list at all.
+template <class OP, int MODE, class G, int PACKED>
+__device__ __forceinline__ void run_wave_units(const KArgs &p, int wave, uint8_t *smem)
+{
+    constexpr int BM = G::EDGE / G::WTM, BN = G::EDGE / G::WTN;   // wave blocks per tile side
+    const int lane = threadIdx.x & 63;
+    const int x0 = (int)(blockIdx.x % QLD_NUM_XCD);
+    for (int q = 0; q < QLD_NUM_XCD; ++q) {
+        const int x = (x0 + q) % QLD_NUM_XCD;
+        const int64_t begin = p.qbeg[x], end = p.qbeg[x + 1];
+        for (;;) {
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(&p.queue[x], 1u);
+            k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+            const int64_t idx = begin + (int64_t)k;
+            if (idx >= end) break;
+            const WorkItem it = wave_uniform(p.items[idx]);
+            const int bi = it.ti, bj = it.tj;
+            if (p.descs) {
+                const WorkDesc d = wave_uniform(p.descs[it.desc]);
+                mfma_tile<OP, MODE, G, PACKED, true>(d, bi / BM, bj / BN, bi % BM, bj % BN,
+                                                     wave, smem);
+            } else {
+                mfma_tile<OP, MODE, G, PACKED, true>(p.inline_desc, bi / BM, bj / BN, bi % BM,
+                                                     bj % BN, wave, smem);
+            }
+        }
+    }
+}
+
 template <class OP, int MODE, class G, int PERSIST, int PACKED>
 __global__ __launch_bounds__(G::THREADS, G::SPLIT) void ld_mfma_kernel(KArgs p)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     const int wave = wave_id();
     const int wr = wave / G::WAVES_N, wn = wave % G::WAVES_N;
+    if constexpr (PERSIST == 2) {
+        static_assert(PACKED == 2, "per-wave units need direct operands");
+        run_wave_units<OP, MODE, G, PACKED>(p, wave, smem);
+        return;
+    }
     if (!PERSIST) {
         const int64_t u = (int64_t)blockIdx.x;
         run_item<OP, MODE, G, PACKED>(p, u / G::SPLIT, (int)(u % G::SPLIT) * G::WAVES_M + wr,
@@ -1108,6 +1172,8 @@ static hipError_t set_lds_limits_packed()
     if ((e = set_lds_limit<FP4, MODE, GeoS, 0, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoS, 1, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoL8H, 0, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoS, 2, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoL8H, 2, 2>()) != hipSuccess) return e;
     return set_lds_limit<FP4, MODE, GeoL8H, 1, 2>();
 }
 
@@ -1145,6 +1211,37 @@ static hipError_t launch_tile(const KArgs &a, int tile, int persistent, hipStrea
 {
     if (tile == QLD_TILE_SMALL) return launch_mode<OP, MODE, GeoS>(a, persistent, s);
     return launch_mode<OP, MODE, GeoL8>(a, persistent, s);
+}
+
+hipError_t launch_mfma_wave_units(int r2_mode, int tile, const WorkDesc *descs,
+                                  const WorkDesc &inline_desc, const WorkItem *units,
+                                  int64_t nunits, int grid, uint32_t *queue, hipStream_t s)
+{
+    if (nunits <= 0) return hipSuccess;
+    if (!queue || grid <= 0) return hipErrorInvalidValue;
+    KArgs a;
+    a.inline_desc = inline_desc;
+    a.descs = descs;
+    a.items = units;
+    a.nitems = nunits;
+    a.queue = queue;
+    for (int x = 0; x <= QLD_NUM_XCD; ++x) a.qbeg[x] = nunits * x / QLD_NUM_XCD;
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, queue);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (tile == QLD_TILE_SMALL)
+        return r2_mode == QLD_R2_EXACT
+                   ? launch_one<FP4, QLD_R2_EXACT, GeoS, 2, 2>(a, (unsigned)grid, s)
+                   : launch_one<FP4, QLD_R2_REFERENCE, GeoS, 2, 2>(a, (unsigned)grid, s);
+    return r2_mode == QLD_R2_EXACT
+               ? launch_one<FP4, QLD_R2_EXACT, GeoL8H, 2, 2>(a, (unsigned)grid, s)
+               : launch_one<FP4, QLD_R2_REFERENCE, GeoL8H, 2, 2>(a, (unsigned)grid, s);
+}
+
+void wave_block_shape(int tile, int *wtm, int *wtn)
+{
+    *wtm = tile == QLD_TILE_SMALL ? GeoS::WTM : GeoL8H::WTM;
+    *wtn = tile == QLD_TILE_SMALL ? GeoS::WTN : GeoL8H::WTN;
 }
 
 hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,

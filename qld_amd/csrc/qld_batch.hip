@@ -19,6 +19,7 @@
 
 struct PlanClass {
     int tile = QLD_TILE_LARGE;
+    bool wave_units = false;        // items are wave blocks claimed per wave
     int grid = 0;                   // persistent workgroups
     size_t ndesc = 0, nitems = 0;
     WorkDesc *d_descs = nullptr;
@@ -207,9 +208,13 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
     if (p->mode == QLD_BATCH_PERSISTENT) {
         for (auto &c : p->classes) {
             if (!c.nitems) continue;
-            e = launch_mfma_items(p->engine, p->r2_mode, c.tile, p->direct ? QLD_FP4_DIRECT : 0,
-                                  c.d_descs, none, c.d_items,
-                                  (int64_t)c.nitems, c.grid, c.d_queue, nullptr, ctx->stream);
+            if (c.wave_units)
+                e = launch_mfma_wave_units(p->r2_mode, c.tile, c.d_descs, none, c.d_items,
+                                           (int64_t)c.nitems, c.grid, c.d_queue, ctx->stream);
+            else
+                e = launch_mfma_items(p->engine, p->r2_mode, c.tile,
+                                      p->direct ? QLD_FP4_DIRECT : 0, c.d_descs, none, c.d_items,
+                                      (int64_t)c.nitems, c.grid, c.d_queue, nullptr, ctx->stream);
             if (e != hipSuccess) return e;
         }
     } else {
@@ -254,7 +259,8 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
     const int nwin = (int)wins.size();
     for (int k = 0; k < nwin; ++k) {
         const PairWin &w = wins[k];
-        const int64_t lim = 65535LL * QLD_TILE_SMALL;
+        // tile and wave-block indices are 16-bit (64-row wave blocks)
+        const int64_t lim = 65535LL * 64;
         if (w.a_start < 0 || w.a_size < 0 || w.b_start < 0 || w.b_size < 0 ||
             (int64_t)w.a_start + w.a_size > t->rows || (int64_t)w.b_start + w.b_size > t->rows ||
             w.out_offset < 0 || w.a_size > lim || w.b_size > lim ||
@@ -301,25 +307,45 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
 
     int rc = QLD_OK;
     if (mode == QLD_BATCH_PERSISTENT) {
+        // Direct fp4 operands: every wave claims wave blocks of its own
+        // (unless the context asks for whole tiles per workgroup).
+        const bool waves = p->direct && ctx->plan_sched == QLD_SCHED_WAVES;
         for (int tile : {QLD_TILE_LARGE, QLD_TILE_SMALL}) {
             std::vector<WorkDesc> descs;
             std::vector<WorkItem> items;
+            int64_t ntiles = 0;
+            int wtm = 0, wtn = 0;
+            wave_block_shape(tile, &wtm, &wtn);
             for (int k : order) {
                 const PairWin &w = wins[k];
                 if (pair_empty(w) || pair_tile(ctx, w) != tile) continue;
+                const size_t tiles = job_tile_count(w.b_size, w.a_size, w.tri, tile);
                 size_t cnt = items.size();
-                items.resize(cnt + job_tile_count(w.b_size, w.a_size, w.tri, tile));
-                append_job_tiles(items.data(), &cnt, (uint32_t)descs.size(), w.b_size, w.a_size,
-                                 w.tri, tile);
+                if (waves) {
+                    items.resize(cnt + tiles * (size_t)(tile / wtm) * (size_t)(tile / wtn));
+                    cnt += append_job_wave_units(items.data() + cnt, (uint32_t)descs.size(),
+                                                 w.b_size, w.a_size, w.tri, tile, wtm, wtn);
+                    items.resize(cnt);
+                } else {
+                    items.resize(cnt + tiles);
+                    append_job_tiles(items.data(), &cnt, (uint32_t)descs.size(), w.b_size,
+                                     w.a_size, w.tri, tile);
+                }
+                ntiles += (int64_t)tiles;
                 descs.push_back(desc_of(w));
                 p->pairs += pair_count(w);
             }
             if (items.empty()) continue;
             PlanClass c;
             c.tile = tile;
+            c.wave_units = waves;
             c.ndesc = descs.size();
             c.nitems = items.size();
-            c.grid = std::min<int64_t>(persistent_grid(ctx, tile), (int64_t)items.size());
+            // wave units: two 4-wave workgroups per CU for both tile edges
+            c.grid = waves ? (int)std::min<int64_t>(2LL * ctx->num_cu,
+                                                    ((int64_t)items.size() + 3) / 4)
+                           : (int)std::min<int64_t>(persistent_grid(ctx, tile),
+                                                    (int64_t)items.size());
             if ((rc = plan_upload(ctx, p, (void **)&c.d_descs, descs.data(),
                                   descs.size() * sizeof(WorkDesc))) ||
                 (rc = plan_upload(ctx, p, (void **)&c.d_items, items.data(),
@@ -328,7 +354,7 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
                 plan_release(ctx, p);
                 return rc;
             }
-            p->tiles += (int64_t)items.size();
+            p->tiles += ntiles;
             p->launches += 1;
             p->classes.push_back(c);
         }
@@ -359,6 +385,11 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
     }
 
     // Capture the launch sequence once; replays cost one graph launch.
+    if (!ctx->plan_graphs) {
+        p->use_graph = false;
+        *plan = p;
+        return QLD_OK;
+    }
     hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         hipError_t ei = plan_issue(ctx, p);
@@ -458,6 +489,25 @@ extern "C" int qld_plan_free(qld_ctx *ctx, qld_plan *p)
 
 // ---- tile-shape autotuner ---------------------------------------------------
 
+extern "C" int qld_set_plan_schedule(qld_ctx *ctx, int sched)
+{
+    if (!ctx || (sched != QLD_SCHED_TILES && sched != QLD_SCHED_WAVES)) return QLD_ERR_INVALID;
+    ctx->plan_sched = sched;
+    return QLD_OK;
+}
+
+extern "C" int qld_set_plan_graphs(qld_ctx *ctx, int capture)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    ctx->plan_graphs = capture != 0;
+    return QLD_OK;
+}
+
+extern "C" int qld_get_plan_schedule(const qld_ctx *ctx)
+{
+    return ctx ? ctx->plan_sched : -1;
+}
+
 extern "C" int qld_set_tile_cutoff(qld_ctx *ctx, int cutoff)
 {
     if (!ctx || cutoff < 0) return QLD_ERR_INVALID;
@@ -472,7 +522,9 @@ extern "C" int qld_get_tile_cutoff(const qld_ctx *ctx)
 
 // Time the window set as persistent plans under each candidate cutoff
 // (windows of at least `cutoff` SNPs on 256-tiles, the rest on 128-tiles)
-// and keep the fastest for later plans of this context.
+// and, for direct fp4 tables, each work schedule (wave blocks claimed per
+// wave, or whole tiles per workgroup), and keep the fastest pair for later
+// plans of this context.
 extern "C" int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
                                     int nwin, float *out, int reps, int *best_cutoff,
                                     float *best_ms)
@@ -480,8 +532,8 @@ extern "C" int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_
     if (!ctx || !t || nwin <= 0 || !wins || !out) return QLD_ERR_INVALID;
     if (reps <= 0) reps = 3;
     static const int candidates[] = {0, 256, 384, 512, 768, 1024, 1536, 2048, 1 << 30};
-    const int saved = ctx->tile_cutoff;
-    int best = saved;
+    const int saved = ctx->tile_cutoff, saved_sched = ctx->plan_sched;
+    int best = saved, best_sched = saved_sched;
     float best_t = 3.4e38f;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return QLD_ERR_DEVICE;
@@ -489,31 +541,53 @@ extern "C" int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_
         hipEventDestroy(e0);
         return QLD_ERR_DEVICE;
     }
+    // Every candidate plan is built first and then timed in two rounds
+    // (fastest round kept), after a warm-up: the GPU's clocks ramp up over
+    // the first launches after idle, which would otherwise penalize
+    // whichever candidate runs first.
+    struct Cand {
+        int sched, cut;
+        qld_plan *p;
+        float ms;
+    };
+    std::vector<Cand> cands;
     int rc = QLD_OK;
-    for (int cut : candidates) {
-        ctx->tile_cutoff = cut;
-        qld_plan *p = nullptr;
-        if ((rc = qld_plan_windows(ctx, t, wins, nwin, out, QLD_BATCH_PERSISTENT, &p))) break;
-        if ((rc = qld_plan_run(ctx, p)) == QLD_OK) {            // warm-up
+    const int nsched = t->direct ? 2 : 1;
+    for (int si = 0; si < nsched && rc == QLD_OK; ++si) {
+        ctx->plan_sched = t->direct ? (si ? QLD_SCHED_TILES : QLD_SCHED_WAVES) : saved_sched;
+        for (int cut : candidates) {
+            ctx->tile_cutoff = cut;
+            qld_plan *p = nullptr;
+            if ((rc = qld_plan_windows(ctx, t, wins, nwin, out, QLD_BATCH_PERSISTENT, &p))) break;
+            cands.push_back(Cand{ctx->plan_sched, cut, p, 3.4e38f});
+        }
+    }
+    for (int w = 0; w < 3 * reps && rc == QLD_OK && !cands.empty(); ++w)
+        rc = qld_plan_run(ctx, cands[0].p);
+    for (int round = 0; round < 2 && rc == QLD_OK; ++round)
+        for (auto &c : cands) {
+            if ((rc = qld_plan_run(ctx, c.p))) break;           // warm-up
             hipEventRecord(e0, ctx->stream);
-            for (int r = 0; r < reps && rc == QLD_OK; ++r) rc = qld_plan_run(ctx, p);
+            for (int r = 0; r < reps && rc == QLD_OK; ++r) rc = qld_plan_run(ctx, c.p);
             hipEventRecord(e1, ctx->stream);
             float ms = 0.0f;
             if (rc == QLD_OK && hipEventSynchronize(e1) == hipSuccess &&
-                hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
-                ms /= reps;
-                if (ms < best_t) {
-                    best_t = ms;
-                    best = cut;
-                }
-            }
+                hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                c.ms = std::min(c.ms, ms / reps);
+            if (rc) break;
         }
-        qld_plan_free(ctx, p);
-        if (rc) break;
+    for (auto &c : cands) {
+        if (rc == QLD_OK && c.ms < best_t) {
+            best_t = c.ms;
+            best = c.cut;
+            best_sched = c.sched;
+        }
+        qld_plan_free(ctx, c.p);
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     ctx->tile_cutoff = rc == QLD_OK ? best : saved;
+    ctx->plan_sched = rc == QLD_OK ? best_sched : saved_sched;
     if (best_cutoff) *best_cutoff = ctx->tile_cutoff;
     if (best_ms) *best_ms = best_t;
     return rc;

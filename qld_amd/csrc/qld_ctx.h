@@ -43,6 +43,8 @@ struct qld_ctx {
     int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape
     int fp4_packed = QLD_FP4_DIRECT;   // fp4 operands: QLD_FP4_DIRECT, _PACKED or _EXPANDED
     void *scratch = nullptr;  // device address of this device's static scratch array
+    int plan_sched = QLD_SCHED_WAVES;  // persistent plans: per-wave units or whole tiles
+    bool plan_graphs = true;  // capture each plan's launches in a HIP graph
     char err[512] = {0};
 };
 

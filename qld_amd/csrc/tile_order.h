@@ -69,3 +69,37 @@ __host__ __device__ inline int64_t job_tile_index(int ti, int tj, int ti_count, 
     const int64_t before = S * S * (J * (J - 1) / 2) + J * (S * (S + 1) / 2) + I * S * cols;
     return before + (I < J ? b * S + a : b * (b + 1) / 2 + a);
 }
+
+// Wave blocks of one job (WTM x WTN blocks inside tiles of edge T) that hold
+// at least one cell to compute, tile by tile in append_job_tiles' order and
+// row by row inside a tile, as units (desc, bi, bj): block rows
+// [bi*WTM, bi*WTM + WTM), columns [bj*WTN, bj*WTN + WTN).  A block is left
+// out when it has no row or no column inside the job, or lies wholly below
+// a triangle's diagonal (every column <= every row).  Returns the units
+// written; `out` needs room for job_tile_count(...) * (T/WTM) * (T/WTN).
+inline size_t append_job_wave_units(WorkItem *out, uint32_t desc, int ni, int nj, int tri,
+                                    int T, int WTM, int WTN)
+{
+    const int S = QLD_SUPER_TILE;
+    const int ti_count = (ni + T - 1) / T, tj_count = (nj + T - 1) / T;
+    const int si = (ti_count + S - 1) / S, sj = (tj_count + S - 1) / S;
+    const int BM = T / WTM, BN = T / WTN;
+    size_t k = 0;
+    for (int J = 0; J < sj; ++J)
+        for (int I = 0; I < si; ++I) {
+            if (tri && I > J) continue;
+            for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj)
+                for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti) {
+                    if (tri && ti > tj) continue;
+                    for (int wm = 0; wm < BM; ++wm)
+                        for (int wn = 0; wn < BN; ++wn) {
+                            const int a0 = ti * T + wm * WTM, b0 = tj * T + wn * WTN;
+                            if (a0 >= ni || b0 >= nj) continue;
+                            if (tri && b0 + WTN <= a0 + 1) continue;
+                            out[k++] = WorkItem{desc, (uint16_t)(ti * BM + wm),
+                                                (uint16_t)(tj * BN + wn)};
+                        }
+                }
+        }
+    return k;
+}

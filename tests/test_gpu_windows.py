@@ -92,6 +92,62 @@ def test_window_plans(engine, mode, cutoff, engine_factory):
     out.free()
 
 
+@pytest.mark.parametrize("sched", ["tiles", "waves"])
+@pytest.mark.parametrize("cutoff", [300, 0, 1 << 30])
+def test_window_plans_both_schedules(sched, cutoff, engine_factory):
+    """Persistent fp4 plans with whole tiles per workgroup and with wave
+    blocks claimed per wave, 128- and 256-tiles, every window size from 1
+    to 1,100 SNPs in the mix (ragged edges and diagonals of both wave block
+    shapes), in both r^2 modes."""
+    N, rows = 16622, 6000
+    A, bc = qld_amd.synth_blocks(4244, rows, N)
+    rng = np.random.default_rng(cutoff % 89 + (sched == "tiles"))
+    wins = _windows(rng, rows, 48, 2, 1100)
+    wins = qld_amd.window_layout(np.concatenate([wins["start"], [0, 5, 700, 1000]]),
+                                 np.concatenate([wins["size"], [1, 256, 512, 1024]]))
+    total = int((wins["size"].astype(np.int64) ** 2).sum())
+    for mode in ("reference", "exact"):
+        e = engine_factory(engine="mfma_fp4", r2_mode=mode)
+        e.set_tile_cutoff(cutoff)
+        e.set_plan_schedule(sched)
+        t = e.table(A, bc, N)
+        out = e.alloc(total * 4)
+        plan = e.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
+        say("plan", sched, cutoff, mode, plan.stats())
+        for use_graph in (True, False):
+            out.fill(0x7FC00000)
+            plan.use_graph(use_graph)
+            plan.run()
+            e.sync()
+            _check_windows(out.download(total, np.float32), wins, A, bc, N, mode)
+        plan.free()
+        t.free()
+        out.free()
+
+
+def test_window_plan_without_graph_capture(engine_factory):
+    """A plan built with graph capture switched off runs by direct launches
+    (and refuses graph replay) with the same results."""
+    N, rows = 16622, 3000
+    A, bc = qld_amd.synth_blocks(4245, rows, N)
+    wins = _windows(np.random.default_rng(11), rows, 24, 2, 900)
+    total = int((wins["size"].astype(np.int64) ** 2).sum())
+    e = engine_factory(engine="mfma_fp4", r2_mode="reference")
+    e.set_plan_graphs(False)
+    t = e.table(A, bc, N)
+    out = e.alloc(total * 4)
+    plan = e.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
+    with pytest.raises(qld_amd.QldError):
+        plan.use_graph(True)
+    out.fill(0x7FC00000)
+    plan.run()
+    e.sync()
+    _check_windows(out.download(total, np.float32), wins, A, bc, N)
+    plan.free()
+    t.free()
+    out.free()
+
+
 @pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x"])
 def test_table_from_device_rows(engine, engine_factory):
     """A table built from device buffers (qld_table_create_dev) gives the
@@ -173,11 +229,13 @@ def test_autotuner_picks_a_candidate(engine_factory):
     out.free()
 
 
-@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x", "mfma_i8"])
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_fp4x", "mfma_i8", "mfma_fp4:tiles"])
 @pytest.mark.parametrize("mode", [qld_amd.BATCH_PERSISTENT, qld_amd.BATCH_PLAIN])
 def test_pair_window_plans_mix_triangles_and_rectangles(engine, mode, engine_factory):
     """qld_plan_pairs: triangles and rectangles (overlapping, disjoint,
-    empty, 1-row) in one plan, each against the oracle."""
+    empty, 1-row) in one plan, each against the oracle (fp4 with wave
+    blocks claimed per wave, the default, and with whole tiles)."""
+    engine, _, sched = engine.partition(":")
     N, rows = 16622, 3000
     A, bc = qld_amd.synth_blocks(777, rows, N)
     rng = np.random.default_rng(mode * 7 + (engine == "mfma_i8"))
@@ -196,6 +254,8 @@ def test_pair_window_plans_mix_triangles_and_rectangles(engine, mode, engine_fac
         off += a * b
     e = engine_factory(engine=engine, r2_mode="reference")
     e.set_tile_cutoff(400)
+    if sched:
+        e.set_plan_schedule(sched)
     t = e.table(A, bc, N)
     out = e.alloc(max(off, 1) * 4)
     plan = e.plan_windows(t, w, out, mode)
