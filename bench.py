@@ -89,29 +89,125 @@ def roofline(engine, pairs, words, samples, kernel_s, traffic=None, rows=None):
     return out
 
 
-def cpu_baseline(A, bc, N, budget_s=10.0):
-    """Time the oracle's CPU restatement on the first S SNPs (triangle)."""
-    import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+def host_cpu():
+    """Physical cores (lscpu: sockets x cores per socket), the CPUs this
+    process may run on, the thread count the box sets, and the CPU model."""
+    import subprocess
+    info = {"model": "unknown", "physical_cores": None,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name", "unknown")
+        info["physical_cores"] = int(kv["Socket(s)"]) * int(kv["Core(s) per socket"])
+    except (OSError, KeyError, ValueError, subprocess.SubprocessError):
+        pass
+    env = os.environ.get("OMP_NUM_THREADS")
+    info["omp_num_threads"] = int(env) if env and env.isdigit() else None
+    return info
 
-    def run(S):
+
+def cpu_baseline(A, bc, N, budget_s=10.0):
+    """The oracle's C restatement of the reference's BLIS CPU path (blocked
+    AND+popcount counts, get_pairwise_ld_score's r^2) on the first S SNPs of
+    the workload (triangle), on every physical core this process may use and
+    on one core; with the calibration against the reference's own BLIS build
+    (oracle/calibrate_blis.py, run where the reference can be built)."""
+    import oracle
+    cpu = host_cpu()
+    threads = min(x for x in (cpu["physical_cores"], cpu["affinity_cpus"],
+                              cpu["omp_num_threads"]) if x)
+
+    def run(S, T):
         sub = np.ascontiguousarray(A[:S])
         t0 = time.perf_counter()
-        c = oracle.counts_blocked(sub, tri=True, nthreads=threads)
+        c = oracle.counts_blocked(sub, tri=True, nthreads=T)
         oracle.r2_reference(bc[:S], bc[:S], c, N, tri=True)
         return time.perf_counter() - t0
 
-    S = min(3000, A.shape[0])
-    t = run(S)
-    S2 = int(min(A.shape[0], S * (budget_s / max(t, 1e-3)) ** 0.5))
-    if S2 > S:
-        S = S2
-        t = run(S)
-    pairs = S * (S - 1) / 2
-    return {"value": pairs / t / 1e9, "unit": "Gpairs/s", "cores": threads,
-            "kind": "port",
+    def sample(T, budget):
+        S = min(3000, A.shape[0])
+        t = run(S, T)
+        S2 = int(min(A.shape[0], S * (budget / max(t, 1e-3)) ** 0.5))
+        if S2 > S:
+            S = S2
+            t = run(S, T)
+        return S, t
+
+    S, t = sample(threads, budget_s)
+    S1, t1 = sample(1, budget_s / 2)
+    pairs, pairs1 = S * (S - 1) / 2, S1 * (S1 - 1) / 2
+    calib = None
+    try:
+        with open(os.path.join(ROOT, "oracle", "blis_calibration.json")) as f:
+            cal = json.load(f)
+        calib = {"measured_on": cal["host"],
+                 "restatement_vs_reference_blis": {
+                     f"{c['snps']}_snps_{k}": c[k]["restatement_speedup"]
+                     for c in cal["configs"] for k in c if k.startswith("threads_")},
+                 "reference_blis_counts_r2_s": {
+                     f"{c['snps']}_snps_{k}": c[k]["reference_counts_r2_s"]
+                     for c in cal["configs"] for k in c if k.startswith("threads_")}}
+    except (OSError, KeyError, ValueError):
+        pass
+    return {"value": pairs / t / 1e9, "unit": "Gpairs/s", "cores": threads, "kind": "port",
+            "physical_cores": cpu["physical_cores"], "cpu_model": cpu["model"],
+            "popcount_bits": oracle.counts_blocked_isa(),
             "sample": f"first {S} SNPs x {N} samples, lower triangle "
-                      f"({int(pairs)} pairs), counts + r^2, {t:.2f} s"}
+                      f"({int(pairs)} pairs), counts + r^2, {t:.2f} s on {threads} threads",
+            "one_core": {"value": pairs1 / t1 / 1e9, "unit": "Gpairs/s",
+                         "sample": f"first {S1} SNPs, {int(pairs1)} pairs, {t1:.2f} s"},
+            "calibration_vs_blis": calib}
+
+
+def config1_bench(args, steps):
+    """Configuration 1: 2,000 SNPs x 16,622 samples, all-pairs r^2 (lower
+    triangle) on the GPU -- the plain single-region call -- and on the CPU
+    through the restatement of the BLIS path, all threads and one."""
+    import oracle
+    import qld_amd
+    N, m = 16622, 2000
+    A, bc = qld_amd.synth_blocks(20261203 + 1, m, N)
+    W = A.shape[1]
+    eng = qld_amd.Engine(engine="auto", r2_mode=args.r2_mode, timing=True)
+    dA = eng.alloc(A.nbytes).upload(A)
+    dbc = eng.alloc(bc.nbytes).upload(bc)
+    dres = eng.alloc(m * m * 4)
+    for _ in range(5):
+        eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
+    eng.sync()
+    eng.timer_start()
+    for _ in range(steps):
+        eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
+    ms = eng.timer_stop() / steps
+    engine = eng.last_engine
+    for b in (dA, dbc, dres):
+        b.free()
+    eng.close()
+    pairs = m * (m - 1) // 2
+    cpu = host_cpu()
+    threads = min(x for x in (cpu["physical_cores"], cpu["affinity_cpus"],
+                              cpu["omp_num_threads"]) if x)
+    out = {"workload": f"single region all-pairs r2, {m} SNPs x {N} samples, lower triangle",
+           "pairs": pairs, "gpu": {"ms": round(ms, 4), "gpairs_s": round(pairs / ms / 1e6, 2),
+                                   "engine": engine}}
+    for T in sorted({1, threads}):
+        best = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            c = oracle.counts_blocked(A, tri=True, nthreads=T)
+            oracle.r2_reference(bc, bc, c, N, tri=True)
+            best = min(best, time.perf_counter() - t0)
+        out[f"cpu_{T}_threads"] = {"s": round(best, 4), "gpairs_s": round(pairs / best / 1e9, 4)}
+    out["gpu_vs_cpu_all_threads"] = round(out["gpu"]["gpairs_s"] /
+                                          out[f"cpu_{threads}_threads"]["gpairs_s"], 1)
+    log(f"[bench] config 1: GPU {ms:.4f} ms, CPU {out[f'cpu_{threads}_threads']['s']} s "
+        f"on {threads} threads")
+    return out
 
 
 def windows_bench(args, steps):
@@ -628,6 +724,8 @@ def main():
         out["input_list"] = list_bench(args)
 
     if not args.no_cpu:
+        log("[bench] config 1 (2,000 SNPs): GPU and CPU")
+        out["config1"] = config1_bench(args, max(3, args.steps))
         log("[bench] CPU baseline (oracle restatement of the BLIS path)")
         out["cpu_baseline"] = cpu_baseline(A, bc, N)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

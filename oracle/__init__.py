@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
         for f in ("qo_r2_reference", "qo_r2_exact", "qo_r2_fp64"):
             getattr(L, f).argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int,
                                       C.c_int, vp]
+        L.qo_counts_blocked_isa.restype = C.c_int
+        L.qo_counts_blocked_isa.argtypes = []
         L.qo_fprint_f.restype = C.c_int
         L.qo_fprint_f.argtypes = [C.c_char_p, C.c_float, C.c_int]
         L.qo_write_results.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp,
@@ -82,6 +84,12 @@ def counts_naive(A, B=None):
     out = np.zeros((n, m), dtype=np.uint32)
     lib().qo_counts_naive(_p(A), m, _p(B), n, W, _p(out))
     return out
+
+
+def counts_blocked_isa() -> int:
+    """512 when counts_blocked runs its AVX-512 VPOPCNTQ micro-kernel here,
+    64 for the scalar popcnt one."""
+    return int(lib().qo_counts_blocked_isa())
 
 
 def counts_blocked(A, B=None, tri=None, nthreads=0, out=None):

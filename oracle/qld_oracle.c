@@ -8,6 +8,7 @@
  */
 #include "qld_oracle.h"
 
+#include <immintrin.h>
 #include <math.h>
 #include <string.h>
 #include <stdlib.h>
@@ -81,6 +82,44 @@ static void micro_4x4(const uint64_t *const *a, const uint64_t *const *b,
             acc[q][p] += (uint32_t)t[q][p];
 }
 
+/* The same micro-tile with 512-bit AND + VPOPCNTQ: eight words of a row per
+ * instruction, the 16 pair counts in 16 vector accumulators (the hosts the
+ * CPU baseline runs on have AVX-512 VPOPCNTDQ; others take micro_4x4). */
+__attribute__((target("avx512f,avx512vpopcntdq")))
+static void micro_4x4_vpop(const uint64_t *const *a, const uint64_t *const *b,
+                           int kc, uint32_t acc[QO_NR][QO_MR])
+{
+    __m512i t[QO_NR][QO_MR];
+    for (int q = 0; q < QO_NR; q++)
+        for (int p = 0; p < QO_MR; p++)
+            t[q][p] = _mm512_setzero_si512();
+    int k = 0;
+    for (; k < kc; k += 8) {
+        const __mmask8 msk = kc - k >= 8 ? (__mmask8)0xFF : (__mmask8)((1u << (kc - k)) - 1u);
+        __m512i av[QO_MR];
+        for (int p = 0; p < QO_MR; p++) av[p] = _mm512_maskz_loadu_epi64(msk, a[p] + k);
+        for (int q = 0; q < QO_NR; q++) {
+            const __m512i bq = _mm512_maskz_loadu_epi64(msk, b[q] + k);
+            for (int p = 0; p < QO_MR; p++)
+                t[q][p] = _mm512_add_epi64(t[q][p],
+                                           _mm512_popcnt_epi64(_mm512_and_si512(av[p], bq)));
+        }
+    }
+    for (int q = 0; q < QO_NR; q++)
+        for (int p = 0; p < QO_MR; p++)
+            acc[q][p] += (uint32_t)_mm512_reduce_add_epi64(t[q][p]);
+}
+
+static int qo_has_vpop(void)
+{
+    static int has = -1;
+    if (has < 0)
+        has = __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512vpopcntdq");
+    return has;
+}
+
+int qo_counts_blocked_isa(void) { return qo_has_vpop() ? 512 : 64; }
+
 static const uint64_t qo_zero_row[QO_KC];
 
 void qo_counts_blocked(const uint64_t *A, int m, const uint64_t *B, int n,
@@ -88,6 +127,7 @@ void qo_counts_blocked(const uint64_t *A, int m, const uint64_t *B, int n,
 {
     int mt = (m + QO_MC - 1) / QO_MC, nt = (n + QO_NC - 1) / QO_NC;
     long ntiles = (long)mt * nt;
+    const int vpop = qo_has_vpop();
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
@@ -111,7 +151,8 @@ void qo_counts_blocked(const uint64_t *A, int m, const uint64_t *B, int n,
                     for (int q = 0; q < QO_NR; q++)
                         bp[q] = i + q < n ? B + (size_t)(i + q) * words + pc
                                           : qo_zero_row;
-                    micro_4x4(ap, bp, kc, acc);
+                    if (vpop) micro_4x4_vpop(ap, bp, kc, acc);
+                    else micro_4x4(ap, bp, kc, acc);
                 }
                 for (int q = 0; q < QO_NR && i + q < n; q++)
                     for (int p = 0; p < QO_MR && j + p < m; p++)

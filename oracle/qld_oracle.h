@@ -44,6 +44,10 @@ void qo_counts_naive(const uint64_t *A, int m, const uint64_t *B, int n,
  * pinned by: qo_counts_naive on ragged shapes (tests). */
 void qo_counts_blocked(const uint64_t *A, int m, const uint64_t *B, int n,
                        int words, int tri, uint32_t *out, int nthreads);
+/* Vector width in bits of the popcount micro-kernel qo_counts_blocked uses
+ * on this host: 512 (AND + VPOPCNTQ on AVX-512 VPOPCNTDQ hosts) or 64
+ * (scalar popcnt, as bin_gemm.c does). */
+int qo_counts_blocked_isa(void);
 
 /* r^2 exactly as get_pairwise_ld_score computes it (gemm_cpu.c:343-404;
  * the same arithmetic as get_pairwise_ld_score_gpu, gemm_gpu.c:700-737), as

@@ -176,8 +176,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t panel_rsrc(const uint8_t *base
     int64_t lim = (int64_t)TILE * row_bytes;
     int64_t nrec = have < lim ? have : lim;
     if (nrec < 0) nrec = 0;
+    // num_records is an unsigned 32-bit field; launches keep TILE * row_bytes
+    // at or below 2^31 (expanded_tile_cap)
+    if (nrec > 0xFFFFFFFFll) nrec = 0xFFFFFFFFll;
     return __builtin_amdgcn_make_buffer_rsrc((void *)(base + (int64_t)row0 * row_bytes),
-                                             0, (int)nrec, 0x00020000);
+                                             0, (int)(uint32_t)nrec, 0x00020000);
 }
 
 // Queue one k stage of one operand: wave w fills tile rows

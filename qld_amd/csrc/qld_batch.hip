@@ -67,7 +67,7 @@ static int table_create(qld_ctx *ctx, const void *rows_src, const void *bc_src,
                         qld_table **out)
 {
     if (!ctx || !out || !rows_src || !bc_src || rows <= 0 || words <= 0 ||
-        snp_size <= 0 || snp_size >= QLD_MAX_SNP_SIZE) {
+        words > QLD_MAX_WORDS || snp_size <= 0 || snp_size >= QLD_MAX_SNP_SIZE) {
         if (ctx) qld_set_error(ctx, "qld_table_create: bad arguments (rows=%d words=%d N=%d)",
                                rows, words, snp_size);
         return QLD_ERR_INVALID;
@@ -248,9 +248,10 @@ static bool pair_empty(const PairWin &w)
     return w.tri ? w.a_size < 2 : (w.a_size == 0 || w.b_size == 0);
 }
 
-static int pair_tile(const qld_ctx *ctx, const PairWin &w)
+static int pair_tile(const qld_ctx *ctx, const qld_table *t, const PairWin &w)
 {
-    return qld_window_tile(ctx, std::min(w.a_size, w.b_size));
+    const int tile = qld_window_tile(ctx, std::min(w.a_size, w.b_size));
+    return t->direct ? tile : expanded_tile_cap(t->row_bytes, tile);
 }
 
 static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWin> &wins,
@@ -318,7 +319,7 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
             wave_block_shape(tile, &wtm, &wtn);
             for (int k : order) {
                 const PairWin &w = wins[k];
-                if (pair_empty(w) || pair_tile(ctx, w) != tile) continue;
+                if (pair_empty(w) || pair_tile(ctx, t, w) != tile) continue;
                 const size_t tiles = job_tile_count(w.b_size, w.a_size, w.tri, tile);
                 size_t cnt = items.size();
                 if (waves) {
@@ -363,7 +364,7 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
         for (int k : order) {
             const PairWin &w = wins[k];
             if (pair_empty(w)) continue;
-            const int tile = pair_tile(ctx, w);
+            const int tile = pair_tile(ctx, t, w);
             PlainLaunch l;
             l.tile = tile;
             l.item_off = (int64_t)items.size();

@@ -11,6 +11,18 @@
 // fp32 accumulators of the MFMA engines and c/N must round like the
 // reference's float division (see ld_epilogue.h).
 #define QLD_MAX_SNP_SIZE (1 << 24)
+// Largest row width in packed words: a count is at most 64 * words, and
+// fp32 sums hold every integer up to 2^24 exactly.
+#define QLD_MAX_WORDS (1 << 18)
+
+// Tile edge an expanded-operand launch may use: its buffer descriptors span
+// a tile's rows and address them with 32-bit offsets, so tile * row_bytes
+// is kept at or below 2^31 (128-tiles always are: row_bytes <= 64 *
+// QLD_MAX_WORDS = 2^24).
+static inline int expanded_tile_cap(int64_t row_bytes, int tile)
+{
+    return (int64_t)tile * row_bytes <= ((int64_t)1 << 31) ? tile : 128;
+}
 
 // One all-pairs job over device buffers.  Row i indexes table B, column j
 // indexes table A (the reference's results[i*m + j]).

@@ -323,8 +323,9 @@ extern "C" int qld_sync(qld_ctx *ctx)
 
 static int check_job(qld_ctx *ctx, const LdJob &j)
 {
-    if (j.m < 0 || j.n < 0 || j.words <= 0) {
-        qld_set_error(ctx, "bad shape m=%d n=%d words=%d", j.m, j.n, j.words);
+    if (j.m < 0 || j.n < 0 || j.words <= 0 || j.words > QLD_MAX_WORDS) {
+        qld_set_error(ctx, "bad shape m=%d n=%d words=%d (words 1 .. %d)", j.m, j.n, j.words,
+                      QLD_MAX_WORDS);
         return QLD_ERR_INVALID;
     }
     // snp_size is the N of the frequencies (the header's sample count, as
@@ -411,7 +412,8 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     // asks for row-major packed operands (PACKED) or an expanded copy.
     const int pmode = engine == QLD_ENGINE_MFMA_FP4 ? ctx->fp4_packed : QLD_FP4_EXPANDED;
     const bool packed = pmode != QLD_FP4_EXPANDED;
-    const int T = pmode == QLD_FP4_PACKED ? QLD_TILE_SMALL : plain_tile_edge(ctx, job, pmode);
+    int T = pmode == QLD_FP4_PACKED ? QLD_TILE_SMALL : plain_tile_edge(ctx, job, pmode);
+    if (!packed) T = expanded_tile_cap(mfma_row_bytes(engine, job.words), T);
     if ((job.n + T - 1) / T > 65535 || (job.m + T - 1) / T > 65535) {
         qld_set_error(ctx, "too many rows for one launch (m=%d n=%d)", job.m, job.n);
         return QLD_ERR_INVALID;

@@ -336,3 +336,58 @@ def test_forced_tile_shapes(engine, mode, tile, engine_factory):
     for k, i in enumerate(rows):
         j = np.arange(i + 1, 4200)
         assert (cnt[i, j] == want_c[k, j]).all()
+
+
+@pytest.mark.parametrize("engine", ["mfma_i8", "mfma_fp4x", "mfma_fp4", "popcount"])
+def test_widest_rows_many_rows(engine, engine_factory):
+    """Rows of the largest accepted width, 262,144 words (N = 2^24 - 1), 300
+    of them: counts up to 2^24 in the fp32 sums, and expanded operand rows
+    of 8 MiB (fp4) / 16 MiB (int8), whose 256-row tiles would span 2^32
+    bytes -- such launches drop to 128-tiles.  Triangle with either tile
+    edge requested, a rectangle, and a window plan with 256-tiles asked for;
+    every count against the oracle.  A width of 262,145 words is rejected."""
+    N = (1 << 24) - 1
+    W = (N + 63) // 64
+    rows = 300
+    rng = np.random.default_rng(262144)
+    A = rng.integers(0, 2**63, (rows, W), dtype=np.uint64) | (rng.integers(0, 2, (rows, W),
+                                                                          dtype=np.uint64) << 63)
+    A[:, -1] &= ~np.uint64(1)            # the pad bit of the last word (N = 64 W - 1)
+    A[0, :] = ~np.uint64(0)              # an all-ones row: counts of 2^24 - 1
+    A[0, -1] = ~np.uint64(1)
+    bc = np.bitwise_count(A).sum(axis=1).astype(np.uint32)
+    want = oracle.counts_blocked(A, tri=False)
+    e = engine_factory(engine=engine, r2_mode="exact")
+    for tile in (256, 128):
+        if engine != "popcount":
+            e.set_plain_tile(tile)
+        say("widest rows", engine, tile)
+        res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+        sel = np.triu(np.ones((rows, rows), dtype=bool), 1)
+        assert (cnt[sel] == want[sel]).all(), f"tile {tile}: counts differ"
+    res, cnt = e.correlate(A[:130], bc[:130], A[130:], bc[130:], snp_size=N, triangular=False,
+                           counts=True)
+    assert (cnt == want[130:, :130]).all(), "rectangle counts differ"
+    if engine != "popcount":
+        e.set_tile_cutoff(0)
+        t = e.table(A, bc, N)
+        wins = qld_amd.window_layout(np.array([0, 20], np.int64), np.array([300, 270], np.int64))
+        total = int((wins["size"].astype(np.int64) ** 2).sum())
+        out = e.alloc(total * 4)
+        plan = e.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
+        plan.run()
+        e.sync()
+        got = out.download(total, np.float32)
+        for w in wins:
+            s, n, off = int(w["start"]), int(w["size"]), int(w["out_offset"])
+            g = got[off:off + n * n].reshape(n, n)
+            want_r2 = oracle.r2_exact(bc[s:s + n], bc[s:s + n], want[s:s + n, s:s + n], N)
+            sel = np.triu(np.ones((n, n), dtype=bool), 1)
+            assert (g.view(np.uint32)[sel] == want_r2.view(np.uint32)[sel]).all(), \
+                f"window {s}+{n} differs"
+        plan.free()
+        t.free()
+        out.free()
+    with pytest.raises(qld_amd.QldError) as x:
+        e.correlate(np.zeros((2, W + 1), np.uint64), np.zeros(2, np.uint32), snp_size=1000)
+    assert x.value.code == qld_amd.QLD_ERR_INVALID
