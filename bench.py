@@ -164,6 +164,44 @@ def cpu_baseline(A, bc, N, budget_s=10.0):
             "calibration_vs_blis": calib}
 
 
+def engine_grid(args):
+    """Every contraction engine (popcount VALU, int8 MFMA, fp4 MFMA with
+    direct operands) on single-region triangles of 64 to 30,000 SNPs at 256
+    to 100,000 samples: kernel time (HIP events around the contraction
+    kernel), Gpairs/s and the fraction of the engine's peak.  The evidence
+    behind the AUTO engine rule."""
+    import qld_amd
+    rows = []
+    for N in (256, 1000, 2048, 4096, 16622, 22554, 100000):
+        for m in (64, 200, 500, 2000, 8000, 30000):
+            A, bc = qld_amd.synth_blocks(20261203 + 60 + m, m, N)
+            W = A.shape[1]
+            pairs = m * (m - 1) // 2
+            for engine in ("popcount", "mfma_i8", "mfma_fp4"):
+                eng = qld_amd.Engine(engine=engine, r2_mode="exact", timing=True)
+                dA = eng.alloc(A.nbytes).upload(A)
+                dbc = eng.alloc(bc.nbytes).upload(bc)
+                dres = eng.alloc(m * m * 4)
+                for _ in range(2):
+                    eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
+                ks = []
+                for _ in range(3):
+                    eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
+                    ks.append(eng.last_contract_ms())
+                for b in (dA, dbc, dres):
+                    b.free()
+                eng.close()
+                k = float(np.median(ks))
+                rl = roofline(engine, pairs, W, N, k / 1e3)
+                rows.append({"samples": N, "snps": m, "engine": engine, "kernel_ms": round(k, 4),
+                             "gpairs_s": round(pairs / k / 1e6, 2), "frac": round(rl["frac"], 3)})
+                log(f"[bench] grid N={N} m={m} {engine}: {k:.4f} ms")
+            best = min((r for r in rows if r["samples"] == N and r["snps"] == m),
+                       key=lambda r: r["kernel_ms"])
+            log(f"[bench] grid N={N} m={m}: fastest {best['engine']}")
+    return rows
+
+
 def config1_bench(args, steps):
     """Configuration 1: 2,000 SNPs x 16,622 samples, all-pairs r^2 (lower
     triangle) on the GPU -- the plain single-region call -- and on the CPU
@@ -529,6 +567,7 @@ def serving_bench(args):
         sizes = np.minimum(sizes, [tabs[w][1] for w in which])
         starts = [int(rng.integers(0, tabs[w][1] - z + 1)) for w, z in zip(which, sizes)]
         ids = []
+        b0 = srv.stats()["batches"]
         t0 = time.perf_counter()
         due = t0
         for k in range(nreq):
@@ -542,6 +581,7 @@ def serving_bench(args):
             lat.append(srv.wait(i, 120000.0))
             srv.release(i)
         span = time.perf_counter() - t0
+        nb = srv.stats()["batches"] - b0
         lat = np.array(lat)
         pairs = int((sizes * (sizes - 1) // 2).sum())
         pairs16 = int(sum(z * (z - 1) // 2 for w, z in zip(which, sizes) if tabs[w][2] == 16622))
@@ -551,9 +591,10 @@ def serving_bench(args):
                "p99_ms": round(float(np.percentile(lat, 99)), 3),
                "mean_ms": round(float(lat.mean()), 3),
                "gpairs_s": round(pairs / span / 1e9, 3),
-               "gpairs_s_16622": round(pairs16 / span / 1e9, 3)}
+               "gpairs_s_16622": round(pairs16 / span / 1e9, 3),
+               "batches": nb, "requests_per_batch": round(nreq / max(nb, 1), 2)}
         log(f"[bench] serving {rate:g} req/s: p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, "
-            f"{res['achieved_req_s']} req/s")
+            f"{res['achieved_req_s']} req/s, {res['requests_per_batch']} requests per batch")
         results.append(res)
     st = srv.stats()
     srv.close()
@@ -595,7 +636,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size "
                          "kernel for calibrating HBM traffic counters)")
-    ap.add_argument("--serving-rates", default="100,1000,5000")
+    # the configuration's sweep (100 to 5,000 requests/s) and one rate past it,
+    # where requests queue and batches grow
+    ap.add_argument("--serving-rates", default="100,200,500,1000,2000,5000,20000")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--serving-rows16", type=int, default=100000)
     ap.add_argument("--serving-rows200", type=int, default=20000)
@@ -607,6 +650,8 @@ def main():
                     help="window plans: work schedule with --win-cutoff")
     ap.add_argument("--no-graph", action="store_true",
                     help="window plans: direct launches, no HIP graph capture")
+    ap.add_argument("--engine-grid", action="store_true",
+                    help="only time every engine on a grid of shapes and print that table")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -616,6 +661,10 @@ def main():
         sys.exit(2)
 
     import qld_amd
+
+    if args.engine_grid:
+        print(json.dumps({"engine_grid": engine_grid(args)}), flush=True)
+        return
 
     N, m = args.samples, args.snps
     log(f"[bench] generating {m} SNPs x {N} samples (seed {args.seed})")

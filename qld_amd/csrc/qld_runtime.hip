@@ -352,16 +352,19 @@ static int check_job(qld_ctx *ctx, const LdJob &j)
 
 int qld_check_job(qld_ctx *ctx, const LdJob &j) { return check_job(ctx, j); }
 
-// AUTO policy from measurements on MI355X (30,000 SNPs x 16,622 samples,
-// exact r^2: popcount 17.0 ms, int8 MFMA 7.7 ms, fp4 MFMA 2.16 ms per
-// triangle): the fp4 engine wins once a launch has more than a handful of
-// tiles; tiny problems stay on the popcount engine, which needs neither the
-// row-tiling pass nor a tile list.
+// AUTO policy from a grid of kernel times on MI355X (single-region
+// triangles of 64 to 30,000 SNPs at 256 to 100,000 samples, every engine,
+// exact r^2): the fp4 engine is fastest almost everywhere -- at 16,622
+// samples 5x faster than the popcount engine already at 64 SNPs, whose tile
+// latency grows with the row width -- except on narrow rows (at most 16
+// words, N <= 1,024) with 0.5 to 8 million pairs, where the popcount engine
+// is about 10% faster (2,000 SNPs at 256 / 1,000 samples: 0.025 vs 0.028 ms).
+// The int8 engine is never the fastest.
 int qld_pick_engine(const qld_ctx *ctx, const LdJob &j)
 {
     if (ctx->cfg.engine != QLD_ENGINE_AUTO) return ctx->cfg.engine;
     const double pairs = j.tri ? 0.5 * (double)j.m * (double)j.n : (double)j.m * (double)j.n;
-    if (pairs < 64.0 * 1024.0) return QLD_ENGINE_POPCOUNT;
+    if (j.words <= 16 && pairs >= 5e5 && pairs <= 8e6) return QLD_ENGINE_POPCOUNT;
     return QLD_ENGINE_MFMA_FP4;
 }
 

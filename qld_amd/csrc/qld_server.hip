@@ -133,17 +133,16 @@ static int slot_reserve(qld_server *s, Slot &sl, size_t bytes)
     return QLD_OK;
 }
 
-// Launch one batch on slot `k` (lock held by the caller for the request map).
-static int launch_batch(qld_server *s, Slot &sl)
+// Launch one batch on slot `sl`: `batch` holds copies of its requests, taken
+// under the server lock, so this runs without it (the HIP calls must not
+// stall submit, wait or release); only the dispatcher thread touches slots.
+static int launch_batch(qld_server *s, Slot &sl, const std::vector<Request> &batch)
 {
     qld_ctx *ctx = s->ctx;
     const int T = QLD_TILE_SMALL;
     size_t nitems = 0;
-    for (int64_t id : sl.reqs) {
-        const Request &r = s->reqs[id];
-        nitems += job_tile_count(r.size, r.size, 1, T);
-    }
-    const size_t nd = sl.reqs.size();
+    for (const Request &r : batch) nitems += job_tile_count(r.size, r.size, 1, T);
+    const size_t nd = batch.size();
     const size_t desc_bytes = (nd * sizeof(WorkDesc) + 255) / 256 * 256;
     const size_t bytes = desc_bytes + nitems * sizeof(WorkItem);
     int rc = slot_reserve(s, sl, bytes);
@@ -158,7 +157,7 @@ static int launch_batch(qld_server *s, Slot &sl)
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1) split = cnt;
         for (size_t k = 0; k < nd; ++k) {
-            Request &r = s->reqs[sl.reqs[k]];
+            const Request &r = batch[k];
             const qld_table *t = r.t;
             if (t->direct != (pass == 0)) continue;
             engine = t->engine;
@@ -175,7 +174,6 @@ static int launch_batch(qld_server *s, Slot &sl)
                                t->snp_size, out, r.size, nullptr, r.size);
             }
             append_job_tiles(hi, &cnt, (uint32_t)k, r.size, r.size, 1, T);
-            s->pairs += (int64_t)r.size * (r.size - 1) / 2;
         }
     }
     hipError_t e = hipMemcpyAsync(sl.d_stage, sl.h_stage, bytes, hipMemcpyHostToDevice,
@@ -191,8 +189,7 @@ static int launch_batch(qld_server *s, Slot &sl)
         e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, 0, (const WorkDesc *)sl.d_stage, none,
                               d_items + split, (int64_t)(cnt - split), 0, nullptr, nullptr,
                               sl.stream);
-    for (int64_t id : sl.reqs) {
-        Request &r = s->reqs[id];
+    for (const Request &r : batch) {
         if (e == hipSuccess && (r.flags & QLD_REQ_COPY) && r.host_out)
             e = hipMemcpyAsync(r.host_out, s->arena_mem + r.out_off,
                                (size_t)r.size * r.size * sizeof(float),
@@ -200,8 +197,6 @@ static int launch_batch(qld_server *s, Slot &sl)
     }
     if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
     if (e != hipSuccess) return qld_hip_fail(ctx, e, "serving batch");
-    s->batches++;
-    sl.busy = true;
     return QLD_OK;
 }
 
@@ -247,7 +242,21 @@ static void dispatcher(qld_server *s)
                 s->pending.pop_front();
             }
             if (sl.reqs.empty()) break;
-            if (launch_batch(s, sl) != QLD_OK) {
+            std::vector<Request> batch;
+            batch.reserve(sl.reqs.size());
+            int64_t batch_pairs = 0;
+            for (int64_t id : sl.reqs) {
+                batch.push_back(s->reqs[id]);
+                batch_pairs += (int64_t)batch.back().size * (batch.back().size - 1) / 2;
+            }
+            lk.unlock();
+            const int launched = launch_batch(s, sl, batch);
+            lk.lock();
+            if (launched == QLD_OK) {
+                sl.busy = true;
+                s->batches++;
+                s->pairs += batch_pairs;
+            } else {
                 for (int64_t id : sl.reqs) {
                     Request &r = s->reqs[id];
                     r.state = FAILED;

@@ -62,13 +62,14 @@ def _case(eng, A, bcA, B, bcB, N, tri, counts):
 
 @pytest.mark.parametrize("mode", ["reference", "exact"])
 def test_noalloc_triangles_match_correlate_dev(mode, engine_factory):
-    """12,000 rows (256-tiles), then 3,000 (128-tiles), then 200 (popcount
-    under AUTO), each identical to qld_correlate_dev."""
+    """12,000 rows (256-tiles), then 3,000 (128-tiles) at 16,622 samples,
+    then 2,000 rows of 16 words (the popcount engine under AUTO), each
+    identical to qld_correlate_dev."""
     eng = engine_factory(engine="auto", r2_mode=mode)
-    N = 16622
-    for rows, counts, engine in ((12000, False, "mfma_fp4"), (3000, True, "mfma_fp4"),
-                                 (200, True, "popcount")):
-        say("noalloc triangle", mode, rows)
+    for rows, N, counts, engine in ((12000, 16622, False, "mfma_fp4"),
+                                    (3000, 16622, True, "mfma_fp4"),
+                                    (2000, 1000, True, "popcount")):
+        say("noalloc triangle", mode, rows, N)
         A, bc = qld_amd.synth_blocks(20261203 + rows, rows, N)
         assert _case(eng, A, bc, A, bc, N, True, counts) == engine
 
