@@ -44,12 +44,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(engine, r2_mode, snps, samples, operands="direct"):
+def measured_traffic(workload, r2_mode):
     """HBM bytes per launch of the contraction kernel for this workload, read
-    from the committed rocprofv3 PMC summary (FETCH_SIZE x2 + WRITE_SIZE x1,
-    scaled by a copy kernel of known size); None when no pass covers it."""
-    key = (f"engine={engine} operands={operands} mode={r2_mode} snps={snps} "
-           f"samples={samples} ")
+    from the committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE in
+    KiB per dispatch, each scaled by what it reads for a copy kernel of known
+    size); None when no pass covers it."""
+    key = f"workload={workload} mode={r2_mode} "
     try:
         for path in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
             with open(os.path.join(ROOT, "profiles", path)) as f:
@@ -182,7 +182,7 @@ def engine_grid(args):
                 dA = eng.alloc(A.nbytes).upload(A)
                 dbc = eng.alloc(bc.nbytes).upload(bc)
                 dres = eng.alloc(m * m * 4)
-                for _ in range(2):
+                for _ in range(10):     # clocks ramp over the first launches
                     eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
                 ks = []
                 for _ in range(3):
@@ -215,13 +215,14 @@ def config1_bench(args, steps):
     dA = eng.alloc(A.nbytes).upload(A)
     dbc = eng.alloc(bc.nbytes).upload(bc)
     dres = eng.alloc(m * m * 4)
-    for _ in range(5):
+    for _ in range(200):      # tiny launches: many to ramp the clocks
         eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
     eng.sync()
+    reps = max(steps, 100)
     eng.timer_start()
-    for _ in range(steps):
+    for _ in range(reps):
         eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
-    ms = eng.timer_stop() / steps
+    ms = eng.timer_stop() / reps
     engine = eng.last_engine
     for b in (dA, dbc, dres):
         b.free()
@@ -282,8 +283,11 @@ def windows_bench(args, steps):
            "windows": nwin, "pairs": pairs, "tile_cutoff": cutoff, "schedule": sched}
 
     def timed(plan, use_graph, k):
+        # warm-up first: the GPU's clocks ramp over the first launches after
+        # any idle gap (about ten launches of milliseconds each)
         plan.use_graph(use_graph)
-        plan.run()
+        for _ in range(10):
+            plan.run()
         eng.sync()
         eng.timer_start()
         for _ in range(k):
@@ -294,29 +298,41 @@ def windows_bench(args, steps):
     batched = eng.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
     ms_b = timed(batched, graphs, steps)
     st = batched.stats()
+    kms = []
+    for _ in range(3):
+        batched.run()
+        kms.append(eng.last_contract_ms())
+    k_s = float(np.median(kms)) / 1e3
     batched.free()
     # The baseline, one launch per window, on 128-tiles: a single window's
     # launch needs the finer tiles to fill the GPU (its fastest setting).
-    eng.set_tile_cutoff(1 << 30)
-    plain = eng.plan_windows(t, wins, out, qld_amd.BATCH_PLAIN)
-    ms_pg = timed(plain, True, max(2, steps // 3)) if graphs else None
-    ms_pd = timed(plain, False, max(2, steps // 3))
-    plain.free()
+    ms_pg = ms_pd = None
+    if not args.no_plain:
+        eng.set_tile_cutoff(1 << 30)
+        plain = eng.plan_windows(t, wins, out, qld_amd.BATCH_PLAIN)
+        ms_pg = timed(plain, True, max(2, steps // 3)) if graphs else None
+        ms_pd = timed(plain, False, max(2, steps // 3))
+        plain.free()
     t.free()
     out.free()
     eng.close()
     res.update({"batched_ms": round(ms_b, 3), "batched_gpairs_s": round(pairs / ms_b / 1e6, 2),
                 "batched_launches": st["launches"], "tiles": st["tiles"],
                 "graphs": graphs,
-                "plain_direct_ms": round(ms_pd, 3),
-                "plain_direct_gpairs_s": round(pairs / ms_pd / 1e6, 2),
-                "speedup_vs_plain": round(ms_pd / ms_b, 2)})
-    if graphs:
+                "roofline": roofline("mfma_fp4", pairs, (N + 63) // 64, N, k_s,
+                                     measured_traffic(f"windows{nwin}", args.r2_mode)
+                                     if (nwin, rows, N) == (4096, 500000, 16622) else None,
+                                     rows=int(sizes.sum()))})
+    if ms_pd is not None:
+        res.update({"plain_direct_ms": round(ms_pd, 3),
+                    "plain_direct_gpairs_s": round(pairs / ms_pd / 1e6, 2),
+                    "speedup_vs_plain": round(ms_pd / ms_b, 2)})
+    if ms_pg is not None:
         res.update({"plain_graph_ms": round(ms_pg, 3),
                     "plain_graph_gpairs_s": round(pairs / ms_pg / 1e6, 2),
                     "speedup_vs_plain_graph": round(ms_pg / ms_b, 2)})
     log(f"[bench] windows: batched {ms_b:.3f} ms, plain graph {ms_pg} ms, "
-        f"plain direct {ms_pd:.3f} ms (cutoff {cutoff}, schedule {sched}, graphs {graphs})")
+        f"plain direct {ms_pd} ms (cutoff {cutoff}, schedule {sched}, graphs {graphs})")
     return res
 
 
@@ -343,7 +359,8 @@ def rect_bench(args, steps):
 
     def step():
         eng.correlate_dev(dA, dbA, m, dB, dbB, n, W, N, False, dres, m)
-    step()
+    for _ in range(10):       # clocks ramp over the first launches
+        step()
     eng.sync()
     eng.timer_start()
     for _ in range(steps):
@@ -363,7 +380,8 @@ def rect_bench(args, steps):
     return {"workload": f"region A {m} SNPs x region B {n} SNPs over {N} samples, "
                         "full rectangle FP32 r2", "pairs": pairs, "engine": engine,
             "ms_per_step": round(ms, 3), "gpairs_s": round(pairs / ms / 1e6, 2),
-            "roofline": roofline(engine, pairs, W, N, k_s)}
+            "roofline": roofline(engine, pairs, W, N, k_s,
+                                 measured_traffic("rect100k", args.r2_mode), rows=m + n)}
 
 
 def report_bench(args, A, bc, N):
@@ -650,6 +668,8 @@ def main():
                     help="window plans: work schedule with --win-cutoff")
     ap.add_argument("--no-graph", action="store_true",
                     help="window plans: direct launches, no HIP graph capture")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="window plans: skip the one-launch-per-window baseline")
     ap.add_argument("--engine-grid", action="store_true",
                     help="only time every engine on a grid of shapes and print that table")
     args = ap.parse_args()
@@ -722,7 +742,8 @@ def main():
     dA2 = eng2.alloc(A.nbytes).upload(A)
     dbc2 = eng2.alloc(bc.nbytes).upload(bc)
     dres2 = eng2.alloc(m * m * 4)
-    eng2.correlate_dev(dA2, dbc2, m, dA2, dbc2, m, W, N, True, dres2, m)
+    for _ in range(args.warmup):
+        eng2.correlate_dev(dA2, dbc2, m, dA2, dbc2, m, W, N, True, dres2, m)
     eng2.sync()
     eng2.timer_start()
     for _ in range(args.steps):
@@ -747,8 +768,9 @@ def main():
         f"{other}_mode": {"ms_per_step": round(other_ms, 4),
                           "gpairs_s": round(pairs / other_ms / 1e6, 3)},
         "roofline": roofline(engine, pairs, W, N, kernel_s,
-                             measured_traffic(engine, args.r2_mode, m, N,
-                                              args.fp4_operands or "direct"), rows=m),
+                             measured_traffic(f"single{m // 1000}k", args.r2_mode)
+                             if (m, N) == (30000, 16622) and not args.fp4_operands else None,
+                             rows=m),
     }
     if args.copy_probe:
         nb = args.copy_probe << 20
