@@ -164,6 +164,35 @@ def cpu_baseline(A, bc, N, budget_s=10.0):
             "calibration_vs_blis": calib}
 
 
+def r2_error_vs_fp64(eng, dA, dbc, m, W, N, rows):
+    """Max and mean |r2 - r2_fp64| over the sampled rows of one triangle:
+    the engine's counts (exact integers) and r2 for those rows, against the
+    FP64 closed form r2 = (c N - a b)^2 / (a (N - a) b (N - b)) evaluated in
+    numpy on the same counts."""
+    dres = eng.alloc(m * m * 4)
+    dcnt = eng.alloc(m * m * 4)
+    eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m, dcnt, m)
+    eng.sync()
+    bc = dbc.download(m, np.uint32).astype(np.float64)
+    errs = []
+    for r in rows:
+        j = np.arange(int(r) + 1, m)
+        if j.size == 0:
+            continue
+        got = dres.download_at(int(r) * m * 4, m, np.float32)[j].astype(np.float64)
+        c = dcnt.download_at(int(r) * m * 4, m, np.uint32)[j].astype(np.float64)
+        a, b = bc[j], bc[int(r)]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            want = (c * N - a * b) ** 2 / (a * (N - a) * b * (N - b))
+        ok = np.isfinite(want) & (a > 0) & (b > 0)
+        errs.append(np.abs(got[ok] - want[ok]))
+    dres.free()
+    dcnt.free()
+    e = np.concatenate(errs)
+    return {"max": float(e.max()), "mean": float(e.mean()), "cells": int(e.size),
+            "rows": len(rows)}
+
+
 def engine_grid(args):
     """Every contraction engine (popcount VALU, int8 MFMA, fp4 MFMA with
     direct operands) on single-region triangles of 64 to 30,000 SNPs at 256
@@ -749,6 +778,8 @@ def main():
     for _ in range(args.steps):
         eng2.correlate_dev(dA2, dbc2, m, dA2, dbc2, m, W, N, True, dres2, m)
     other_ms = eng2.timer_stop() / args.steps
+    sample_rows = np.sort(np.random.default_rng(64).choice(m, min(64, m), replace=False))
+    err_other = r2_error_vs_fp64(eng2, dA2, dbc2, m, W, N, sample_rows)
     for b in (dA2, dbc2, dres2):
         b.free()
     eng2.close()
@@ -767,6 +798,8 @@ def main():
         "device_ms_per_step": round(dev_ms / args.steps, 4),
         f"{other}_mode": {"ms_per_step": round(other_ms, 4),
                           "gpairs_s": round(pairs / other_ms / 1e6, 3)},
+        "r2_error_vs_fp64": {args.r2_mode: r2_error_vs_fp64(eng, dA, dbc, m, W, N, sample_rows),
+                             other: err_other},
         "roofline": roofline(engine, pairs, W, N, kernel_s,
                              measured_traffic(f"single{m // 1000}k", args.r2_mode)
                              if (m, N) == (30000, 16622) and not args.fp4_operands else None,

@@ -693,3 +693,36 @@ def test_mdf_reader_many_files_asan(tmp_path):
         outs[name] = r.stdout.splitlines()
     assert outs["plain"] == outs["asan"]
     assert outs["plain"][0].startswith("all 600 ")
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/src/correlator"),
+                    reason="needs the reference checkout (build container only)")
+def test_reference_gpu_binding_builds_and_keeps_cpu_path(tmp_path):
+    """The binding a maintainer adds to the reference (oracle/ref_gpu_binding.c:
+    gpu_init / gpu_release / correlate_gpu over the engine's C ABI, and the
+    per-task dispatch of the CPU task driver) compiles and links against the
+    reference's own correlator sources with no OpenCL left; without -gpu the
+    bound command still writes the reference's golden report (the CPU
+    threads keep correlate()), and with -gpu it reaches the engine's init."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref-gpu"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    exe = os.path.join(ROOT, "oracle", "_ref", "qLD-compute-gpu")
+    syms = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True,
+                          text=True).stdout
+    names = {ln.split()[-1].split("@")[0] for ln in syms.splitlines() if ln.strip()}
+    assert {"qld_correlate_report", "qld_gpu_init", "qld_gpu_release"} <= names
+    assert not any(n.startswith("cl") and n[2:3].isupper() for n in names)   # no OpenCL
+    os.symlink(os.path.join(CLI_DIR, "data"), tmp_path / "data")
+    os.makedirs(tmp_path / "out")
+    args = MANIFEST["cases"]["single_win"]
+    r = subprocess.run([exe] + args + ["-mdf", "-threads", "1", "-output", "out/report.txt"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = open(os.path.join(CLI_DIR, "expected", "single_win", "report.txt"), "rb").read()
+    assert open(tmp_path / "out" / "report.txt", "rb").read() == want
+    r = subprocess.run([exe] + args + ["-mdf", "-threads", "1", "-gpu", "-output",
+                                       "out/report_gpu.txt"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:       # no GPU in this container: the engine's init says so
+        assert "GPU init failed" in r.stderr, r.stderr
