@@ -227,6 +227,58 @@ __device__ __forceinline__ T wave_uniform(const T &v)
     return out;
 }
 
+// Element idx of an array that no lane writes while the kernel runs (work
+// items, job descriptors), read through the constant address space: with a
+// wave-uniform index the loads are scalar (s_load, counted by lgkmcnt), so
+// they never wait behind the vector-memory counter -- which the r^2 stores
+// of the unit before keep busy.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T *base, int64_t idx)
+{
+    static_assert(sizeof(T) % 4 == 0, "dword-sized");
+    typedef const __attribute__((address_space(4))) uint32_t cu32;
+    const cu32 *src = (const cu32 *)(uintptr_t)(base + idx);
+    T out;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+    return out;
+}
+
+// Claim the next unit of queue q: the old head, incremented.  An atomic
+// increment, not an add: the compiler rewrites a lane's atomic add into a
+// wave-wide form whose result it broadcasts at once -- a wait, right where
+// the claim is issued, for every memory operation pending before it, the
+// r^2 stores included -- and leaves increments alone.
+__device__ __forceinline__ uint32_t claim_head(uint32_t *q)
+{
+    return __builtin_amdgcn_atomic_inc32(q, 0xFFFFFFFFu, __ATOMIC_RELAXED, "agent");
+}
+
+// What a claim loop does inside a unit: begin() once the k loop's first
+// operands have arrived (the k loop drains every memory operation there, the
+// unit before's r^2 stores among them, so a claim issued then waits behind
+// nothing), end() after the k loop, before the epilogue's stores (where the
+// claim, its result long back, is read without a wait).
+struct NoHook {
+    __device__ __forceinline__ void begin() const {}
+    __device__ __forceinline__ void end() const {}
+};
+
+template <class B, class E>
+struct Hooks {
+    B b;
+    E e;
+    __device__ __forceinline__ void begin() const { b(); }
+    __device__ __forceinline__ void end() const { e(); }
+};
+
+template <class B, class E>
+__device__ __forceinline__ Hooks<B, E> make_hooks(const B &b, const E &e)
+{
+    return Hooks<B, E>{b, e};
+}
+
 template <int N>
 __device__ __forceinline__ void wait_vm()
 {
@@ -547,9 +599,11 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 // hold rows / columns of the job (a ragged edge): the others are neither
 // loaded nor computed.
 #define QLD_NO_DIAG 99
-template <class G, int DOFF = QLD_NO_DIAG, int MA = G::NBM, int NA = G::NBN>
+template <class G, int DOFF = QLD_NO_DIAG, int MA = G::NBM, int NA = G::NBN,
+          class Hook = NoHook>
 __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, int wm, int wn,
-                                           v16f (&acc)[G::NBM][G::NBN])
+                                           v16f (&acc)[G::NBM][G::NBN],
+                                           const Hook &hook = Hook())
 {
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN;
     const int lane = threadIdx.x & 63;
@@ -632,6 +686,7 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     // for everything (vmcnt 0) at every k step of the loop instead of for
     // the older group only.
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+    hook.begin();
     int g = 0;
     for (; g + 2 <= full; g += 2) {
         // Scheduling barriers keep each group's loads a whole compute ahead
@@ -753,9 +808,9 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
 // expanded in the kernel), then the fused epilogue.
 // wm, wn: the wave's block of the tile; wave: the wave's LDS epilogue
 // region.  Cooperative k loops (LDS ring) need wm, wn = this wave's own.
-template <class OP, int MODE, class G, int PACKED, bool WAVE_UNITS = false>
+template <class OP, int MODE, class G, int PACKED, bool WAVE_UNITS = false, class Hook = NoHook>
 __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int wm, int wn,
-                                          int wave, uint8_t *smem)
+                                          int wave, uint8_t *smem, const Hook &hook = Hook())
 {
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, TILE = G::EDGE;
     constexpr int RPL = G::RPL;                         // epilogue rows per lane
@@ -768,8 +823,11 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
     // share a k loop with the others, so it simply skips the tile.
     if constexpr (PACKED == 2)
         if ((d.tri && j0 + wn * WTN + WTN <= i0 + wm * WTM + 1) || i0 + wm * WTM >= d.ni ||
-            j0 + wn * WTN >= d.nj)
+            j0 + wn * WTN >= d.nj) {
+            hook.begin();
+            hook.end();
             return;
+        }
 
     // Bit counts of this lane's epilogue rows/columns, fetched now so their
     // latency hides under the k loop.
@@ -803,15 +861,15 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
             const bool mfull = a0 + 32 < d.ni, nfull = b0 + 32 < d.nj;
             if (mfull && nfull) {
                 if (d.tri && a0 == b0)
-                    kloop_regs<G, 0>(d, i0, j0, wm, wn, acc);
+                    kloop_regs<G, 0, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook);
                 else
-                    kloop_regs<G>(d, i0, j0, wm, wn, acc);
+                    kloop_regs<G, QLD_NO_DIAG, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook);
             } else if (mfull) {
-                kloop_regs<G, QLD_NO_DIAG, 2, 1>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, QLD_NO_DIAG, 2, 1, Hook>(d, i0, j0, wm, wn, acc, hook);
             } else if (nfull) {
-                kloop_regs<G, QLD_NO_DIAG, 1, 2>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, QLD_NO_DIAG, 1, 2, Hook>(d, i0, j0, wm, wn, acc, hook);
             } else {
-                kloop_regs<G, QLD_NO_DIAG, 1, 1>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, QLD_NO_DIAG, 1, 1, Hook>(d, i0, j0, wm, wn, acc, hook);
             }
         } else if constexpr (WAVE_UNITS && WTM == 128 && WTN == 64) {
             // 128 x 64 wave blocks of 256-tiles: a block whose columns start
@@ -824,30 +882,33 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
             // of plans carry the variant: in a single region diagonal blocks
             // are rare and the extra code would only grow the kernel.
             if (d.tri && b0 == a0)
-                kloop_regs<G, 0, 2, 2>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, 0, 2, 2, Hook>(d, i0, j0, wm, wn, acc, hook);
             else
-                kloop_regs<G>(d, i0, j0, wm, wn, acc);
+                kloop_regs<G, QLD_NO_DIAG, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook);
         } else {
-            kloop_regs<G>(d, i0, j0, wm, wn, acc);
+            kloop_regs<G, QLD_NO_DIAG, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook);
         }
     } else {
+        hook.begin();
         if constexpr (PACKED == 1) kloop_packed<G>(d, i0, j0, acc, smem);
         else kloop_dma<OP, G>(d, i0, j0, acc, smem);
         lds_barrier();   // every wave is done with the ring before the epilogue reuses it
     }
+    hook.end();
     tile_epilogue<OP, MODE, G>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
 }
 
-template <class OP, int MODE, class G, int PACKED>
+template <class OP, int MODE, class G, int PACKED, class Hook = NoHook>
 __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, int wm, int wn, int wave,
-                                         uint8_t *smem)
+                                         uint8_t *smem, const Hook &hook = Hook())
 {
-    const WorkItem it = wave_uniform(p.items[idx]);
+    const WorkItem it = load_uniform(p.items, idx);
     if (p.descs) {
-        const WorkDesc d = wave_uniform(p.descs[it.desc]);
-        mfma_tile<OP, MODE, G, PACKED>(d, it.ti, it.tj, wm, wn, wave, smem);
+        const WorkDesc d = load_uniform(p.descs, it.desc);
+        mfma_tile<OP, MODE, G, PACKED, false, Hook>(d, it.ti, it.tj, wm, wn, wave, smem, hook);
     } else {
-        mfma_tile<OP, MODE, G, PACKED>(p.inline_desc, it.ti, it.tj, wm, wn, wave, smem);
+        mfma_tile<OP, MODE, G, PACKED, false, Hook>(p.inline_desc, it.ti, it.tj, wm, wn, wave,
+                                                    smem, hook);
     }
 }
 
@@ -868,27 +929,39 @@ __device__ __forceinline__ void run_wave_units(const KArgs &p, int wave, uint8_t
 {
     constexpr int BM = G::EDGE / G::WTM, BN = G::EDGE / G::WTN;   // wave blocks per tile side
     const int lane = threadIdx.x & 63;
-    const int x0 = (int)(blockIdx.x % QLD_NUM_XCD);
-    for (int q = 0; q < QLD_NUM_XCD; ++q) {
-        const int x = (x0 + q) % QLD_NUM_XCD;
-        const int64_t begin = p.qbeg[x], end = p.qbeg[x + 1];
-        for (;;) {
-            uint32_t k = 0;
-            if (lane == 0) k = atomicAdd(&p.queue[x], 1u);
-            k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
-            const int64_t idx = begin + (int64_t)k;
-            if (idx >= end) break;
-            const WorkItem it = wave_uniform(p.items[idx]);
-            const int bi = it.ti, bj = it.tj;
-            if (p.descs) {
-                const WorkDesc d = wave_uniform(p.descs[it.desc]);
-                mfma_tile<OP, MODE, G, PACKED, true>(d, bi / BM, bj / BN, bi % BM, bj % BN,
-                                                     wave, smem);
-            } else {
-                mfma_tile<OP, MODE, G, PACKED, true>(p.inline_desc, bi / BM, bj / BN, bi % BM,
-                                                     bj % BN, wave, smem);
-            }
+    int x = (int)(blockIdx.x % QLD_NUM_XCD);
+    int left = QLD_NUM_XCD;                  // queues this wave has not found empty
+    // Claims run one unit ahead: the claim of the next unit is issued once
+    // this unit's k loop has its first operands (nothing pending then) and
+    // read after the k loop, so no claim waits behind r^2 stores.
+    auto issue = [&]() -> uint32_t {
+        uint32_t k = 0;
+        if (lane == 0) k = claim_head(&p.queue[x]);
+        return k;
+    };
+    int64_t k = (uint32_t)__builtin_amdgcn_readfirstlane((int)issue());
+    for (;;) {
+        int64_t idx = p.qbeg[x] + k;
+        while (idx >= p.qbeg[x + 1]) {       // queue x drained: steal from the next
+            if (--left == 0) return;
+            x = (x + 1) % QLD_NUM_XCD;
+            idx = p.qbeg[x] + (uint32_t)__builtin_amdgcn_readfirstlane((int)issue());
         }
+        const WorkItem it = load_uniform(p.items, idx);
+        uint32_t pend = 0;
+        int next = 0;
+        const auto ahead = make_hooks([&]() { pend = issue(); },
+                                      [&]() { next = __builtin_amdgcn_readfirstlane((int)pend); });
+        const int bi = it.ti, bj = it.tj;
+        if (p.descs) {
+            const WorkDesc d = load_uniform(p.descs, it.desc);
+            mfma_tile<OP, MODE, G, PACKED, true>(d, bi / BM, bj / BN, bi % BM, bj % BN, wave,
+                                                 smem, ahead);
+        } else {
+            mfma_tile<OP, MODE, G, PACKED, true>(p.inline_desc, bi / BM, bj / BN, bi % BM,
+                                                 bj % BN, wave, smem, ahead);
+        }
+        k = (uint32_t)next;
     }
 }
 
@@ -909,28 +982,54 @@ __global__ __launch_bounds__(G::THREADS, G::SPLIT) void ld_mfma_kernel(KArgs p)
                                       wn, wave, smem);
         return;
     }
-    __shared__ int64_t claimed;
-    const int x0 = (int)(blockIdx.x % QLD_NUM_XCD);
-    for (int q = 0; q < QLD_NUM_XCD; ++q) {
-        const int x = (x0 + q) % QLD_NUM_XCD;
-        const int64_t begin = p.qbeg[x] * G::SPLIT;
-        const int64_t end = p.qbeg[x + 1] * G::SPLIT;
+    // Thread 0 claims for the workgroup, one unit ahead: the next unit's
+    // claim is issued before this unit's k loop and published in LDS after
+    // it, so no claim waits behind the r^2 stores of the unit before.  The
+    // two slots alternate, so the barrier that ends a unit is the only one.
+    // claimed_x[s] = -1 once every queue is drained.
+    __shared__ uint32_t claimed[2];
+    __shared__ int claimed_x[2];
+    int x = (int)(blockIdx.x % QLD_NUM_XCD);
+    int left = QLD_NUM_XCD;                  // queues thread 0 has not found empty
+    const bool leader = threadIdx.x == 0;
+    // thread 0: publish head k of queue x in slot s, or move on to the next
+    // queue (a blocking claim there) if queue x is drained
+    auto settle = [&](int s, uint32_t k) {
         for (;;) {
-            if (threadIdx.x == 0) {
-                int64_t k = (int64_t)atomicAdd(&p.queue[x], 1u);
-                claimed = begin + k < end ? begin + k : -1;
+            if ((int64_t)k < (p.qbeg[x + 1] - p.qbeg[x]) * G::SPLIT) {
+                claimed[s] = k;
+                claimed_x[s] = x;
+                return;
             }
-            __syncthreads();
-            const int64_t c = claimed;      // the same in every lane: make it scalar
-            const int64_t idx =
-                (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(c >> 32)) << 32) |
-                          (uint32_t)__builtin_amdgcn_readfirstlane((int)c));
-            __syncthreads();
-            if (idx < 0) break;
-            run_item<OP, MODE, G, PACKED>(p, idx / G::SPLIT,
-                                          (int)(idx % G::SPLIT) * G::WAVES_M + wr, wn, wave,
-                                          smem);
+            if (--left == 0) {
+                claimed_x[s] = -1;
+                return;
+            }
+            x = (x + 1) % QLD_NUM_XCD;
+            k = claim_head(&p.queue[x]);
         }
+    };
+    if (leader) settle(0, claim_head(&p.queue[x]));
+    __syncthreads();
+    for (int slot = 0;; slot ^= 1) {
+        const int cx = __builtin_amdgcn_readfirstlane(claimed_x[slot]);
+        if (cx < 0) break;
+        const int64_t u = p.qbeg[cx] * G::SPLIT +
+                          (uint32_t)__builtin_amdgcn_readfirstlane((int)claimed[slot]);
+        uint32_t pend = 0;
+        const auto ahead = make_hooks(
+            [&]() {
+                if (leader) pend = claim_head(&p.queue[x]);
+            },
+            [&]() {
+                // read in every wave, not only under the leader's branch:
+                // the compiler then knows the claim complete on every path
+                const uint32_t k = (uint32_t)__builtin_amdgcn_readfirstlane((int)pend);
+                if (leader) settle(slot ^ 1, k);
+            });
+        run_item<OP, MODE, G, PACKED>(p, u / G::SPLIT, (int)(u % G::SPLIT) * G::WAVES_M + wr,
+                                      wn, wave, smem, ahead);
+        __syncthreads();
     }
 }
 
