@@ -71,12 +71,15 @@ __host__ __device__ inline int64_t job_tile_index(int ti, int tj, int ti_count, 
 }
 
 // Wave blocks of one job (WTM x WTN blocks inside tiles of edge T) that hold
-// at least one cell to compute, tile by tile in append_job_tiles' order and
-// row by row inside a tile, as units (desc, bi, bj): block rows
-// [bi*WTM, bi*WTM + WTM), columns [bj*WTN, bj*WTN + WTN).  A block is left
-// out when it has no row or no column inside the job, or lies wholly below
-// a triangle's diagonal (every column <= every row).  Returns the units
-// written; `out` needs room for job_tile_count(...) * (T/WTM) * (T/WTN).
+// at least one cell to compute, as units (desc, bi, bj): block rows
+// [bi*WTM, bi*WTM + WTM), columns [bj*WTN, bj*WTN + WTN).  Super-tiles come
+// in append_job_tiles' order; inside one, row band by row band (WTM rows),
+// each band left to right across the super-tile's tiles, so that runs of
+// consecutive units share their rows -- the kernels hand a workgroup's
+// waves such runs together -- however many blocks a band has.  A block is
+// left out when it has no row or no column inside the job, or no cell of a
+// triangle's (no column inside the job right of any of its rows).  Returns the
+// units written; `out` needs room for job_tile_count(...) * (T/WTM) * (T/WTN).
 inline size_t append_job_wave_units(WorkItem *out, uint32_t desc, int ni, int nj, int tri,
                                     int T, int WTM, int WTN)
 {
@@ -88,18 +91,22 @@ inline size_t append_job_wave_units(WorkItem *out, uint32_t desc, int ni, int nj
     for (int J = 0; J < sj; ++J)
         for (int I = 0; I < si; ++I) {
             if (tri && I > J) continue;
-            for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj)
-                for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti) {
-                    if (tri && ti > tj) continue;
-                    for (int wm = 0; wm < BM; ++wm)
+            for (int ti = I * S; ti < (I + 1) * S && ti < ti_count; ++ti)
+                for (int wm = 0; wm < BM; ++wm)
+                    for (int tj = J * S; tj < (J + 1) * S && tj < tj_count; ++tj) {
+                        if (tri && ti > tj) continue;
                         for (int wn = 0; wn < BN; ++wn) {
                             const int a0 = ti * T + wm * WTM, b0 = tj * T + wn * WTN;
                             if (a0 >= ni || b0 >= nj) continue;
-                            if (tri && b0 + WTN <= a0 + 1) continue;
+                            // a triangle's block holds a cell j > i iff its
+                            // last column inside the job lies right of its
+                            // first row
+                            const int jl = (b0 + WTN < nj ? b0 + WTN : nj) - 1;
+                            if (tri && jl <= a0) continue;
                             out[k++] = WorkItem{desc, (uint16_t)(ti * BM + wm),
                                                 (uint16_t)(tj * BN + wn)};
                         }
-                }
+                    }
         }
     return k;
 }

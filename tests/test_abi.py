@@ -127,8 +127,10 @@ def test_tile_index_matches_tile_list(tmp_path):
     """The closed-form position of a tile in a job's work list
     (job_tile_index, which the device tile-list kernel of the allocation-free
     entry point uses) equals its position in append_job_tiles' list for every
-    tile grid up to 70 x 70, rectangles and triangles (a host program built
-    from the same header)."""
+    tile grid up to 70 x 70, rectangles and triangles; and the per-wave block
+    lists of plans (append_job_wave_units, both kernel geometries) cover every
+    cell a job computes exactly once with no empty block, in row-band order
+    inside each super-tile (a host program built from the same header)."""
     import subprocess
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     if not os.path.exists(hipcc):
@@ -139,4 +141,4 @@ def test_tile_index_matches_tile_list(tmp_path):
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0 and r.stdout.startswith("ok 4970"), r.stdout + r.stderr
+    assert r.returncode == 0 and r.stdout.startswith("ok 4970 544"), r.stdout + r.stderr
