@@ -324,8 +324,12 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
                 size_t cnt = items.size();
                 if (waves) {
                     items.resize(cnt + tiles * (size_t)(tile / wtm) * (size_t)(tile / wtn));
-                    cnt += append_job_wave_units(items.data() + cnt, (uint32_t)descs.size(),
-                                                 w.b_size, w.a_size, w.tri, tile, wtm, wtn);
+                    const size_t n = append_job_wave_units(items.data() + cnt,
+                                                           (uint32_t)descs.size(), w.b_size,
+                                                           w.a_size, w.tri, tile, wtm, wtn);
+                    order_wave_units_by_work(items.data() + cnt, n, w.b_size, w.a_size, w.tri,
+                                             wtm, wtn);
+                    cnt += n;
                     items.resize(cnt);
                 } else {
                     items.resize(cnt + tiles);

@@ -16,6 +16,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 // One tile: rows [ti*T, ti*T + T) x columns [tj*T, tj*T + T) of job `desc`.
 struct WorkItem {
     uint32_t desc;
@@ -109,4 +111,37 @@ inline size_t append_job_wave_units(WorkItem *out, uint32_t desc, int ni, int nj
                     }
         }
     return k;
+}
+
+// 32 x 32 MFMA blocks the per-wave kernels compute for unit u of a job (the
+// kernels' block variants, ld_mfma.hip): 128 x 64 blocks (WTM = 128) run in
+// full, as their first 32 columns only on the job's ragged right edge, or as
+// the 3 blocks (1 on the edge) of a triangle's diagonal; 64 x 64 blocks skip
+// the 32-row / 32-column halves past the job's edge and the block left of a
+// diagonal block's diagonal.
+inline int wave_unit_blocks(const WorkItem &u, int ni, int nj, int tri, int WTM, int WTN)
+{
+    const int a0 = u.ti * WTM, b0 = u.tj * WTN;
+    const bool nfull = b0 + 32 < nj;
+    if (WTM == 128) {
+        if (tri && b0 == a0) return nfull ? 3 : 1;
+        return nfull ? 8 : 4;
+    }
+    const bool mfull = a0 + 32 < ni;
+    if (mfull && nfull) return tri && a0 == b0 ? 3 : 4;
+    return (mfull ? 2 : 1) * (nfull ? 2 : 1);
+}
+
+// Reorders one job's units (as append_job_wave_units wrote them) so that the
+// full blocks come first, in their order, and the partial ones after them,
+// most blocks first: the kernels hand runs of consecutive units to a
+// workgroup's waves side by side, and a run that mixes a diagonal block
+// with full ones leaves three of its waves idle for most of the run.
+inline void order_wave_units_by_work(WorkItem *units, size_t n, int ni, int nj, int tri,
+                                     int WTM, int WTN)
+{
+    std::stable_sort(units, units + n, [&](const WorkItem &x, const WorkItem &y) {
+        return wave_unit_blocks(x, ni, nj, tri, WTM, WTN) >
+               wave_unit_blocks(y, ni, nj, tri, WTM, WTN);
+    });
 }

@@ -130,7 +130,9 @@ def test_tile_index_matches_tile_list(tmp_path):
     tile grid up to 70 x 70, rectangles and triangles; and the per-wave block
     lists of plans (append_job_wave_units, both kernel geometries) cover every
     cell a job computes exactly once with no empty block, in row-band order
-    inside each super-tile (a host program built from the same header)."""
+    inside each super-tile, and their reordering by work is a permutation
+    with the most blocks first (a host program built from the same
+    header)."""
     import subprocess
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     if not os.path.exists(hipcc):

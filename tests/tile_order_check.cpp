@@ -83,6 +83,31 @@ int main()
                                 printf("cell (%d, %d) missed: %d %d %d\n", i, j, ni, nj, tri);
                                 return 1;
                             }
+                    // ordered by work: a permutation, most blocks first, list
+                    // order kept among equals, block counts those of the
+                    // kernels' variants (a block count covers its cells)
+                    std::vector<WorkItem> byw(units.begin(), units.begin() + n);
+                    order_wave_units_by_work(byw.data(), n, ni, nj, tri, WTM, WTN);
+                    std::vector<size_t> pos((size_t)(ni / WTM + 2) * (nj / WTN + 2), 0);
+                    for (size_t k = 0; k < n; ++k)
+                        pos[(size_t)units[k].ti * (nj / WTN + 2) + units[k].tj] = k + 1;
+                    for (size_t k = 0; k < n; ++k) {
+                        const int w = wave_unit_blocks(byw[k], ni, nj, tri, WTM, WTN);
+                        const size_t at = pos[(size_t)byw[k].ti * (nj / WTN + 2) + byw[k].tj];
+                        if (!at || w < 1 || w > (WTM / 32) * (WTN / 32)) {
+                            printf("work order %d %d %d: unit %zu\n", ni, nj, tri, k);
+                            return 1;
+                        }
+                        if (k) {
+                            const int wp = wave_unit_blocks(byw[k - 1], ni, nj, tri, WTM, WTN);
+                            const size_t ap =
+                                pos[(size_t)byw[k - 1].ti * (nj / WTN + 2) + byw[k - 1].tj];
+                            if (wp < w || (wp == w && ap >= at)) {
+                                printf("work order %d %d %d: unit %zu\n", ni, nj, tri, k);
+                                return 1;
+                            }
+                        }
+                    }
                     ++lists;
                 }
     printf("ok %ld %ld\n", grids, lists);
