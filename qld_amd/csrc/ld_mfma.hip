@@ -319,6 +319,7 @@ union TermBits {
 template <>
 struct Epi<QLD_R2_REFERENCE> {
     typedef RefTerm term;
+    static constexpr int ROW_UNROLL = 1;   // rows in flight (register budget)
     static __device__ __forceinline__ term make(uint32_t bc, const WorkDesc &d)
     {
         return ref_term(bc, d.fN);
@@ -327,8 +328,8 @@ struct Epi<QLD_R2_REFERENCE> {
     {
         TermBits b;
         b.s.f = t.p;
-        b.s.z = t.zero;
-        b.s.d = t.omp;
+        b.s.z = __float_as_uint(t.ip);
+        b.s.d = t.A;
         return b.u;
     }
     static __device__ __forceinline__ term unpack(uint4 u)
@@ -337,22 +338,35 @@ struct Epi<QLD_R2_REFERENCE> {
         b.u = u;
         term t;
         t.p = b.s.f;
-        t.zero = b.s.z != 0;
-        t.omp = b.s.d;
+        t.ip = __uint_as_float(b.s.z);
+        t.A = b.s.d;
         return t;
     }
+    // Four pairs without a division; a lane whose pair may round otherwise
+    // (about one pair in 2^19) redoes that pair with the reference's
+    // division, under one branch for the four.
     static __device__ __forceinline__ void r2x4(const float (&c)[4], const term (&col)[4],
                                                 const term &row, const WorkDesc &d,
                                                 float (&out)[4])
     {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) out[q] = r2_reference(c[q], col[q], row, d.invN);
+        bool s0, s1, s2, s3;
+        out[0] = r2_reference_fast(c[0], col[0], row, d.invN, s0);
+        out[1] = r2_reference_fast(c[1], col[1], row, d.invN, s1);
+        out[2] = r2_reference_fast(c[2], col[2], row, d.invN, s2);
+        out[3] = r2_reference_fast(c[3], col[3], row, d.invN, s3);
+        if (s0 | s1 | s2 | s3) {
+            if (s0) out[0] = r2_reference_div(c[0], col[0].p, row.p, d.invN);
+            if (s1) out[1] = r2_reference_div(c[1], col[1].p, row.p, d.invN);
+            if (s2) out[2] = r2_reference_div(c[2], col[2].p, row.p, d.invN);
+            if (s3) out[3] = r2_reference_div(c[3], col[3].p, row.p, d.invN);
+        }
     }
 };
 
 template <>
 struct Epi<QLD_R2_EXACT> {
     typedef ExactTerm term;
+    static constexpr int ROW_UNROLL = 2;
     static __device__ __forceinline__ term make(uint32_t bc, const WorkDesc &d)
     {
         return exact_term(bc, d.dN);
@@ -736,7 +750,10 @@ __device__ __forceinline__ void tile_epilogue_impl(const WorkDesc &d, int i0, in
                 ep[row * G::EP_PITCH + b * 32 + r32] = OP::bits(acc[a][b][r]);
             }
         const int ibase = i0 + wm * WTM + a * 32;
-#pragma unroll 2
+        // this lane's first row of the block; row steps are uniform offsets
+        // (scalar arithmetic), so a row costs one vector address add
+        g_f32 *const res0 = (g_f32 *)d.res + (int64_t)(ibase + rq) * d.ldr + jc;
+#pragma unroll E::ROW_UNROLL
         for (int it = 0; it < 32 / G::ROWS_PER_IT; ++it) {
             const int rl = it * G::ROWS_PER_IT + rq;
             const int i = ibase + rl;
@@ -747,30 +764,25 @@ __device__ __forceinline__ void tile_epilogue_impl(const WorkDesc &d, int i0, in
                                  OP::count(c4.w)};
             float rv[4];
             E::r2x4(cv, col, rowt, d, rv);
+            g_f32 *const rdst = res0 + (int64_t)(it * G::ROWS_PER_IT) * d.ldr;
+            g_u32 *const cdst = (g_u32 *)d.cnt + (int64_t)i * d.ldc + jc;   // rarely asked for
             if constexpr (INTERIOR) {
                 // Every column of the tile is inside the job (and above the
                 // diagonal for a triangle) and rows are 16-byte aligned.
-                if (d.res) {
-                    const v4f v = {rv[0], rv[1], rv[2], rv[3]};
-                    *(g_v4f *)((g_f32 *)d.res + (int64_t)i * d.ldr + jc) = v;
-                }
-                if (d.cnt) {
-                    const v4u v = {(uint32_t)cv[0], (uint32_t)cv[1], (uint32_t)cv[2],
-                                   (uint32_t)cv[3]};
-                    *(g_v4u *)((g_u32 *)d.cnt + (int64_t)i * d.ldc + jc) = v;
-                }
+                if (d.res) *(g_v4f *)rdst = v4f{rv[0], rv[1], rv[2], rv[3]};
+                if (d.cnt)
+                    *(g_v4u *)cdst = v4u{(uint32_t)cv[0], (uint32_t)cv[1], (uint32_t)cv[2],
+                                         (uint32_t)cv[3]};
             } else {
                 if (d.res) {
-                    g_f32 *dst = (g_f32 *)d.res + (int64_t)i * d.ldr + jc;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        if (jc + q < d.nj && (!d.tri || jc + q > i)) dst[q] = rv[q];
+                        if (jc + q < d.nj && (!d.tri || jc + q > i)) rdst[q] = rv[q];
                 }
                 if (d.cnt) {
-                    g_u32 *dst = (g_u32 *)d.cnt + (int64_t)i * d.ldc + jc;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        if (jc + q < d.nj && (!d.tri || jc + q > i)) dst[q] = (uint32_t)cv[q];
+                        if (jc + q < d.nj && (!d.tri || jc + q > i)) cdst[q] = (uint32_t)cv[q];
                 }
             }
         }
