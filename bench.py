@@ -278,6 +278,17 @@ def config1_bench(args, steps):
     return out
 
 
+def union_pairs(starts, sizes, rows):
+    """Pairs (i, j), i < j, that at least one window [s, s + size) holds:
+    row i pairs with every j below the furthest end of a window starting at
+    or before i (windows that end before i add nothing)."""
+    best = np.full(rows, -1, np.int64)
+    np.maximum.at(best, np.asarray(starts, np.int64),
+                  np.asarray(starts, np.int64) + np.asarray(sizes, np.int64))
+    best = np.maximum.accumulate(best)
+    return int(np.maximum(best - np.arange(rows) - 1, 0).sum())
+
+
 def windows_bench(args, steps):
     """4,096 windows, sizes round(exp(U(ln 50, ln 2000))), starts uniform,
     over a 500,000-SNP x 16,622-sample chromosome: one persistent launch per
@@ -333,6 +344,18 @@ def windows_bench(args, steps):
         kms.append(eng.last_contract_ms())
     k_s = float(np.median(kms)) / 1e3
     batched.free()
+    # Shared-pair plan: each pair some window holds is counted once, over the
+    # chromosome's triangle, and its r2 written into every window holding it.
+    shared = eng.plan_windows(t, wins, out, qld_amd.BATCH_SHARED)
+    ms_s = timed(shared, graphs, steps)
+    sst = shared.stats()
+    skms = []
+    for _ in range(3):
+        shared.run()
+        skms.append(eng.last_contract_ms())
+    ks_s = float(np.median(skms)) / 1e3
+    shared.free()
+    upairs = union_pairs(starts, sizes, rows)
     # The baseline, one launch per window, on 128-tiles: a single window's
     # launch needs the finer tiles to fill the GPU (its fastest setting).
     ms_pg = ms_pd = None
@@ -352,7 +375,16 @@ def windows_bench(args, steps):
                                      measured_traffic(f"windows{nwin}", args.r2_mode)
                                      if (nwin, rows, N) == (4096, 500000, 16622) else None,
                                      rows=int(sizes.sum()))})
+    res["shared"] = {
+        "ms": round(ms_s, 3), "gpairs_s": round(pairs / ms_s / 1e6, 2),
+        "units": sst["tiles"], "launches": sst["launches"],
+        "distinct_pairs": upairs,
+        "roofline": roofline("mfma_fp4", upairs, (N + 63) // 64, N, ks_s,
+                             measured_traffic(f"windows{nwin}_shared", args.r2_mode)
+                             if (nwin, rows, N) == (4096, 500000, 16622) else None,
+                             rows=int(sizes.sum()))}
     if ms_pd is not None:
+        res["shared"]["speedup_vs_plain"] = round(ms_pd / ms_s, 2)
         res.update({"plain_direct_ms": round(ms_pd, 3),
                     "plain_direct_gpairs_s": round(pairs / ms_pd / 1e6, 2),
                     "speedup_vs_plain": round(ms_pd / ms_b, 2)})
@@ -360,6 +392,7 @@ def windows_bench(args, steps):
         res.update({"plain_graph_ms": round(ms_pg, 3),
                     "plain_graph_gpairs_s": round(pairs / ms_pg / 1e6, 2),
                     "speedup_vs_plain_graph": round(ms_pg / ms_b, 2)})
+    log(f"[bench] windows: shared-pair plan {ms_s:.3f} ms ({upairs} distinct pairs)")
     log(f"[bench] windows: batched {ms_b:.3f} ms, plain graph {ms_pg} ms, "
         f"plain direct {ms_pd} ms (cutoff {cutoff}, schedule {sched}, graphs {graphs})")
     return res

@@ -216,6 +216,12 @@ typedef struct qld_window {
 #define QLD_BATCH_PLAIN      0  /* one kernel launch per window             */
 #define QLD_BATCH_PERSISTENT 1  /* every window's tiles in one persistent
                                    launch with per-XCD work queues         */
+#define QLD_BATCH_SHARED     2  /* one persistent launch over the table's
+                                   triangle: each pair that any window holds
+                                   is counted once and its r^2 written into
+                                   every window holding it (overlapping
+                                   windows share work); single-region windows
+                                   of a direct fp4 table only               */
 
 typedef struct qld_plan qld_plan;
 
@@ -250,7 +256,9 @@ int qld_plan_use_graph(qld_plan *plan, int use_graph);
 /* Whether plans built afterwards capture their launches in a HIP graph
  * (default 1); without one, qld_plan_run launches the kernels directly. */
 int qld_set_plan_graphs(qld_ctx *ctx, int capture);
-/* Tiles and pairs the plan computes; kernel launches per run. */
+/* Tiles and pairs the plan computes; kernel launches per run.  Pairs count
+ * every window's pairs (the outputs); tiles are work units (256 x 256 tiles,
+ * or for QLD_BATCH_SHARED the 128 x 64 blocks counted once). */
 int qld_plan_stats(const qld_plan *plan, int64_t *tiles, int64_t *pairs,
                    int *launches);
 int qld_plan_free(qld_ctx *ctx, qld_plan *plan);

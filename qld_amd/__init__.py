@@ -129,6 +129,7 @@ PAIR_WINDOW_DTYPE = np.dtype([("a_start", "<i4"), ("a_size", "<i4"), ("b_start",
                               ("reserved", "<i4")])
 BATCH_PLAIN = 0
 BATCH_PERSISTENT = 1
+BATCH_SHARED = 2
 
 HOST_SYMBOLS = [
     ("qld_format_float", C.c_int, [C.c_char_p, C.c_float, C.c_int]),

@@ -100,6 +100,25 @@ hipError_t mfma_init_attributes();
 hipError_t launch_mfma_wave_units(int r2_mode, int tile, const WorkDesc *descs,
                                   const WorkDesc &inline_desc, const WorkItem *units,
                                   int64_t nunits, int grid, uint32_t *queue, hipStream_t s);
+
+// Output of one window of a shared-pair plan: the window's rows
+// [start, start + size) of the table and its size x size r^2 block at `out`
+// (cell (i, j) of the window at out[i * size + j], j > i written).
+struct WinOut {
+    int32_t start, size;
+    float *out;
+};
+
+// Shared-pair units: `units` lists 128 x 64 wave blocks (0, bi, bj) of one
+// triangle job over a whole table (inline_desc, its r^2 pointer unused), each
+// computed once; item.desc is the offset in `wlist` of the block's window
+// list (a count, then that many indices into `wins`), and the epilogue writes
+// the block's r^2 into every listed window that holds the cell.  Tile 256
+// geometry, direct fp4 operands; otherwise as launch_mfma_wave_units.
+hipError_t launch_mfma_shared_units(int r2_mode, const WorkDesc &inline_desc,
+                                    const WorkItem *units, int64_t nunits,
+                                    const uint32_t *wlist, const WinOut *wins, int grid,
+                                    uint32_t *queue, hipStream_t s);
 void wave_block_shape(int tile, int *wtm, int *wtn);
 
 // Launch the MFMA kernel over `nitems` items.

@@ -154,6 +154,8 @@ struct KArgs {
     int64_t nitems;
     uint32_t *queue;          // persistent launches: 8 per-XCD queue heads
     int64_t qbeg[QLD_NUM_XCD + 1];   // queue x owns items [qbeg[x], qbeg[x+1])
+    const uint32_t *wlist;    // shared-pair units: window lists (item.desc = offset)
+    const WinOut *wins;       // shared-pair units: window outputs
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -361,6 +363,23 @@ struct Epi<QLD_R2_REFERENCE> {
             if (s3) out[3] = r2_reference_div(c[3], col[3].p, row.p, d.invN);
         }
     }
+    // Four rows of one column.
+    static __device__ __forceinline__ void r2x4_rows(const float (&c)[4], const term &col,
+                                                     const term (&row)[4], const WorkDesc &d,
+                                                     float (&out)[4])
+    {
+        bool s0, s1, s2, s3;
+        out[0] = r2_reference_fast(c[0], col, row[0], d.invN, s0);
+        out[1] = r2_reference_fast(c[1], col, row[1], d.invN, s1);
+        out[2] = r2_reference_fast(c[2], col, row[2], d.invN, s2);
+        out[3] = r2_reference_fast(c[3], col, row[3], d.invN, s3);
+        if (s0 | s1 | s2 | s3) {
+            if (s0) out[0] = r2_reference_div(c[0], col.p, row[0].p, d.invN);
+            if (s1) out[1] = r2_reference_div(c[1], col.p, row[1].p, d.invN);
+            if (s2) out[2] = r2_reference_div(c[2], col.p, row[2].p, d.invN);
+            if (s3) out[3] = r2_reference_div(c[3], col.p, row[3].p, d.invN);
+        }
+    }
 };
 
 template <>
@@ -396,6 +415,13 @@ struct Epi<QLD_R2_EXACT> {
     {
 #pragma unroll
         for (int q = 0; q < 4; ++q) out[q] = r2_exact(c[q], col[q], row, d.dN);
+    }
+    static __device__ __forceinline__ void r2x4_rows(const float (&c)[4], const term &col,
+                                                     const term (&row)[4], const WorkDesc &d,
+                                                     float (&out)[4])
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = r2_exact(c[q], col, row[q], d.dN);
     }
 };
 
@@ -816,13 +842,96 @@ __device__ __forceinline__ void tile_epilogue(const WorkDesc &d, int i0, int j0,
         tile_epilogue_impl<OP, MODE, G, false>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
 }
 
+// Shared-pair epilogue: one wave block of a triangle over a whole table,
+// written into every window of its list that holds some of its cells.  Lane
+// l owns column jb + l of the block (its bit count came in bc_col[0]): for
+// each 32-row block the wave stages the counts in LDS, reads row by row its
+// column's count (rows are uniform, row terms a broadcast read) and keeps
+// the 32 r^2 values in registers; then, window by window, the rows inside
+// the window (a uniform range) are stored at the window's
+// out[(i - start) * size + (j - start)] for columns j in (i, window end):
+// one store per row of up to 64 consecutive floats, its row address scalar.
+template <class OP, int MODE, class G>
+__device__ __forceinline__ void tile_epilogue_shared(const WorkDesc &d, int i0, int j0, int wm,
+                                                     int wn, int wave,
+                                                     typename OP::acc_t (&acc)[G::NBM][G::NBN],
+                                                     const uint32_t *bc_row,
+                                                     const uint32_t *bc_col, uint8_t *smem,
+                                                     const uint32_t *wl, const WinOut *wins)
+{
+    typedef Epi<MODE> E;
+    constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, RPL = G::RPL;
+    static_assert(WTN == 64, "one column per lane");
+    const int lane = threadIdx.x & 63;
+    const int r32 = lane & 31, half = lane >> 5;
+    const int jb = j0 + wn * WTN, j = jb + lane;
+    uint32_t *ep = (uint32_t *)(smem + wave * G::EP_WAVE_BYTES);
+    uint4 *terms = (uint4 *)(smem + G::EP_TERMS_OFF) + wave * WTM;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q)
+        if (lane + q * 64 < WTM) terms[q * 64 + lane] = E::pack(E::make(bc_row[q], d));
+    const typename E::term colt = E::make(bc_col[0], d);
+    const int nw = (int)load_uniform(wl, 0);
+#pragma unroll
+    for (int a = 0; a < NBM; ++a) {
+#pragma unroll
+        for (int b = 0; b < NBN; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                int row = 8 * (r >> 2) + 4 * half + (r & 3);
+                ep[row * G::EP_PITCH + b * 32 + r32] = OP::bits(acc[a][b][r]);
+            }
+        const int ibase = i0 + wm * WTM + a * 32;
+        if (ibase >= d.ni) continue;
+        float r2v[32];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+            float cv[4];
+            typename E::term rowt[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cv[u] = OP::count(ep[(4 * g + u) * G::EP_PITCH + lane]);
+                rowt[u] = E::unpack(terms[a * 32 + 4 * g + u]);
+            }
+            float rv[4];
+            E::r2x4_rows(cv, colt, rowt, d, rv);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) r2v[4 * g + u] = rv[u];
+        }
+        for (int k = 0; k < nw; ++k) {
+            const WinOut w = load_uniform(wins, (int64_t)load_uniform(wl, 1 + k));
+            const int ws = w.start, we = w.start + w.size;
+            const int r0 = (ws > ibase ? ws : ibase) - ibase;
+            const int r1 = (we < ibase + 32 ? we : ibase + 32) - ibase;
+            if (r0 >= r1) continue;
+            g_f32 *row = (g_f32 *)w.out + (int64_t)(ibase - ws) * w.size + (jb - ws);
+            const bool colok = j < we;
+            if (r0 == 0 && r1 == 32 && jb >= ibase + 32) {
+                // all 32 rows inside the window and every column right of
+                // them: one lane mask for the block, scalar row addresses
+                if (colok) {
+#pragma unroll
+                    for (int rl = 0; rl < 32; ++rl, row += w.size) row[lane] = r2v[rl];
+                }
+            } else {
+#pragma unroll
+                for (int rl = 0; rl < 32; ++rl, row += w.size)
+                    if (rl >= r0 && rl < r1 && colok && j > ibase + rl) row[lane] = r2v[rl];
+            }
+        }
+    }
+}
+
 // One tile of one job: k loop (expanded operands by DMA, or packed operands
 // expanded in the kernel), then the fused epilogue.
 // wm, wn: the wave's block of the tile; wave: the wave's LDS epilogue
 // region.  Cooperative k loops (LDS ring) need wm, wn = this wave's own.
-template <class OP, int MODE, class G, int PACKED, bool WAVE_UNITS = false, class Hook = NoHook>
+template <class OP, int MODE, class G, int PACKED, bool WAVE_UNITS = false, class Hook = NoHook,
+          bool SHARED = false>
 __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int wm, int wn,
-                                          int wave, uint8_t *smem, const Hook &hook = Hook())
+                                          int wave, uint8_t *smem, const Hook &hook = Hook(),
+                                          const uint32_t *wl = nullptr,
+                                          const WinOut *wins = nullptr)
 {
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, TILE = G::EDGE;
     constexpr int RPL = G::RPL;                         // epilogue rows per lane
@@ -851,8 +960,14 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
     }
     const int jc = j0 + wn * WTN + 4 * (lane % G::LANES_PER_ROW);
     uint32_t bc_col[4];
+    if constexpr (SHARED) {
+        // the shared-pair epilogue gives each lane one column
+        const int jl = j0 + wn * WTN + lane;
+        bc_col[0] = jl < d.nj ? ((g_cu32 *)d.bcJ)[jl] : 0u;
+    } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? ((g_cu32 *)d.bcJ)[jc + q] : 0u;
+        for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? ((g_cu32 *)d.bcJ)[jc + q] : 0u;
+    }
 
     typename OP::acc_t acc[NBM][NBN];
 #pragma unroll
@@ -919,7 +1034,11 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
         lds_barrier();   // every wave is done with the ring before the epilogue reuses it
     }
     hook.end();
-    tile_epilogue<OP, MODE, G>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
+    if constexpr (SHARED)
+        tile_epilogue_shared<OP, MODE, G>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem, wl,
+                                          wins);
+    else
+        tile_epilogue<OP, MODE, G>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
 }
 
 template <class OP, int MODE, class G, int PACKED, class Hook = NoHook>
@@ -938,14 +1057,18 @@ __device__ __forceinline__ void run_item(const KArgs &p, int64_t idx, int wm, in
 
 // One wave block of a per-wave unit list: item (desc, bi, bj) names rows
 // [bi*WTM, bi*WTM + WTM) and columns [bj*WTN, bj*WTN + WTN) of the job.
-template <class OP, int MODE, class G, int PACKED, class Hook>
+template <class OP, int MODE, class G, int PACKED, bool SHARED, class Hook>
 __device__ __forceinline__ void run_wave_unit(const KArgs &p, int64_t idx, int wave,
                                               uint8_t *smem, const Hook &hook)
 {
     constexpr int BM = G::EDGE / G::WTM, BN = G::EDGE / G::WTN;   // wave blocks per tile side
     const WorkItem it = load_uniform(p.items, idx);
     const int bi = it.ti, bj = it.tj;
-    if (p.descs) {
+    if constexpr (SHARED) {
+        mfma_tile<OP, MODE, G, PACKED, true, Hook, true>(p.inline_desc, bi / BM, bj / BN, bi % BM,
+                                                         bj % BN, wave, smem, hook,
+                                                         p.wlist + it.desc, p.wins);
+    } else if (p.descs) {
         const WorkDesc d = load_uniform(p.descs, it.desc);
         mfma_tile<OP, MODE, G, PACKED, true, Hook>(d, bi / BM, bj / BN, bi % BM, bj % BN, wave,
                                                    smem, hook);
@@ -977,8 +1100,8 @@ __device__ __forceinline__ void run_wave_unit(const KArgs &p, int64_t idx, int w
 template <class OP, int MODE, class G, int PERSIST, int PACKED>
 __device__ __forceinline__ void run_queues(const KArgs &p, int wave, uint8_t *smem)
 {
-    constexpr int64_t PER = PERSIST == 2 ? G::WAVES : 1;   // list entries per claim
-    constexpr int64_t SUB = PERSIST == 2 ? 1 : G::SPLIT;   // claims per list entry
+    constexpr int64_t PER = PERSIST >= 2 ? G::WAVES : 1;   // list entries per claim
+    constexpr int64_t SUB = PERSIST >= 2 ? 1 : G::SPLIT;   // claims per list entry
     const int wr = wave / G::WAVES_N, wn = wave % G::WAVES_N;
     __shared__ uint32_t claimed[2];
     __shared__ int claimed_x[2];      // -1 once every queue is drained
@@ -1019,11 +1142,12 @@ __device__ __forceinline__ void run_queues(const KArgs &p, int wave, uint8_t *sm
                 const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)pend);
                 if (leader) settle(slot ^ 1, c);
             });
-        if constexpr (PERSIST == 2) {
+        if constexpr (PERSIST >= 2) {
             const int64_t idx = p.qbeg[cx] + k * PER + wave;
             // the last strip of a queue may be short (wave 0, the leader's,
             // always has a block)
-            if (idx < p.qbeg[cx + 1]) run_wave_unit<OP, MODE, G, PACKED>(p, idx, wave, smem, ahead);
+            if (idx < p.qbeg[cx + 1])
+                run_wave_unit<OP, MODE, G, PACKED, PERSIST == 3>(p, idx, wave, smem, ahead);
         } else {
             const int64_t u = p.qbeg[cx] * SUB + k;
             run_item<OP, MODE, G, PACKED>(p, u / SUB, (int)(u % SUB) * G::WAVES_M + wr, wn, wave,
@@ -1039,7 +1163,7 @@ __global__ __launch_bounds__(G::THREADS, G::SPLIT) void ld_mfma_kernel(KArgs p)
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     const int wave = wave_id();
     const int wr = wave / G::WAVES_N, wn = wave % G::WAVES_N;
-    static_assert(PERSIST != 2 || PACKED == 2, "per-wave units need direct operands");
+    static_assert(PERSIST < 2 || PACKED == 2, "per-wave units need direct operands");
     if constexpr (PERSIST == 0) {
         // one workgroup per unit: workgroup b takes unit b
         const int64_t u = (int64_t)blockIdx.x;
@@ -1293,6 +1417,7 @@ static hipError_t set_lds_limits_packed()
     if ((e = set_lds_limit<FP4, MODE, GeoL8H, 0, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoS, 2, 2>()) != hipSuccess) return e;
     if ((e = set_lds_limit<FP4, MODE, GeoL8H, 2, 2>()) != hipSuccess) return e;
+    if ((e = set_lds_limit<FP4, MODE, GeoL8H, 3, 2>()) != hipSuccess) return e;
     return set_lds_limit<FP4, MODE, GeoL8H, 1, 2>();
 }
 
@@ -1338,7 +1463,7 @@ hipError_t launch_mfma_wave_units(int r2_mode, int tile, const WorkDesc *descs,
 {
     if (nunits <= 0) return hipSuccess;
     if (!queue || grid <= 0) return hipErrorInvalidValue;
-    KArgs a;
+    KArgs a = {};
     a.inline_desc = inline_desc;
     a.descs = descs;
     a.items = units;
@@ -1357,6 +1482,30 @@ hipError_t launch_mfma_wave_units(int r2_mode, int tile, const WorkDesc *descs,
                : launch_one<FP4, QLD_R2_REFERENCE, GeoL8H, 2, 2>(a, (unsigned)grid, s);
 }
 
+hipError_t launch_mfma_shared_units(int r2_mode, const WorkDesc &inline_desc,
+                                    const WorkItem *units, int64_t nunits,
+                                    const uint32_t *wlist, const WinOut *wins, int grid,
+                                    uint32_t *queue, hipStream_t s)
+{
+    if (nunits <= 0) return hipSuccess;
+    if (!queue || !wlist || !wins || grid <= 0) return hipErrorInvalidValue;
+    KArgs a = {};
+    a.inline_desc = inline_desc;
+    a.descs = nullptr;
+    a.items = units;
+    a.nitems = nunits;
+    a.queue = queue;
+    a.wlist = wlist;
+    a.wins = wins;
+    for (int x = 0; x <= QLD_NUM_XCD; ++x) a.qbeg[x] = nunits * x / QLD_NUM_XCD;
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, queue);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return r2_mode == QLD_R2_EXACT
+               ? launch_one<FP4, QLD_R2_EXACT, GeoL8H, 3, 2>(a, (unsigned)grid, s)
+               : launch_one<FP4, QLD_R2_REFERENCE, GeoL8H, 3, 2>(a, (unsigned)grid, s);
+}
+
 void wave_block_shape(int tile, int *wtm, int *wtn)
 {
     *wtm = tile == QLD_TILE_SMALL ? GeoS::WTM : GeoL8H::WTM;
@@ -1370,7 +1519,7 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
-    KArgs a;
+    KArgs a = {};
     a.inline_desc = inline_desc;
     a.descs = descs;
     a.items = items;

@@ -20,11 +20,15 @@
 struct PlanClass {
     int tile = QLD_TILE_LARGE;
     bool wave_units = false;        // items are wave blocks claimed per wave
+    bool shared = false;            // shared-pair units over one table triangle
     int grid = 0;                   // persistent workgroups
     size_t ndesc = 0, nitems = 0;
     WorkDesc *d_descs = nullptr;
     WorkItem *d_items = nullptr;
     uint32_t *d_queue = nullptr;
+    WorkDesc gdesc;                 // shared: the table's triangle job
+    uint32_t *d_wlist = nullptr;    // shared: window lists per unit
+    WinOut *d_wins = nullptr;       // shared: window outputs
 };
 
 struct PlainLaunch {
@@ -205,10 +209,14 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
     hipError_t e = hipSuccess;
     WorkDesc none;
     memset(&none, 0, sizeof(none));
-    if (p->mode == QLD_BATCH_PERSISTENT) {
+    if (p->mode == QLD_BATCH_PERSISTENT || p->mode == QLD_BATCH_SHARED) {
         for (auto &c : p->classes) {
             if (!c.nitems) continue;
-            if (c.wave_units)
+            if (c.shared)
+                e = launch_mfma_shared_units(p->r2_mode, c.gdesc, c.d_items, (int64_t)c.nitems,
+                                             c.d_wlist, c.d_wins, c.grid, c.d_queue,
+                                             ctx->stream);
+            else if (c.wave_units)
                 e = launch_mfma_wave_units(p->r2_mode, c.tile, c.d_descs, none, c.d_items,
                                            (int64_t)c.nitems, c.grid, c.d_queue, ctx->stream);
             else
@@ -252,6 +260,82 @@ static int pair_tile(const qld_ctx *ctx, const qld_table *t, const PairWin &w)
 {
     const int tile = qld_window_tile(ctx, std::min(w.a_size, w.b_size));
     return t->direct ? tile : expanded_tile_cap(t->row_bytes, tile);
+}
+
+// Shared-pair plan: the 128 x 64 blocks of the table's triangle that hold a
+// cell of some window, each listed once with the windows it serves, in row
+// band order (a workgroup's four waves claim four consecutive blocks, mostly
+// of one band, so they share their rows).  A window [s, e) holds cells of
+// block (bi, bj) iff some row r in [s, e - 1) of the band has a column in
+// (r, e) inside the block: bi from s/128 to (e-2)/128, and for each, bj from
+// (r0 + 1)/64 to (e - 1)/64 with r0 the band's first row inside the window.
+static int build_shared_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWin> &wins,
+                             float *out, qld_plan *p)
+{
+    int wtm = 0, wtn = 0;
+    wave_block_shape(QLD_TILE_LARGE, &wtm, &wtn);
+    if (!t->direct || (int64_t)t->rows > 65535LL * wtn) {
+        qld_set_error(ctx, "shared-pair plans need a direct fp4 table of at most %lld rows",
+                      65535LL * wtn);
+        return QLD_ERR_INVALID;
+    }
+    struct Ref {
+        uint32_t bi, bj, w;
+    };
+    std::vector<Ref> refs;
+    std::vector<WinOut> outs(wins.size());
+    for (size_t k = 0; k < wins.size(); ++k) {
+        const PairWin &w = wins[k];
+        if (!w.tri) {
+            qld_set_error(ctx, "shared-pair plans take single-region windows only (window %zu)",
+                          k);
+            return QLD_ERR_INVALID;
+        }
+        outs[k] = WinOut{w.a_start, w.a_size, out + w.out_offset};
+        p->pairs += pair_count(w);
+        if (pair_empty(w)) continue;
+        const int s = w.a_start, e = w.a_start + w.a_size;
+        for (int bi = s / wtm; bi <= (e - 2) / wtm; ++bi) {
+            const int r0 = std::max(s, bi * wtm);
+            for (int bj = (r0 + 1) / wtn; bj <= (e - 1) / wtn; ++bj)
+                refs.push_back(Ref{(uint32_t)bi, (uint32_t)bj, (uint32_t)k});
+        }
+    }
+    std::sort(refs.begin(), refs.end(), [](const Ref &x, const Ref &y) {
+        return x.bi != y.bi ? x.bi < y.bi : x.bj != y.bj ? x.bj < y.bj : x.w < y.w;
+    });
+    std::vector<WorkItem> items;
+    std::vector<uint32_t> wl;
+    for (size_t a = 0; a < refs.size();) {
+        size_t b = a;
+        while (b < refs.size() && refs[b].bi == refs[a].bi && refs[b].bj == refs[a].bj) ++b;
+        items.push_back(WorkItem{(uint32_t)wl.size(), (uint16_t)refs[a].bi, (uint16_t)refs[a].bj});
+        wl.push_back((uint32_t)(b - a));
+        for (size_t k = a; k < b; ++k) wl.push_back(refs[k].w);
+        a = b;
+    }
+    if (items.empty()) return QLD_OK;
+    PlanClass c;
+    c.tile = QLD_TILE_LARGE;
+    c.wave_units = true;
+    c.shared = true;
+    c.nitems = items.size();
+    c.grid = (int)std::min<int64_t>(2LL * ctx->num_cu, ((int64_t)items.size() + 3) / 4);
+    fill_work_desc(c.gdesc, nullptr, nullptr, t->bc, t->bc, t->rows, t->rows, 0, 1, t->snp_size,
+                   nullptr, 0, nullptr, 0);
+    set_tiled_operands(c.gdesc, t->packed, 0, t->rows, t->packed, 0, t->rows, t->words);
+    int rc;
+    if ((rc = plan_upload(ctx, p, (void **)&c.d_items, items.data(),
+                          items.size() * sizeof(WorkItem))) ||
+        (rc = plan_upload(ctx, p, (void **)&c.d_wlist, wl.data(), wl.size() * sizeof(uint32_t))) ||
+        (rc = plan_upload(ctx, p, (void **)&c.d_wins, outs.data(),
+                          outs.size() * sizeof(WinOut))) ||
+        (rc = plan_alloc(ctx, p, (void **)&c.d_queue, 16 * sizeof(uint32_t))))
+        return rc;
+    p->tiles = (int64_t)items.size();
+    p->launches = 1;
+    p->classes.push_back(c);
+    return QLD_OK;
 }
 
 static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWin> &wins,
@@ -307,7 +391,12 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
     };
 
     int rc = QLD_OK;
-    if (mode == QLD_BATCH_PERSISTENT) {
+    if (mode == QLD_BATCH_SHARED) {
+        if ((rc = build_shared_plan(ctx, t, wins, out, p))) {
+            plan_release(ctx, p);
+            return rc;
+        }
+    } else if (mode == QLD_BATCH_PERSISTENT) {
         // Direct fp4 operands: every wave claims wave blocks of its own
         // (unless the context asks for whole tiles per workgroup).
         const bool waves = p->direct && ctx->plan_sched == QLD_SCHED_WAVES;
@@ -418,7 +507,7 @@ extern "C" int qld_plan_windows(qld_ctx *ctx, const qld_table *t, const qld_wind
                                 int nwin, float *out, int mode, qld_plan **plan)
 {
     if (!ctx || !t || !plan || nwin < 0 || (nwin && (!wins || !out)) ||
-        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT)) {
+        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT && mode != QLD_BATCH_SHARED)) {
         if (ctx) qld_set_error(ctx, "qld_plan_windows: bad arguments");
         return QLD_ERR_INVALID;
     }
@@ -434,7 +523,7 @@ extern "C" int qld_plan_pairs(qld_ctx *ctx, const qld_table *t, const qld_pair_w
                               int nwin, float *out, int mode, qld_plan **plan)
 {
     if (!ctx || !t || !plan || nwin < 0 || (nwin && (!wins || !out)) ||
-        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT)) {
+        (mode != QLD_BATCH_PLAIN && mode != QLD_BATCH_PERSISTENT && mode != QLD_BATCH_SHARED)) {
         if (ctx) qld_set_error(ctx, "qld_plan_pairs: bad arguments");
         return QLD_ERR_INVALID;
     }

@@ -3,8 +3,8 @@ on sampled output.  Needs an MI355X.
 
 * 4,096 windows, sizes LogUniform(50, 2000), over a 500,000-SNP x
   16,622-sample chromosome, in one persistent plan (the benchmark's own
-  window set): 48 windows checked in full, the largest and the smallest
-  among them.
+  window set), per-window and shared-pair: 48 windows checked in full, the
+  largest and the smallest among them.
 * Region A 8,000 SNPs x region B 12,000 SNPs over 100,000 samples, the full
   rectangle: 64 rows checked, counts and r^2.
 * A server over a 200,000-sample table next to a 16,622-sample one, fed a
@@ -56,7 +56,8 @@ def _check_block(got, A_cols, bc_cols, B_rows, bc_rows, N, mode, upper_from=None
     return float(err.max(initial=0.0)), float(err.mean()) if err.size else 0.0
 
 
-def test_config_batched_windows_full_plan(engine_factory):
+@pytest.mark.parametrize("batch", [qld_amd.BATCH_PERSISTENT, qld_amd.BATCH_SHARED])
+def test_config_batched_windows_full_plan(batch, engine_factory):
     N, rows, nwin = 16622, 500000, 4096
     seed = 20261203 + 4
     say("config windows: generating", rows, "SNPs")
@@ -70,10 +71,10 @@ def test_config_batched_windows_full_plan(engine_factory):
     t = e.table(A, bc, N)
     out = e.alloc(total * 4)
     out.fill(0x7FC00000)
-    plan = e.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
+    plan = e.plan_windows(t, wins, out, batch)
     st = plan.stats()
     assert st["pairs"] == int((sizes * (sizes - 1) // 2).sum())
-    say("config windows: plan", st)
+    say("config windows: plan", batch, st)
     plan.run()
     e.sync()
     pick = {int(np.argmax(sizes)), int(np.argmin(sizes))}

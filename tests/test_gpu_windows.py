@@ -296,3 +296,74 @@ def test_pair_window_validation(engine_factory):
         assert x.value.code == qld_amd.QLD_ERR_INVALID
     t.free()
     out.free()
+
+
+@pytest.mark.parametrize("r2_mode", ["reference", "exact"])
+def test_shared_pair_plans(r2_mode, engine_factory):
+    """QLD_BATCH_SHARED: overlapping windows -- random, nested, duplicated,
+    sizes 0 to 3, one reaching the table's last row (5,003 rows: no multiple
+    of the 128 x 64 blocks) -- each pair counted once and written to every
+    window holding it: every window against the oracle and bit-identical to
+    the per-window plan, graph-replayed and by direct launches."""
+    N, rows = 16622, 5003
+    A, bc = qld_amd.synth_blocks(4246, rows, N)
+    rng = np.random.default_rng(31 + (r2_mode == "exact"))
+    wins = _windows(rng, rows, 40, 2, 1500)
+    extra_s = [0, 0, 0, 64, 127, 128, 1000, 1000, 1100, rows - 700, rows - 2, rows - 1, 2500]
+    extra_n = [0, 1, 2, 3, 129, 64, 800, 800, 300, 700, 2, 1, 1500]
+    wins = qld_amd.window_layout(np.concatenate([wins["start"], extra_s]),
+                                 np.concatenate([wins["size"], extra_n]))
+    total = max(int((wins["size"].astype(np.int64) ** 2).sum()), 1)
+    e = engine_factory(engine="mfma_fp4", r2_mode=r2_mode)
+    t = e.table(A, bc, N)
+    out = e.alloc(total * 4)
+    ref_out = e.alloc(total * 4)
+    per_window = e.plan_windows(t, wins, ref_out, qld_amd.BATCH_PERSISTENT)
+    ref_out.fill(0x7FC00000)
+    per_window.run()
+    e.sync()
+    want_all = ref_out.download(total, np.float32)
+    shared = e.plan_windows(t, wins, out, qld_amd.BATCH_SHARED)
+    st = shared.stats()
+    say("shared plan", r2_mode, st, "per-window", per_window.stats())
+    assert st["pairs"] == per_window.stats()["pairs"] and st["launches"] == 1
+    for use_graph in (True, False):
+        out.fill(0x7FC00000)
+        shared.use_graph(use_graph)
+        shared.run()
+        e.sync()
+        got_all = out.download(total, np.float32)
+        for w in wins:
+            s, n, off = int(w["start"]), int(w["size"]), int(w["out_offset"])
+            if n < 2:
+                continue
+            sel = np.triu(np.ones((n, n), dtype=bool), 1)
+            g = got_all[off:off + n * n].reshape(n, n)[sel].view(np.uint32)
+            r = want_all[off:off + n * n].reshape(n, n)[sel].view(np.uint32)
+            assert (g == r).all(), f"window start={s} size={n} differs from the per-window plan"
+        _check_windows(got_all, wins, A, bc, N, r2_mode)
+    for b in (shared, per_window, t, out, ref_out):
+        b.free()
+
+
+def test_shared_pair_plan_validation(engine_factory):
+    """Shared-pair plans take single-region windows of a direct fp4 table."""
+    A, bc = qld_amd.synth_blocks(5, 300, 1000)
+    e = engine_factory(engine="mfma_fp4")
+    t = e.table(A, bc, 1000)
+    out = e.alloc(4 * 300 * 300)
+    w = np.zeros(1, dtype=qld_amd.PAIR_WINDOW_DTYPE)
+    w[0] = (0, 10, 20, 10, 0, 0, 0)                   # a rectangle
+    with pytest.raises(qld_amd.QldError) as x:
+        e.plan_windows(t, w, out, qld_amd.BATCH_SHARED)
+    assert x.value.code == qld_amd.QLD_ERR_INVALID
+    t.free()
+    for engine in ("mfma_fp4x", "mfma_i8"):            # expanded operands
+        e2 = engine_factory(engine=engine)
+        t2 = e2.table(A, bc, 1000)
+        with pytest.raises(qld_amd.QldError) as x:
+            e2.plan_windows(t2, qld_amd.window_layout([0], [100]), out,
+                            qld_amd.BATCH_SHARED)
+        assert x.value.code == qld_amd.QLD_ERR_INVALID
+        t2.free()
+    out.free()
