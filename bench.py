@@ -291,8 +291,9 @@ def union_pairs(starts, sizes, rows):
 
 def windows_bench(args, steps):
     """4,096 windows, sizes round(exp(U(ln 50, ln 2000))), starts uniform,
-    over a 500,000-SNP x 16,622-sample chromosome: one persistent launch per
-    tile shape (graph-replayed) against one launch per window."""
+    over a 500,000-SNP x 16,622-sample chromosome: one persistent launch
+    (graph-replayed; the autotuner picks per-window or shared-pair work)
+    against one launch per window."""
     import qld_amd
     N, rows, nwin = args.samples, args.win_rows, args.windows
     seed = 20261203 + 4
@@ -315,6 +316,7 @@ def windows_bench(args, steps):
         cutoff = args.win_cutoff
         eng.set_tile_cutoff(cutoff)
         eng.set_plan_schedule(args.win_sched)
+        eng.set_plan_shared(args.win_shared)
     if args.no_graph:
         eng.set_plan_graphs(False)
     sched = eng.plan_schedule
@@ -335,27 +337,44 @@ def windows_bench(args, steps):
         return eng.timer_stop() / k
 
     graphs = not args.no_graph
-    batched = eng.plan_windows(t, wins, out, qld_amd.BATCH_PERSISTENT)
-    ms_b = timed(batched, graphs, steps)
-    st = batched.stats()
-    kms = []
-    for _ in range(3):
-        batched.run()
-        kms.append(eng.last_contract_ms())
-    k_s = float(np.median(kms)) / 1e3
-    batched.free()
-    # Shared-pair plan: each pair some window holds is counted once, over the
-    # chromosome's triangle, and its r2 written into every window holding it.
-    shared = eng.plan_windows(t, wins, out, qld_amd.BATCH_SHARED)
-    ms_s = timed(shared, graphs, steps)
-    sst = shared.stats()
-    skms = []
-    for _ in range(3):
-        shared.run()
-        skms.append(eng.last_contract_ms())
-    ks_s = float(np.median(skms)) / 1e3
-    shared.free()
+    shared_chosen = eng.plan_shared
+    res["plan"] = "shared-pair" if shared_chosen else "per-window"
+
+    def measure(mode):
+        plan = eng.plan_windows(t, wins, out, mode)
+        ms = timed(plan, graphs, steps)
+        st = plan.stats()
+        kms = []
+        for _ in range(3):
+            plan.run()
+            kms.append(eng.last_contract_ms())
+        plan.free()
+        return ms, st, float(np.median(kms)) / 1e3
+
+    # Per-window plan: every window's own wave blocks (the autotuned cutoff
+    # and schedule).  Shared-pair plan: each pair some window holds counted
+    # once over the chromosome's triangle, its r2 written into every window
+    # holding it.
+    eng.set_plan_shared(False)
+    ms_w, st_w, k_w = measure(qld_amd.BATCH_PERSISTENT)
+    ms_s, st_s, k_s_sh = measure(qld_amd.BATCH_SHARED)
+    eng.set_plan_shared(shared_chosen)
     upairs = union_pairs(starts, sizes, rows)
+    traffic_ok = (nwin, rows, N) == (4096, 500000, 16622)
+    per_window = {
+        "ms": round(ms_w, 3), "gpairs_s": round(pairs / ms_w / 1e6, 2),
+        "launches": st_w["launches"], "tiles": st_w["tiles"],
+        "roofline": roofline("mfma_fp4", pairs, (N + 63) // 64, N, k_w,
+                             measured_traffic(f"windows{nwin}", args.r2_mode)
+                             if traffic_ok else None, rows=int(sizes.sum()))}
+    shared = {
+        "ms": round(ms_s, 3), "gpairs_s": round(pairs / ms_s / 1e6, 2),
+        "units": st_s["tiles"], "launches": st_s["launches"], "distinct_pairs": upairs,
+        "roofline": roofline("mfma_fp4", upairs, (N + 63) // 64, N, k_s_sh,
+                             measured_traffic(f"windows{nwin}_shared", args.r2_mode)
+                             if traffic_ok else None, rows=int(sizes.sum()))}
+    ms_b = ms_s if shared_chosen else ms_w
+    st = st_s if shared_chosen else st_w
     # The baseline, one launch per window, on 128-tiles: a single window's
     # launch needs the finer tiles to fill the GPU (its fastest setting).
     ms_pg = ms_pd = None
@@ -369,22 +388,10 @@ def windows_bench(args, steps):
     out.free()
     eng.close()
     res.update({"batched_ms": round(ms_b, 3), "batched_gpairs_s": round(pairs / ms_b / 1e6, 2),
-                "batched_launches": st["launches"], "tiles": st["tiles"],
-                "graphs": graphs,
-                "roofline": roofline("mfma_fp4", pairs, (N + 63) // 64, N, k_s,
-                                     measured_traffic(f"windows{nwin}", args.r2_mode)
-                                     if (nwin, rows, N) == (4096, 500000, 16622) else None,
-                                     rows=int(sizes.sum()))})
-    res["shared"] = {
-        "ms": round(ms_s, 3), "gpairs_s": round(pairs / ms_s / 1e6, 2),
-        "units": sst["tiles"], "launches": sst["launches"],
-        "distinct_pairs": upairs,
-        "roofline": roofline("mfma_fp4", upairs, (N + 63) // 64, N, ks_s,
-                             measured_traffic(f"windows{nwin}_shared", args.r2_mode)
-                             if (nwin, rows, N) == (4096, 500000, 16622) else None,
-                             rows=int(sizes.sum()))}
+                "batched_launches": st["launches"], "graphs": graphs,
+                "roofline": (shared if shared_chosen else per_window)["roofline"],
+                "per_window": per_window, "shared": shared})
     if ms_pd is not None:
-        res["shared"]["speedup_vs_plain"] = round(ms_pd / ms_s, 2)
         res.update({"plain_direct_ms": round(ms_pd, 3),
                     "plain_direct_gpairs_s": round(pairs / ms_pd / 1e6, 2),
                     "speedup_vs_plain": round(ms_pd / ms_b, 2)})
@@ -392,7 +399,8 @@ def windows_bench(args, steps):
         res.update({"plain_graph_ms": round(ms_pg, 3),
                     "plain_graph_gpairs_s": round(pairs / ms_pg / 1e6, 2),
                     "speedup_vs_plain_graph": round(ms_pg / ms_b, 2)})
-    log(f"[bench] windows: shared-pair plan {ms_s:.3f} ms ({upairs} distinct pairs)")
+    log(f"[bench] windows: per-window plan {ms_w:.3f} ms, shared-pair plan {ms_s:.3f} ms "
+        f"({upairs} distinct pairs); the autotuner chose {res['plan']}")
     log(f"[bench] windows: batched {ms_b:.3f} ms, plain graph {ms_pg} ms, "
         f"plain direct {ms_pd} ms (cutoff {cutoff}, schedule {sched}, graphs {graphs})")
     return res
@@ -728,6 +736,8 @@ def main():
                     help="window plans: tile cutoff to use instead of autotuning")
     ap.add_argument("--win-sched", default="waves", choices=["waves", "tiles"],
                     help="window plans: work schedule with --win-cutoff")
+    ap.add_argument("--win-shared", action="store_true",
+                    help="with --win-cutoff: the batched plan is the shared-pair form")
     ap.add_argument("--no-graph", action="store_true",
                     help="window plans: direct launches, no HIP graph capture")
     ap.add_argument("--no-plain", action="store_true",

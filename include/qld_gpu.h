@@ -327,12 +327,18 @@ int qld_get_plan_schedule(const qld_ctx *ctx);
  * 256 x 256 tiles, smaller ones 128 x 128 (0 = all large). */
 int qld_set_tile_cutoff(qld_ctx *ctx, int cutoff);
 int qld_get_tile_cutoff(const qld_ctx *ctx);
+/* Whether persistent plans (QLD_BATCH_PERSISTENT) of single-region windows
+ * on a direct fp4 table take the shared-pair form of QLD_BATCH_SHARED
+ * (default 0; the autotuner sets it when that form is faster). */
+int qld_set_plan_shared(qld_ctx *ctx, int shared);
+int qld_get_plan_shared(const qld_ctx *ctx);
 /* Autotuner: time the given window set as persistent plans under each
  * candidate cutoff -- and, for tables with direct fp4 operands, under both
- * work schedules -- (`reps` runs each after a warm-up), keep the fastest
- * cutoff and schedule in the context and report the cutoff
- * (qld_get_plan_schedule gives the schedule).  Writes r^2 into `out` as a
- * plan run does. */
+ * work schedules and in the shared-pair form -- (`reps` runs each after a
+ * warm-up), keep the fastest cutoff and schedule of the per-window forms
+ * and whether the shared-pair form beat them in the context, and report the
+ * cutoff (qld_get_plan_schedule and qld_get_plan_shared give the rest).
+ * Writes r^2 into `out` as a plan run does. */
 int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_window *wins,
                          int nwin, float *out, int reps, int *best_cutoff,
                          float *best_ms);

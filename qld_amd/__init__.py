@@ -72,6 +72,8 @@ GPU_SYMBOLS = [
     ("qld_last_kernel_ms", C.c_int, [_vp, _f32p]),
     ("qld_set_plan_schedule", C.c_int, [_vp, C.c_int]),
     ("qld_get_plan_schedule", C.c_int, [_vp]),
+    ("qld_set_plan_shared", C.c_int, [_vp, C.c_int]),
+    ("qld_get_plan_shared", C.c_int, [_vp]),
     ("qld_set_plan_graphs", C.c_int, [_vp, C.c_int]),
     ("qld_last_contract_ms", C.c_int, [_vp, _f32p]),
     ("qld_timer_start", C.c_int, [_vp]),
@@ -674,6 +676,15 @@ class Engine:
         """'waves': every wave claims wave blocks of its own (default);
         'tiles': a workgroup claims a whole tile."""
         self._check(hip_lib().qld_set_plan_schedule(self._ctx, {"tiles": 0, "waves": 1}[sched]))
+
+    def set_plan_shared(self, shared: bool):
+        """Whether persistent plans of single-region windows count each pair
+        once (the QLD_BATCH_SHARED form); the autotuner sets it."""
+        self._check(hip_lib().qld_set_plan_shared(self._ctx, 1 if shared else 0))
+
+    @property
+    def plan_shared(self) -> bool:
+        return hip_lib().qld_get_plan_shared(self._ctx) == 1
 
     def set_plan_graphs(self, capture: bool):
         """Whether plans built afterwards are captured in a HIP graph."""

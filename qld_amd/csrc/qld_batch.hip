@@ -265,10 +265,20 @@ static int pair_tile(const qld_ctx *ctx, const qld_table *t, const PairWin &w)
 // Shared-pair plan: the 128 x 64 blocks of the table's triangle that hold a
 // cell of some window, each listed once with the windows it serves, in row
 // band order (a workgroup's four waves claim four consecutive blocks, mostly
-// of one band, so they share their rows).  A window [s, e) holds cells of
+// of one band, so they share their rows), diagonal blocks grouped.  A window [s, e) holds cells of
 // block (bi, bj) iff some row r in [s, e - 1) of the band has a column in
 // (r, e) inside the block: bi from s/128 to (e-2)/128, and for each, bj from
 // (r0 + 1)/64 to (e - 1)/64 with r0 the band's first row inside the window.
+static bool shared_plan_fits(const qld_table *t, const std::vector<PairWin> &wins)
+{
+    int wtm = 0, wtn = 0;
+    wave_block_shape(QLD_TILE_LARGE, &wtm, &wtn);
+    if (!t->direct || (int64_t)t->rows > 65535LL * wtn) return false;
+    for (const PairWin &w : wins)
+        if (!w.tri) return false;
+    return true;
+}
+
 static int build_shared_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWin> &wins,
                              float *out, qld_plan *p)
 {
@@ -315,6 +325,17 @@ static int build_shared_plan(qld_ctx *ctx, const qld_table *t, const std::vector
         a = b;
     }
     if (items.empty()) return QLD_OK;
+    // A diagonal block computes 3 of its 8 MFMA blocks; a workgroup's four
+    // waves wait for the slowest block of their strip, so within every run
+    // of 8 bands the diagonal blocks go last, together.
+    for (size_t a = 0; a < items.size();) {
+        size_t b = a;
+        while (b < items.size() && items[b].ti / 8 == items[a].ti / 8) ++b;
+        std::stable_partition(items.begin() + a, items.begin() + b, [&](const WorkItem &u) {
+            return (int)u.tj * wtn != (int)u.ti * wtm;
+        });
+        a = b;
+    }
     PlanClass c;
     c.tile = QLD_TILE_LARGE;
     c.wave_units = true;
@@ -391,6 +412,9 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
     };
 
     int rc = QLD_OK;
+    if (mode == QLD_BATCH_PERSISTENT && ctx->plan_shared && shared_plan_fits(t, wins))
+        mode = QLD_BATCH_SHARED;
+    p->mode = mode;
     if (mode == QLD_BATCH_SHARED) {
         if ((rc = build_shared_plan(ctx, t, wins, out, p))) {
             plan_release(ctx, p);
@@ -597,6 +621,18 @@ extern "C" int qld_set_plan_graphs(qld_ctx *ctx, int capture)
     return QLD_OK;
 }
 
+extern "C" int qld_set_plan_shared(qld_ctx *ctx, int shared)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    ctx->plan_shared = shared != 0;
+    return QLD_OK;
+}
+
+extern "C" int qld_get_plan_shared(const qld_ctx *ctx)
+{
+    return ctx ? (int)ctx->plan_shared : -1;
+}
+
 extern "C" int qld_get_plan_schedule(const qld_ctx *ctx)
 {
     return ctx ? ctx->plan_sched : -1;
@@ -646,6 +682,8 @@ extern "C" int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_
     };
     std::vector<Cand> cands;
     int rc = QLD_OK;
+    const bool saved_shared = ctx->plan_shared;
+    ctx->plan_shared = false;          // the per-window candidates first
     const int nsched = t->direct ? 2 : 1;
     for (int si = 0; si < nsched && rc == QLD_OK; ++si) {
         ctx->plan_sched = t->direct ? (si ? QLD_SCHED_TILES : QLD_SCHED_WAVES) : saved_sched;
@@ -654,6 +692,20 @@ extern "C" int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_
             qld_plan *p = nullptr;
             if ((rc = qld_plan_windows(ctx, t, wins, nwin, out, QLD_BATCH_PERSISTENT, &p))) break;
             cands.push_back(Cand{ctx->plan_sched, cut, p, 3.4e38f});
+        }
+    }
+    // and the shared-pair form, where it applies (its tile shape is fixed)
+    ctx->plan_sched = saved_sched;
+    ctx->tile_cutoff = saved;
+    if (rc == QLD_OK) {
+        std::vector<PairWin> pw((size_t)nwin);
+        for (int k = 0; k < nwin; ++k)
+            pw[k] = PairWin{wins[k].start, wins[k].size, wins[k].start, wins[k].size,
+                            wins[k].out_offset, 1};
+        if (shared_plan_fits(t, pw)) {
+            qld_plan *p = nullptr;
+            rc = qld_plan_windows(ctx, t, wins, nwin, out, QLD_BATCH_SHARED, &p);
+            if (rc == QLD_OK) cands.push_back(Cand{-1, -1, p, 3.4e38f});
         }
     }
     for (int w = 0; w < 3 * reps && rc == QLD_OK && !cands.empty(); ++w)
@@ -670,18 +722,34 @@ extern "C" int qld_autotune_windows(qld_ctx *ctx, const qld_table *t, const qld_
                 c.ms = std::min(c.ms, ms / reps);
             if (rc) break;
         }
+    bool best_shared = saved_shared;
     for (auto &c : cands) {
         if (rc == QLD_OK && c.ms < best_t) {
             best_t = c.ms;
-            best = c.cut;
-            best_sched = c.sched;
+            best_shared = c.cut < 0;
+            if (!best_shared) {
+                best = c.cut;
+                best_sched = c.sched;
+            }
         }
         qld_plan_free(ctx, c.p);
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    // the per-window settings stay those of the fastest per-window candidate
+    // (they serve window sets the shared form cannot take)
+    if (rc == QLD_OK && best_shared) {
+        float bt = 3.4e38f;
+        for (auto &c : cands)
+            if (c.cut >= 0 && c.ms < bt) {
+                bt = c.ms;
+                best = c.cut;
+                best_sched = c.sched;
+            }
+    }
     ctx->tile_cutoff = rc == QLD_OK ? best : saved;
     ctx->plan_sched = rc == QLD_OK ? best_sched : saved_sched;
+    ctx->plan_shared = rc == QLD_OK ? best_shared : saved_shared;
     if (best_cutoff) *best_cutoff = ctx->tile_cutoff;
     if (best_ms) *best_ms = best_t;
     return rc;

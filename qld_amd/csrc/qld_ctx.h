@@ -45,6 +45,8 @@ struct qld_ctx {
     void *scratch = nullptr;  // device address of this device's static scratch array
     int plan_sched = QLD_SCHED_WAVES;  // persistent plans: per-wave units or whole tiles
     bool plan_graphs = true;  // capture each plan's launches in a HIP graph
+    bool plan_shared = false; // persistent plans of single-region windows count
+                              // each pair once (QLD_BATCH_SHARED form)
     char err[512] = {0};
 };
 

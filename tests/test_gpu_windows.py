@@ -357,6 +357,22 @@ def test_shared_pair_plan_validation(engine_factory):
     with pytest.raises(qld_amd.QldError) as x:
         e.plan_windows(t, w, out, qld_amd.BATCH_SHARED)
     assert x.value.code == qld_amd.QLD_ERR_INVALID
+    # with the shared form preferred, a persistent plan of pair windows that
+    # include a rectangle keeps the per-window form, and stays correct
+    e.set_plan_shared(True)
+    assert e.plan_shared
+    w2 = np.zeros(2, dtype=qld_amd.PAIR_WINDOW_DTYPE)
+    w2[0] = (0, 100, 150, 120, 0, 0, 0)
+    w2[1] = (40, 200, 40, 200, 100 * 120, 1, 0)
+    out.fill(0x7FC00000)
+    plan = e.plan_windows(t, w2, out, qld_amd.BATCH_PERSISTENT)
+    plan.run()
+    e.sync()
+    got = out.download(100 * 120 + 200 * 200, np.float32)
+    want = oracle.r2_reference(bc[0:100], bc[150:270], oracle.counts_naive(A[0:100], A[150:270]),
+                               1000)
+    assert (got[:12000].reshape(120, 100).view(np.uint32) == want.view(np.uint32)).all()
+    plan.free()
     t.free()
     for engine in ("mfma_fp4x", "mfma_i8"):            # expanded operands
         e2 = engine_factory(engine=engine)
