@@ -64,7 +64,9 @@ def measured_traffic(workload, r2_mode):
     return None
 
 
-def roofline(engine, pairs, words, samples, kernel_s, traffic=None, rows=None):
+def roofline(engine, pairs, words, samples, kernel_s, traffic=None, rows=None, out_pairs=None):
+    """pairs: the pairs the kernel computes (the flop count); out_pairs: the
+    r2 values it writes when that differs (shared-pair plans)."""
     if engine == "popcount":
         ops = pairs * 4.0 * words          # 2x v_and_b32 + 2x v_bcnt per word pair
         ach = ops / kernel_s
@@ -85,7 +87,7 @@ def roofline(engine, pairs, words, samples, kernel_s, traffic=None, rows=None):
             # each operand row read once (fp4: 4 bits per sample, padded to the
             # 64-sample word) + 4 B of r2 written per pair
             opb = {"mfma_i8": 64}.get(engine, 8)   # fp4 reads the packed rows
-            out["algorithmic_bytes"] = rows * words * opb + pairs * 4
+            out["algorithmic_bytes"] = rows * words * opb + (out_pairs or pairs) * 4
     return out
 
 
@@ -360,6 +362,10 @@ def windows_bench(args, steps):
     ms_s, st_s, k_s_sh = measure(qld_amd.BATCH_SHARED)
     eng.set_plan_shared(shared_chosen)
     upairs = union_pairs(starts, sizes, rows)
+    cover = np.zeros(rows + 1, np.int64)
+    np.add.at(cover, starts, 1)
+    np.add.at(cover, starts + sizes, -1)
+    urows = int((np.cumsum(cover)[:rows] > 0).sum())     # rows some window holds
     traffic_ok = (nwin, rows, N) == (4096, 500000, 16622)
     per_window = {
         "ms": round(ms_w, 3), "gpairs_s": round(pairs / ms_w / 1e6, 2),
@@ -372,7 +378,7 @@ def windows_bench(args, steps):
         "units": st_s["tiles"], "launches": st_s["launches"], "distinct_pairs": upairs,
         "roofline": roofline("mfma_fp4", upairs, (N + 63) // 64, N, k_s_sh,
                              measured_traffic(f"windows{nwin}_shared", args.r2_mode)
-                             if traffic_ok else None, rows=int(sizes.sum()))}
+                             if traffic_ok else None, rows=urows, out_pairs=pairs)}
     ms_b = ms_s if shared_chosen else ms_w
     st = st_s if shared_chosen else st_w
     # The baseline, one launch per window, on 128-tiles: a single window's
