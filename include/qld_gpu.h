@@ -87,9 +87,13 @@ const char *qld_gpu_last_error(const qld_ctx *ctx);
  * buffers, synchronous.  A: m x words, B: n x words (B may equal A);
  * triangular requires A == B content and m == n.  results (n*m floats) and
  * counts (n*m uint32) are each optional (NULL = not wanted).  snp_size is
- * the N of the frequencies, 1 .. 2^24 - 1: the header's sample count, as
- * correlate_function_gpu passes it (correlate_function_gpu.c:240-275), which
- * exceeds the rows' bits when a sample list selects fewer samples. */
+ * the N of the frequencies, any positive int as in the reference: the
+ * header's sample count, as correlate_function_gpu passes it
+ * (correlate_function_gpu.c:240-275), which exceeds the rows' bits when a
+ * sample list selects fewer samples.  The MFMA engines take N < 2^24 and
+ * rows of at most 2^18 words (their fp32 sums are exact below 2^24); larger
+ * jobs run on the popcount engine (uint32 counts, rows of up to 2^25 words),
+ * which AUTO picks for them, and an explicit MFMA engine rejects them. */
 int qld_gpu_correlate(qld_ctx *ctx,
                       const uint64_t *A, const uint32_t *bcA, int m,
                       const uint64_t *B, const uint32_t *bcB, int n,
@@ -188,7 +192,8 @@ typedef struct qld_table qld_table;
 /* Copy `rows` packed rows and their bit counts from host memory into device
  * memory, in the layout the context's engine reads: a row-block-tiled copy
  * of the same size for the default fp4 direct operands, the packed rows plus
- * an expanded fp4 / int8 copy otherwise.  The caller's rows are not kept. */
+ * an expanded fp4 / int8 copy otherwise.  The caller's rows are not kept.
+ * Tables feed the MFMA engines: words 1 .. 2^18, snp_size 1 .. 2^24 - 1. */
 int qld_table_create(qld_ctx *ctx, const uint64_t *rows_host,
                      const uint32_t *bc_host, int rows, int words,
                      int snp_size, qld_table **out);

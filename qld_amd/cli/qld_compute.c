@@ -691,7 +691,8 @@ static void run_batch(qld_ctx *ctx, const options *o, task *const *bt, loaded *L
 
 /* Tasks in list order; consecutive tasks with the same packing go into one
  * batch until -batch-cells r^2 cells (2^29 = 2 GiB by default); a task above
- * a quarter of that runs alone through the streamed drop-in call. */
+ * a quarter of that, or with 2^24 samples or more, runs alone through the
+ * streamed drop-in call. */
 static void run_list(qld_ctx *ctx, const options *o, task *tasks, int ntask)
 {
     task **bt = (task **)xmalloc((size_t)ntask * sizeof(task *));
@@ -704,7 +705,11 @@ static void run_list(qld_ctx *ctx, const options *o, task *tasks, int ntask)
         int flush = i == ntask;
         if (!flush) {
             load_task(o, &tasks[i], &cur);
-            if (task_cells(&cur) > o->batch_cells / 4) {
+            /* resident tables feed the MFMA engines: a sample count of 2^24
+             * or more (or rows of more than 2^18 words) runs alone, through
+             * the streamed call and the popcount engine */
+            if (task_cells(&cur) > o->batch_cells / 4 || tasks[i].h1.snip_size >= (1 << 24) ||
+                cur.A.words > (1 << 18)) {
                 /* every task's tables are read once and in list order: a
                  * VCF read draws the unphased shuffles from the run's
                  * rand() stream */

@@ -116,6 +116,21 @@ QLD_EPI float r2_reference(float c, const RefTerm &a, const RefTerm &b, double i
     return slow ? r2_reference_div(c, a.p, b.p, invN) : r;
 }
 
+// N >= 2^24 (the popcount engine): the reference's operations with p12
+// formed as it forms it, the count rounded to float and divided by the
+// rounded N (c/N may then be a float midpoint, and c and N inexact).
+QLD_EPI float r2_reference_wide(uint32_t c, float p1, float p2, float fN)
+{
+    const float p12 = (float)c / fN;
+    const float x = p1 * p2;
+    const float t = p12 - x;
+    const float num = t * t;
+    const double den = ((double)x * (1.0 - (double)p1)) * (1.0 - (double)p2);
+    float r = (float)((double)num / den);
+    r = (r > QLD_R2_CLAMP_F) ? QLD_R2_FLAG_F : r;
+    return (p1 == 0.0f || p2 == 0.0f) ? 0.0f : r;
+}
+
 // Per-SNP terms of the EXACT epilogue.
 struct ExactTerm {
     double bc;    // bit count
@@ -138,6 +153,19 @@ QLD_EPI float r2_exact(float c, const ExactTerm &a, const ExactTerm &b, double d
 {
     const double d = __builtin_fma((double)c, dN, -(a.bc * b.bc));
     const double q = ((d * d) * a.rv) * b.rv;
+    float r = (float)q;
+    r = (r > QLD_R2_CLAMP_F) ? QLD_R2_FLAG_F : r;
+    return (a.zero || b.zero) ? 0.0f : r;
+}
+
+// N >= 2^24: d = c*N - a*b in 64-bit integers (|d| < 2^62), rounded once to
+// double; ra and rb are then within one double rounding of 1/(a(N-a)) and
+// 1/(b(N-b)), so r^2 stays within one float ulp of the FP64 closed form.
+QLD_EPI float r2_exact_wide(uint32_t c, const ExactTerm &a, const ExactTerm &b, int64_t N)
+{
+    const int64_t d = (int64_t)c * N - (int64_t)a.bc * (int64_t)b.bc;
+    const double dd = (double)d;
+    const double q = ((dd * dd) * a.rv) * b.rv;
     float r = (float)q;
     r = (r > QLD_R2_CLAMP_F) ? QLD_R2_FLAG_F : r;
     return (a.zero || b.zero) ? 0.0f : r;

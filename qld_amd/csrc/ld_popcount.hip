@@ -14,7 +14,9 @@
 // LDS holds the stage word-major (sX[k][row]) so a wave reads 16 distinct
 // consecutive words of A (conflict-free) and 4 broadcast words of B.
 // Per word pair the lane issues 2 v_and_b32 + 2 v_bcnt_u32_b32 (the bcnt
-// accumulates), the minimum for 64-bit words on CDNA.
+// accumulates), the minimum for 64-bit words on CDNA.  Counts are uint32, so
+// this engine also takes the jobs beyond the MFMA engines' exact range
+// (N >= 2^24, rows wider than 2^18 words), with the wide-N epilogue forms.
 #include "qld_internal.h"
 #include "ld_epilogue.h"
 #include "ld_tiles.h"
@@ -127,7 +129,8 @@ __global__ __launch_bounds__(PTHREADS) void ld_popcount_kernel(PopArgs p)
                 if (j >= J.m || (J.tri && j <= i)) continue;
                 if (J.res)
                     J.res[(size_t)i * J.ldr + j] =
-                        r2_reference((float)acc[r][s], ta[s], tb[r], p.ec.invN);
+                        p.ec.wide ? r2_reference_wide(acc[r][s], ta[s].p, tb[r].p, p.ec.fN)
+                                  : r2_reference((float)acc[r][s], ta[s], tb[r], p.ec.invN);
                 if (J.cnt) J.cnt[(size_t)i * J.ldc + j] = acc[r][s];
             }
         }
@@ -153,7 +156,8 @@ __global__ __launch_bounds__(PTHREADS) void ld_popcount_kernel(PopArgs p)
                 if (j >= J.m || (J.tri && j <= i)) continue;
                 if (J.res)
                     J.res[(size_t)i * J.ldr + j] =
-                        r2_exact((float)acc[r][s], ta[s], tb[r], p.ec.dN);
+                        p.ec.wide ? r2_exact_wide(acc[r][s], ta[s], tb[r], p.ec.N)
+                                  : r2_exact((float)acc[r][s], ta[s], tb[r], p.ec.dN);
                 if (J.cnt) J.cnt[(size_t)i * J.ldc + j] = acc[r][s];
             }
         }
@@ -167,6 +171,7 @@ hipError_t launch_popcount(const LdJob &job, int r2_mode, hipStream_t s)
     PopArgs p;
     p.job = job;
     p.ec = make_epi_const(job.snp_size);
+    if (job.words > QLD_MAX_WORDS) p.ec.wide = 1;   // counts may pass 2^24 too
     p.map = make_tile_map(job.n, job.m, PT, PT, job.tri);
     if (p.map.total == 0) return hipSuccess;
     dim3 grid((unsigned)p.map.total), block(PTHREADS);

@@ -72,8 +72,9 @@ static int table_create(qld_ctx *ctx, const void *rows_src, const void *bc_src,
 {
     if (!ctx || !out || !rows_src || !bc_src || rows <= 0 || words <= 0 ||
         words > QLD_MAX_WORDS || snp_size <= 0 || snp_size >= QLD_MAX_SNP_SIZE) {
-        if (ctx) qld_set_error(ctx, "qld_table_create: bad arguments (rows=%d words=%d N=%d)",
-                               rows, words, snp_size);
+        if (ctx) qld_set_error(ctx, "qld_table_create: bad arguments (rows=%d words=%d N=%d; "
+                               "resident tables feed the MFMA engines: words 1 .. %d, N 1 .. %d)",
+                               rows, words, snp_size, QLD_MAX_WORDS, QLD_MAX_SNP_SIZE - 1);
         return QLD_ERR_INVALID;
     }
     *out = nullptr;

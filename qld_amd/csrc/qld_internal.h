@@ -7,13 +7,22 @@
 
 #include "../../include/qld_gpu.h"
 
-// Largest sample count the engine accepts: counts must stay exact in the
-// fp32 accumulators of the MFMA engines and c/N must round like the
-// reference's float division (see ld_epilogue.h).
+// Largest sample count the MFMA engines accept: counts must stay exact in
+// their fp32 accumulators and c/N must round like the reference's float
+// division (see ld_epilogue.h).
 #define QLD_MAX_SNP_SIZE (1 << 24)
-// Largest row width in packed words: a count is at most 64 * words, and
-// fp32 sums hold every integer up to 2^24 exactly.
+// Largest row width of the MFMA engines in packed words: a count is at most
+// 64 * words, and fp32 sums hold every integer up to 2^24 exactly.
 #define QLD_MAX_WORDS (1 << 18)
+// The popcount engine counts in uint32 and takes any sample count an int
+// holds (the reference's snp_size is an int) and rows of up to 2^25 words
+// (counts up to 2^31); jobs beyond the MFMA bounds ("wide" jobs) run there.
+#define QLD_MAX_WORDS_POPCOUNT (1 << 25)
+
+static inline bool qld_wide_job(int words, int snp_size)
+{
+    return words > QLD_MAX_WORDS || snp_size >= QLD_MAX_SNP_SIZE;
+}
 
 // Tile edge an expanded-operand launch may use: its buffer descriptors span
 // a tile's rows and address them with 32-bit offsets, so tile * row_bytes
@@ -47,6 +56,8 @@ struct EpiConst {
     float fN;       // (float)snp_size, as the reference divides by it
     double dN;      // (double)snp_size
     double invN;    // 1.0 / snp_size, correctly rounded
+    int64_t N;      // snp_size
+    int wide;       // snp_size >= 2^24: the popcount engine's wide-N forms
 };
 
 static inline EpiConst make_epi_const(int snp_size)
@@ -55,6 +66,8 @@ static inline EpiConst make_epi_const(int snp_size)
     e.fN = (float)snp_size;
     e.dN = (double)snp_size;
     e.invN = 1.0 / (double)snp_size;
+    e.N = snp_size;
+    e.wide = snp_size >= QLD_MAX_SNP_SIZE;
     return e;
 }
 

@@ -323,16 +323,25 @@ extern "C" int qld_sync(qld_ctx *ctx)
 
 static int check_job(qld_ctx *ctx, const LdJob &j)
 {
-    if (j.m < 0 || j.n < 0 || j.words <= 0 || j.words > QLD_MAX_WORDS) {
+    if (j.m < 0 || j.n < 0 || j.words <= 0 || j.words > QLD_MAX_WORDS_POPCOUNT) {
         qld_set_error(ctx, "bad shape m=%d n=%d words=%d (words 1 .. %d)", j.m, j.n, j.words,
-                      QLD_MAX_WORDS);
+                      QLD_MAX_WORDS_POPCOUNT);
         return QLD_ERR_INVALID;
     }
     // snp_size is the N of the frequencies (the header's sample count, as
     // the reference passes it: with a sample list the rows hold fewer bits
-    // than N).  It only has to keep the r^2 arithmetic exact.
-    if (j.snp_size <= 0 || j.snp_size >= QLD_MAX_SNP_SIZE) {
-        qld_set_error(ctx, "bad snp_size %d (1 .. %d)", j.snp_size, QLD_MAX_SNP_SIZE - 1);
+    // than N), any positive int as in the reference.
+    if (j.snp_size <= 0) {
+        qld_set_error(ctx, "bad snp_size %d", j.snp_size);
+        return QLD_ERR_INVALID;
+    }
+    const int eng = ctx->cfg.engine;
+    if (qld_wide_job(j.words, j.snp_size) && eng != QLD_ENGINE_AUTO &&
+        eng != QLD_ENGINE_POPCOUNT) {
+        qld_set_error(ctx, "snp_size %d with %d-word rows: counts can reach 2^24, beyond what "
+                      "the MFMA engines' fp32 sums hold exactly (snp_size below %d and at "
+                      "most %d words); use the popcount engine or AUTO", j.snp_size, j.words,
+                      QLD_MAX_SNP_SIZE, QLD_MAX_WORDS);
         return QLD_ERR_INVALID;
     }
     if (j.tri && j.m != j.n) {
@@ -359,10 +368,12 @@ int qld_check_job(qld_ctx *ctx, const LdJob &j) { return check_job(ctx, j); }
 // latency grows with the row width -- except on narrow rows (at most 16
 // words, N <= 1,024) with 0.5 to 8 million pairs, where the popcount engine
 // is about 10% faster (2,000 SNPs at 256 / 1,000 samples: 0.025 vs 0.028 ms).
-// The int8 engine is never the fastest.
+// The int8 engine is never the fastest.  Jobs beyond the MFMA engines' exact
+// range (N >= 2^24 or rows wider than 2^18 words) go to the popcount engine.
 int qld_pick_engine(const qld_ctx *ctx, const LdJob &j)
 {
     if (ctx->cfg.engine != QLD_ENGINE_AUTO) return ctx->cfg.engine;
+    if (qld_wide_job(j.words, j.snp_size)) return QLD_ENGINE_POPCOUNT;
     const double pairs = j.tri ? 0.5 * (double)j.m * (double)j.n : (double)j.m * (double)j.n;
     if (j.words <= 16 && pairs >= 5e5 && pairs <= 8e6) return QLD_ENGINE_POPCOUNT;
     return QLD_ENGINE_MFMA_FP4;

@@ -186,9 +186,10 @@ def test_full_size_sampled_rows(engine, engine_factory):
 @pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("mode", ["reference", "exact"])
 def test_largest_sample_count(engine, mode, engine_factory):
-    """N = 2^24 - 1, the largest sample count the engines accept (counts near
-    2^24 must stay exact in fp32 sums): dense, complementary, sparse and
-    all-one rows, a triangle and a rectangle; N = 2^24 is rejected."""
+    """N = 2^24 - 1, the largest sample count the MFMA engines accept (counts
+    near 2^24 must stay exact in fp32 sums): dense, complementary, sparse and
+    all-one rows, a triangle and a rectangle; N = 2^24 is rejected by the MFMA
+    engines and taken by the popcount engine."""
     N = (1 << 24) - 1
     W = (N + 63) // 64
     rng = np.random.default_rng(24)
@@ -209,10 +210,13 @@ def test_largest_sample_count(engine, mode, engine_factory):
     res, cnt = e.correlate(A[:2], bc[:2], A[2:], bc[2:], snp_size=N, triangular=False,
                            counts=True)
     _check(res, cnt, A[:2], bc[:2], A[2:], bc[2:], N, False, mode)
-    with pytest.raises(qld_amd.QldError) as x:
-        e.correlate(np.zeros((2, W + 1), np.uint64), np.zeros(2, np.uint32),
-                    snp_size=1 << 24)
-    assert x.value.code == qld_amd.QLD_ERR_INVALID
+    if engine == "popcount":        # takes any int sample count
+        e.correlate(np.zeros((2, W + 1), np.uint64), np.zeros(2, np.uint32), snp_size=1 << 24)
+    else:
+        with pytest.raises(qld_amd.QldError) as x:
+            e.correlate(np.zeros((2, W + 1), np.uint64), np.zeros(2, np.uint32),
+                        snp_size=1 << 24)
+        assert x.value.code == qld_amd.QLD_ERR_INVALID
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -303,10 +307,13 @@ def test_errors_are_status_codes(engine_factory):
     with pytest.raises(qld_amd.QldError) as x:
         e.correlate(A, bc, B, bB, snp_size=100, triangular=True)
     assert x.value.code == qld_amd.QLD_ERR_INVALID
-    for bad_n in (0, -5, 1 << 24):
+    for bad_n in (0, -5):
         with pytest.raises(qld_amd.QldError) as x:
             e.correlate(A, bc, snp_size=bad_n)
         assert x.value.code == qld_amd.QLD_ERR_INVALID
+    with pytest.raises(qld_amd.QldError) as x:     # beyond the MFMA engines' exact range
+        engine_factory(engine="mfma_fp4").correlate(A, bc, snp_size=1 << 24)
+    assert x.value.code == qld_amd.QLD_ERR_INVALID
     small = engine_factory(engine="popcount", hbm_budget=1 << 10)
     with pytest.raises(qld_amd.QldError) as x:
         small.alloc(1 << 20)
@@ -345,7 +352,8 @@ def test_widest_rows_many_rows(engine, engine_factory):
     of 8 MiB (fp4) / 16 MiB (int8), whose 256-row tiles would span 2^32
     bytes -- such launches drop to 128-tiles.  Triangle with either tile
     edge requested, a rectangle, and a window plan with 256-tiles asked for;
-    every count against the oracle.  A width of 262,145 words is rejected."""
+    every count against the oracle.  A width of 262,145 words is rejected by
+    the MFMA engines (the popcount engine takes it)."""
     N = (1 << 24) - 1
     W = (N + 63) // 64
     rows = 300
@@ -388,6 +396,46 @@ def test_widest_rows_many_rows(engine, engine_factory):
         plan.free()
         t.free()
         out.free()
-    with pytest.raises(qld_amd.QldError) as x:
+    if engine == "popcount":        # uint32 counts: wider rows are taken
         e.correlate(np.zeros((2, W + 1), np.uint64), np.zeros(2, np.uint32), snp_size=1000)
-    assert x.value.code == qld_amd.QLD_ERR_INVALID
+    else:
+        with pytest.raises(qld_amd.QldError) as x:
+            e.correlate(np.zeros((2, W + 1), np.uint64), np.zeros(2, np.uint32), snp_size=1000)
+        assert x.value.code == qld_amd.QLD_ERR_INVALID
+
+
+@pytest.mark.parametrize("engine", ["popcount", "auto"])
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_wide_sample_counts(engine, mode, engine_factory):
+    """Sample counts from 2^24 up, as the reference takes any int snp_size:
+    they run on the popcount engine (AUTO picks it, uint32 counts, the
+    reference's float casts of c and N in REFERENCE mode, 64-bit integer
+    numerators in EXACT mode).  Rows of exactly N bits at N = 2^24 and
+    2^24 + 12,345, rows of 2^26 bits (2^20 words, wider than the MFMA
+    engines' 2^18), and N = 2^30 + 3 over rows of 5,000 bits (a sample list
+    far below the header's count); all-one, dense, complementary, sparse
+    rows; a triangle and a rectangle, counts and r^2 bit-exact against the
+    oracle (pinned for these N by tests/golden/r2_wide.npz)."""
+    rng = np.random.default_rng(26)
+    for N, nbits in [(1 << 24, 1 << 24), ((1 << 24) + 12345, (1 << 24) + 12345),
+                     (1 << 26, 1 << 26), ((1 << 30) + 3, 5000)]:
+        W = (nbits + 63) // 64
+        bits = np.zeros((6, W * 64), dtype=np.uint8)
+        bits[0, :nbits] = 1
+        bits[1, :nbits] = rng.random(nbits) < 0.9
+        bits[2, :nbits] = 1 - bits[1, :nbits]
+        bits[3, :nbits] = bits[1, :nbits]
+        bits[3, rng.integers(0, nbits, 1000)] ^= 1
+        bits[4, :nbits] = rng.random(nbits) < 0.001
+        bits[5, :nbits] = rng.random(nbits) < 0.5
+        A = np.packbits(bits, axis=1, bitorder="big").view(">u8").astype(np.uint64)
+        bc = bits.sum(axis=1).astype(np.uint32)
+        del bits
+        e = engine_factory(engine=engine, r2_mode=mode)
+        say("wide N", engine, mode, N, W)
+        res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+        assert e.last_engine == "popcount"
+        _check(res, cnt, A, bc, A, bc, N, True, mode)
+        res, cnt = e.correlate(A[:2], bc[:2], A[2:], bc[2:], snp_size=N, triangular=False,
+                               counts=True)
+        _check(res, cnt, A[:2], bc[:2], A[2:], bc[2:], N, False, mode)

@@ -204,3 +204,22 @@ def test_oracle_matches_reference_build(N, m, n):
     rr = oracle.ref_r2(bcA, bcB, cr, N)
     ro = oracle.r2_reference(bcA, bcB, co, N)
     assert (rr.view(np.uint32) == ro.view(np.uint32)).all()
+
+
+# ---- sample counts of 2^24 and more (the popcount engine's wide jobs) --------
+
+def test_r2_wide_sample_counts_match_reference_fixture():
+    """N from 2^24 to 2^31 - 1, where the reference's float casts of the
+    counts and of N round: the oracle's r^2 equals get_pairwise_ld_score's
+    bit for bit (tests/golden/r2_wide.npz, written by the reference build),
+    and the exact form stays within 1e-6 of the FP64 closed form."""
+    with np.load(os.path.join(GOLDEN, "r2_wide.npz"), allow_pickle=False) as z:
+        sizes = z["sizes"]
+        for k, N in enumerate(sizes):
+            bcA, bcB, c, want = z[f"bcA_{k}"], z[f"bcB_{k}"], z[f"counts_{k}"], z[f"r2_{k}"]
+            got = oracle.r2_reference(bcA, bcB, c, int(N))
+            assert (got.view(np.uint32) == want.view(np.uint32)).all(), f"N={N}"
+            ex = oracle.r2_exact(bcA, bcB, c, int(N))
+            r64 = oracle.r2_fp64(bcA, bcB, c, int(N))
+            ok = np.isfinite(r64)
+            assert np.abs(ex[ok].astype(np.float64) - r64[ok]).max() <= 1e-6, f"N={N}"
