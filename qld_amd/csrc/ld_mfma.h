@@ -112,9 +112,14 @@ struct WinOut {
 // Shared-pair units: `units` lists 128 x 64 wave blocks (0, bi, bj) of one
 // triangle job over a whole table (inline_desc, its r^2 pointer unused), each
 // computed once; item.desc is the offset in `wlist` of the block's window
-// list (a count, then that many indices into `wins`), and the epilogue writes
-// the block's r^2 into every listed window that holds the cell.  Tile 256
-// geometry, direct fp4 operands; otherwise as launch_mfma_wave_units.
+// list (a header word -- the window count in its low 16 bits, the block's
+// sub-block form above them -- then that many indices into `wins`), and the
+// epilogue writes the block's r^2 into every listed window that holds the
+// cell.  Sub-block forms: 0 all eight 32 x 32 blocks, 1-3 all but the first
+// 1-3 row blocks, QLD_SUB_HALF the first 32 columns only (the windows hold
+// no cell in the others).  Tile 256 geometry, direct fp4 operands; otherwise
+// as launch_mfma_wave_units.
+#define QLD_SUB_HALF 4
 hipError_t launch_mfma_shared_units(int r2_mode, const WorkDesc &inline_desc,
                                     const WorkItem *units, int64_t nunits,
                                     const uint32_t *wlist, const WinOut *wins, int grid,

@@ -266,7 +266,10 @@ static int pair_tile(const qld_ctx *ctx, const qld_table *t, const PairWin &w)
 // Shared-pair plan: the 128 x 64 blocks of the table's triangle that hold a
 // cell of some window, each listed once with the windows it serves, in row
 // band order (a workgroup's four waves claim four consecutive blocks, mostly
-// of one band, so they share their rows), diagonal blocks grouped.  A window [s, e) holds cells of
+// of one band, so they share their rows), each 8 bands' blocks by work.  A
+// block at the ragged edge of the windows' union, where the windows hold
+// cells only in its lower row blocks or its first 32 columns, computes only
+// those (its sub-block form, in its window list's header word).  A window [s, e) holds cells of
 // block (bi, bj) iff some row r in [s, e - 1) of the band has a column in
 // (r, e) inside the block: bi from s/128 to (e-2)/128, and for each, bj from
 // (r0 + 1)/64 to (e - 1)/64 with r0 the band's first row inside the window.
@@ -315,27 +318,77 @@ static int build_shared_plan(qld_ctx *ctx, const qld_table *t, const std::vector
     std::sort(refs.begin(), refs.end(), [](const Ref &x, const Ref &y) {
         return x.bi != y.bi ? x.bi < y.bi : x.bj != y.bj ? x.bj < y.bj : x.w < y.w;
     });
+    // Which 32 x 32 blocks of unit (bi, bj) hold a cell of window [ws, we):
+    // rows R of the block inside the window, columns C before its end, and
+    // some j in C right of some i in R (min R < max C).
+    auto needed = [&](int bi, int bj, int ws, int we) {
+        unsigned mask = 0;
+        for (int sa = 0; sa < wtm / 32; ++sa) {
+            const int rlo = std::max(bi * wtm + 32 * sa, ws);
+            const int rhi = std::min(bi * wtm + 32 * sa + 32, we) - 1;
+            if (rlo > rhi) continue;
+            for (int sb = 0; sb < wtn / 32; ++sb) {
+                const int ca = bj * wtn + 32 * sb, cmax = std::min(ca + 32, we) - 1;
+                if (ca <= cmax && rlo < cmax) mask |= 1u << (sa * 2 + sb);
+            }
+        }
+        return mask;
+    };
+    // MFMA blocks the kernel computes for a unit in sub-block form `sub`
+    // (the kernel's variants: diagonal blocks 3 (1 on the table's last
+    // columns), blocks on the table's last columns 4, forms 1-3 and
+    // QLD_SUB_HALF as named).
+    auto work = [&](const WorkItem &u, unsigned sub) {
+        const bool nfull = (int)u.tj * wtn + 32 < t->rows;
+        if ((int)u.tj * wtn == (int)u.ti * wtm) return nfull ? 3 : 1;
+        if (!nfull || sub == QLD_SUB_HALF) return 4;
+        return sub ? (4 - (int)sub) * 2 : 8;
+    };
     std::vector<WorkItem> items;
     std::vector<uint32_t> wl;
+    std::vector<int> unit_work;
     for (size_t a = 0; a < refs.size();) {
         size_t b = a;
-        while (b < refs.size() && refs[b].bi == refs[a].bi && refs[b].bj == refs[a].bj) ++b;
-        items.push_back(WorkItem{(uint32_t)wl.size(), (uint16_t)refs[a].bi, (uint16_t)refs[a].bj});
-        wl.push_back((uint32_t)(b - a));
+        unsigned mask = 0;
+        while (b < refs.size() && refs[b].bi == refs[a].bi && refs[b].bj == refs[a].bj) {
+            const PairWin &w = wins[refs[b].w];
+            mask |= needed((int)refs[a].bi, (int)refs[a].bj, w.a_start, w.a_start + w.a_size);
+            ++b;
+        }
+        const WorkItem u{(uint32_t)wl.size(), (uint16_t)refs[a].bi, (uint16_t)refs[a].bj};
+        // the cheapest form that computes every needed block: skip leading
+        // row blocks without one, or the second column half
+        unsigned sub = 0;
+        if ((int)u.tj * wtn != (int)u.ti * wtm) {
+            unsigned ro = 0;
+            while (ro < 3 && !((mask >> (2 * ro)) & 3u)) ++ro;
+            if (ro && work(u, ro) < work(u, sub)) sub = ro;
+            if (!(mask & 0xAAu) && work(u, QLD_SUB_HALF) < work(u, sub)) sub = QLD_SUB_HALF;
+        }
+        items.push_back(u);
+        unit_work.push_back(work(u, sub));
+        wl.push_back((uint32_t)(b - a) | (sub << 16));
         for (size_t k = a; k < b; ++k) wl.push_back(refs[k].w);
         a = b;
     }
     if (items.empty()) return QLD_OK;
-    // A diagonal block computes 3 of its 8 MFMA blocks; a workgroup's four
-    // waves wait for the slowest block of their strip, so within every run
-    // of 8 bands the diagonal blocks go last, together.
-    for (size_t a = 0; a < items.size();) {
-        size_t b = a;
-        while (b < items.size() && items[b].ti / 8 == items[a].ti / 8) ++b;
-        std::stable_partition(items.begin() + a, items.begin() + b, [&](const WorkItem &u) {
-            return (int)u.tj * wtn != (int)u.ti * wtm;
-        });
-        a = b;
+    // A workgroup's four waves wait for the slowest block of their strip, so
+    // within every run of 8 bands the blocks go in order of work, most first
+    // (full blocks, then the partial forms, the diagonal blocks together).
+    {
+        std::vector<size_t> idx(items.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        for (size_t a = 0; a < idx.size();) {
+            size_t b = a;
+            while (b < idx.size() && items[idx[b]].ti / 8 == items[idx[a]].ti / 8) ++b;
+            std::stable_sort(idx.begin() + a, idx.begin() + b, [&](size_t x, size_t y) {
+                return unit_work[x] > unit_work[y];
+            });
+            a = b;
+        }
+        std::vector<WorkItem> sorted(items.size());
+        for (size_t k = 0; k < idx.size(); ++k) sorted[k] = items[idx[k]];
+        items.swap(sorted);
     }
     PlanClass c;
     c.tile = QLD_TILE_LARGE;
