@@ -796,10 +796,14 @@ __device__ __forceinline__ void tile_epilogue_impl(const WorkDesc &d, int i0, in
             if constexpr (INTERIOR) {
                 // Every column of the tile is inside the job (and above the
                 // diagonal for a triangle) and rows are 16-byte aligned.
-                if (d.res) *(g_v4f *)rdst = v4f{rv[0], rv[1], rv[2], rv[3]};
+                // Non-temporal: the kernel never reads them back (1% faster
+                // on the 30,000-SNP triangle).
+                if (d.res)
+                    __builtin_nontemporal_store(v4f{rv[0], rv[1], rv[2], rv[3]}, (g_v4f *)rdst);
                 if (d.cnt)
-                    *(g_v4u *)cdst = v4u{(uint32_t)cv[0], (uint32_t)cv[1], (uint32_t)cv[2],
-                                         (uint32_t)cv[3]};
+                    __builtin_nontemporal_store(v4u{(uint32_t)cv[0], (uint32_t)cv[1],
+                                                    (uint32_t)cv[2], (uint32_t)cv[3]},
+                                                (g_v4u *)cdst);
             } else {
                 if (d.res) {
 #pragma unroll
