@@ -379,6 +379,12 @@ def windows_bench(args, steps):
         "roofline": roofline("mfma_fp4", upairs, (N + 63) // 64, N, k_s_sh,
                              measured_traffic(f"windows{nwin}_shared", args.r2_mode)
                              if traffic_ok else None, rows=urows, out_pairs=pairs)}
+    # The flop rate a per-window computation would need for the same outputs
+    # in the shared plan's time: above the matrix cores' peak once pairs are
+    # shared (the shared plan's own roofline counts distinct pairs only).
+    shared["per_window_equivalent"] = {
+        "achieved_tflop_s": round(pairs * 2.0 * N / k_s_sh / 1e12, 1),
+        "frac_of_peak": round(pairs * 2.0 * N / k_s_sh / PEAK["mfma_fp4"], 3)}
     ms_b = ms_s if shared_chosen else ms_w
     st = st_s if shared_chosen else st_w
     # The baseline, one launch per window, on 128-tiles: a single window's
