@@ -178,6 +178,19 @@ int qld_set_fp4_operands(qld_ctx *ctx, int mode);
  * else 128; expanded operands: 128). */
 int qld_set_plain_tile(qld_ctx *ctx, int tile);
 
+/* Split-K for plain calls on 128-tiles of the direct fp4 engine: each tile's
+ * word groups are summed by `chunks` workgroups and the last to finish adds
+ * the partial sums (exact integers) and writes the tile.  0 (default): by
+ * shape -- the chunk count that a cost model of the tiles' k-loop chains and
+ * the per-chunk overhead puts fastest, with at least 4 word groups per chunk
+ * (jobs of a few hundred tiles or fewer are split; large ones are not); 1: never;
+ * 2 .. 64: always that many (fewer where the partial sums would pass 1 GiB).
+ * The no-allocation entry point never splits.  Returns QLD_ERR_INVALID
+ * outside 0 .. 64.  qld_last_ksplit gives the chunks per tile of the last
+ * plain MFMA call (1: not split). */
+int qld_set_ksplit(qld_ctx *ctx, int chunks);
+int qld_last_ksplit(const qld_ctx *ctx);
+
 /* Engine the last call used (QLD_ENGINE_*), after AUTO resolution. */
 int qld_last_engine(const qld_ctx *ctx);
 

@@ -80,6 +80,8 @@ GPU_SYMBOLS = [
     ("qld_timer_stop", C.c_int, [_vp, _f32p]),
     ("qld_last_engine", C.c_int, [_vp]),
     ("qld_set_plain_tile", C.c_int, [_vp, C.c_int]),
+    ("qld_set_ksplit", C.c_int, [_vp, C.c_int]),
+    ("qld_last_ksplit", C.c_int, [_vp]),
     ("qld_set_fp4_operands", C.c_int, [_vp, C.c_int]),
     ("qld_table_create", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     ("qld_table_create_dev", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
@@ -668,6 +670,15 @@ class Engine:
 
     def set_plain_tile(self, tile: int):
         self._check(hip_lib().qld_set_plain_tile(self._ctx, int(tile)))
+
+    def set_ksplit(self, chunks: int):
+        """Split-K chunks per 128-tile of plain calls: 0 by shape, 1 never, 2-64 forced."""
+        self._check(hip_lib().qld_set_ksplit(self._ctx, int(chunks)))
+
+    @property
+    def last_ksplit(self) -> int:
+        """Chunks per tile of the last plain MFMA call (1: not split)."""
+        return hip_lib().qld_last_ksplit(self._ctx)
 
     def set_tile_cutoff(self, cutoff: int):
         self._check(hip_lib().qld_set_tile_cutoff(self._ctx, int(cutoff)))

@@ -111,6 +111,9 @@ struct Geo {
 #define QLD_CFG_L256W8 256, 64, 4, 2, 4
 #define QLD_CFG_L256W8H 256, 64, 4, 2, 4, 2
 #define QLD_CFG_S128 128, 128, 2, 2, 2
+typedef Geo<QLD_CFG_L256W8> GeoL8;
+typedef Geo<QLD_CFG_L256W8H> GeoL8H;
+typedef Geo<QLD_CFG_S128> GeoS;
 
 // v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (scale 2^0): on gfx950
 // it retires twice the flops per cycle of the 16x16x128 form.
@@ -156,6 +159,9 @@ struct KArgs {
     int64_t qbeg[QLD_NUM_XCD + 1];   // queue x owns items [qbeg[x], qbeg[x+1])
     const uint32_t *wlist;    // shared-pair units: window lists (item.desc = offset)
     const WinOut *wins;       // shared-pair units: window outputs
+    int ksplit;               // split-K units: chunks per item
+    float *kpart;             // split-K units: partial sums, 64 KiB per unit
+    uint32_t *kcnt;           // split-K units: arrivals per item (zero between launches)
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -639,12 +645,14 @@ __device__ __forceinline__ void kloop_packed(const WorkDesc &d, int i0, int j0,
 // hold rows / columns of the job (a ragged edge): the others are neither
 // loaded nor computed.  RO: nor are its first RO row blocks (shared-pair
 // units whose windows hold only the block's lower rows).
+// gb, ge: the word groups [gb, ge) to sum over (ge < 0: to the row end);
+// split-K units take one such range of a tile.
 #define QLD_NO_DIAG 99
 template <class G, int DOFF = QLD_NO_DIAG, int MA = G::NBM, int NA = G::NBN,
           class Hook = NoHook, int RO = 0>
 __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, int wm, int wn,
                                            v16f (&acc)[G::NBM][G::NBN],
-                                           const Hook &hook = Hook())
+                                           const Hook &hook = Hook(), int gb = 0, int ge = -1)
 {
     constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN;
     const int lane = threadIdx.x & 63;
@@ -718,9 +726,9 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     // loads in the loop are unconditional, so it carries no register copies
     // (the group read past the last is beyond the rows' blocks or the
     // descriptor, and is never computed).
-    const int full = groups;
+    const int full = ge < 0 ? groups : ge;
     v4i pa[NBM], pb[NBN], qa[NBM], qb[NBN];
-    fetch(pa, pb, 0);
+    fetch(pa, pb, gb);
     // Drain every outstanding memory operation once here (the previous
     // tile's r^2 stores among them).  The compiler treats loads and stores
     // pending together as completing out of order and would otherwise wait
@@ -728,7 +736,7 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
     // the older group only.
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     hook.begin();
-    int g = 0;
+    int g = gb;
     for (; g + 2 <= full; g += 2) {
         // Scheduling barriers keep each group's loads a whole compute ahead
         // of their use (left alone, the scheduler sinks them next to the
@@ -744,6 +752,34 @@ __device__ __forceinline__ void kloop_regs(const WorkDesc &d, int i0, int j0, in
         __builtin_amdgcn_sched_barrier(0);
     }
     if (g < full) compute(pa, pb);
+}
+
+// k loop of one 64 x 64 wave block of a 128-tile (direct operands): a block
+// that reaches past the job's last row or column computes only its first
+// 32-row / 32-column half there; a square block on a triangle's diagonal
+// skips its 32 x 32 block below the diagonal.  Word groups [gb, ge).
+template <class G, class Hook = NoHook>
+__device__ __forceinline__ void kloop_direct128(const WorkDesc &d, int i0, int j0, int wm,
+                                                int wn, v16f (&acc)[G::NBM][G::NBN],
+                                                const Hook &hook = Hook(), int gb = 0,
+                                                int ge = -1)
+{
+    static_assert(G::WTM == 64 && G::WTN == 64, "128-tile wave blocks");
+    const int a0 = i0 + wm * G::WTM, b0 = j0 + wn * G::WTN;
+    const bool mfull = a0 + 32 < d.ni, nfull = b0 + 32 < d.nj;
+    if (mfull && nfull) {
+        if (d.tri && a0 == b0)
+            kloop_regs<G, 0, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook, gb, ge);
+        else
+            kloop_regs<G, QLD_NO_DIAG, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook, gb,
+                                                             ge);
+    } else if (mfull) {
+        kloop_regs<G, QLD_NO_DIAG, 2, 1, Hook>(d, i0, j0, wm, wn, acc, hook, gb, ge);
+    } else if (nfull) {
+        kloop_regs<G, QLD_NO_DIAG, 1, 2, Hook>(d, i0, j0, wm, wn, acc, hook, gb, ge);
+    } else {
+        kloop_regs<G, QLD_NO_DIAG, 1, 1, Hook>(d, i0, j0, wm, wn, acc, hook, gb, ge);
+    }
 }
 
 template <class OP, int MODE, class G, bool INTERIOR>
@@ -991,22 +1027,8 @@ __device__ __forceinline__ void mfma_tile(const WorkDesc &d, int ti, int tj, int
         // no LDS ring: epilogue regions are per wave
         const int a0 = i0 + wm * WTM, b0 = j0 + wn * WTN;
         if constexpr (WTM == WTN && NBM == 2 && NBN == 2) {
-            // 128-tiles (the short windows of plans): a block that reaches
-            // past the job's last row or column computes only its first
-            // 32-row / 32-column half there
-            const bool mfull = a0 + 32 < d.ni, nfull = b0 + 32 < d.nj;
-            if (mfull && nfull) {
-                if (d.tri && a0 == b0)
-                    kloop_regs<G, 0, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook);
-                else
-                    kloop_regs<G, QLD_NO_DIAG, G::NBM, G::NBN, Hook>(d, i0, j0, wm, wn, acc, hook);
-            } else if (mfull) {
-                kloop_regs<G, QLD_NO_DIAG, 2, 1, Hook>(d, i0, j0, wm, wn, acc, hook);
-            } else if (nfull) {
-                kloop_regs<G, QLD_NO_DIAG, 1, 2, Hook>(d, i0, j0, wm, wn, acc, hook);
-            } else {
-                kloop_regs<G, QLD_NO_DIAG, 1, 1, Hook>(d, i0, j0, wm, wn, acc, hook);
-            }
+            // 128-tiles (the short windows of plans)
+            kloop_direct128<G, Hook>(d, i0, j0, wm, wn, acc, hook);
         } else if constexpr (WAVE_UNITS && WTM == 128 && WTN == 64) {
             // 128 x 64 wave blocks of 256-tiles: a block whose columns start
             // on its first row (a triangle's diagonal) has pairs j > i only
@@ -1193,6 +1215,123 @@ __global__ __launch_bounds__(G::THREADS, G::SPLIT) void ld_mfma_kernel(KArgs p)
     } else {
         run_queues<OP, MODE, G, PERSIST, PACKED>(p, wave, smem);
     }
+}
+
+// Split-K units (128-tiles, direct fp4 operands), for jobs too small to
+// fill the GPU with whole tiles: a tile's k loop is long (260 word groups
+// of dependent loads at 16,622 samples) and one workgroup per tile leaves
+// most CUs idle.  Unit u of the launch is chunk c = u % S of item u / S:
+// the word groups [G*c/S, G*(c+1)/S) of its tile (G = the job's groups of 4
+// words).  Every wave writes its partial sums -- integers, exact in fp32 --
+// to its own slice of `kpart`, then the workgroup counts itself in at
+// kcnt[item]; the last of the item's S workgroups to arrive adds the other
+// S-1 slices to its own sums (exact, in any order), resets the counter for
+// the next launch and runs the r^2 epilogue.  No workgroup waits for
+// another.
+// Visibility across XCDs (each has its own L2): the slices are written and
+// read as agent-scope relaxed atomics (stores and loads coherent at the
+// device's coherence point, past the XCD's L2), every wave waits for its
+// stores to complete before the workgroup barrier that precedes the arrival
+// atomic, and the last workgroup reads the slices only after that atomic
+// returned.  C++ release/acquire fences would do the same by writing back
+// and invalidating the whole L2 of the XCD at every arrival, which evicts
+// the operand rows of every other workgroup there (the split launch ran 2-7x
+// slower than the unsplit one with them).
+constexpr int SPLITK_SLICE = GeoS::NBM * GeoS::NBN * 16 * 64;   // floats per wave
+
+template <int MODE>
+__global__ __launch_bounds__(GeoS::THREADS) void ld_splitk_kernel(KArgs p)
+{
+    typedef GeoS G;
+    constexpr int NBM = G::NBM, NBN = G::NBN, WTM = G::WTM, WTN = G::WTN, TILE = G::EDGE;
+    constexpr int RPL = G::RPL;
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ int last;
+    const int S = p.ksplit;
+    const int64_t item = (int64_t)blockIdx.x / S;
+    const int c = (int)((int64_t)blockIdx.x % S);
+    const WorkItem it = load_uniform(p.items, item);
+    const WorkDesc d = p.descs ? load_uniform(p.descs, (int64_t)it.desc) : p.inline_desc;
+    const int wave = wave_id();
+    const int wm = wave / G::WAVES_N, wn = wave % G::WAVES_N;
+    const int lane = threadIdx.x & 63;
+    const int i0 = it.ti * TILE, j0 = it.tj * TILE;
+    const int a0 = i0 + wm * WTM, b0 = j0 + wn * WTN;
+    // as in mfma_tile: a block with no pair j > i, or no row or column in
+    // the job, computes and writes nothing (the same in every chunk)
+    const bool skip = (d.tri && b0 + WTN <= a0 + 1) || a0 >= d.ni || b0 >= d.nj;
+    const int groups = (d.words + 3) / 4;
+    const int gb = (int)((int64_t)groups * c / S), ge = (int)((int64_t)groups * (c + 1) / S);
+
+    uint32_t bc_row[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        const int i = a0 + q * 64 + lane;
+        bc_row[q] = (lane + q * 64 < WTM && i < d.ni) ? ((g_cu32 *)d.bcI)[i] : 0u;
+    }
+    const int jc = b0 + 4 * (lane % G::LANES_PER_ROW);
+    uint32_t bc_col[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bc_col[q] = jc + q < d.nj ? ((g_cu32 *)d.bcJ)[jc + q] : 0u;
+
+    v16f acc[NBM][NBN];
+#pragma unroll
+    for (int a = 0; a < NBM; ++a)
+#pragma unroll
+        for (int b = 0; b < NBN; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+    if (!skip) kloop_direct128<G>(d, i0, j0, wm, wn, acc, NoHook(), gb, ge);
+
+    // slice (item, chunk, wave), in 8-byte words: word 64 * (8 * block + r / 2)
+    // + lane holds registers r, r + 1 (r even) of MFMA block a * NBN + b
+    constexpr int WORDS = SPLITK_SLICE / 2;
+    uint64_t *const slices = (uint64_t *)p.kpart + (item * S * G::WAVES + wave) * WORDS + lane;
+    auto word = [&](int a, int b, int k) {
+        return (uint64_t)__float_as_uint(acc[a][b][2 * k]) |
+               ((uint64_t)__float_as_uint(acc[a][b][2 * k + 1]) << 32);
+    };
+    if (!skip) {
+        uint64_t *const mine = slices + (int64_t)c * G::WAVES * WORDS;
+#pragma unroll
+        for (int a = 0; a < NBM; ++a)
+#pragma unroll
+            for (int b = 0; b < NBN; ++b)
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    __hip_atomic_store(mine + 64 * (8 * (a * NBN + b) + k), word(a, b, k),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's slice is written
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t prev =
+            __hip_atomic_fetch_add(p.kcnt + item, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = prev == (uint32_t)(S - 1);
+        // every unit of the item has counted itself in: clear for the next launch
+        if (last) __hip_atomic_store(p.kcnt + item, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (!last || skip) return;
+    for (int s2 = 0; s2 < S; ++s2) {
+        if (s2 == c) continue;
+        uint64_t *const other = slices + (int64_t)s2 * G::WAVES * WORDS;
+        uint64_t v[NBM * NBN * 8];
+#pragma unroll
+        for (int k = 0; k < NBM * NBN * 8; ++k)
+            v[k] = __hip_atomic_load(other + 64 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int a = 0; a < NBM; ++a)
+#pragma unroll
+            for (int b = 0; b < NBN; ++b)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint64_t x = v[8 * (a * NBN + b) + k];
+                    acc[a][b][2 * k] += __uint_as_float((uint32_t)x);
+                    acc[a][b][2 * k + 1] += __uint_as_float((uint32_t)(x >> 32));
+                }
+    }
+    tile_epilogue<FP4, MODE, G>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
 }
 
 // Bit expansion.  Word w of a packed row (64 samples) becomes 32 bytes (fp4)
@@ -1413,10 +1552,6 @@ static hipError_t set_lds_limit()
                                lds_bytes<G, PACKED>());
 }
 
-typedef Geo<QLD_CFG_L256W8> GeoL8;
-typedef Geo<QLD_CFG_L256W8H> GeoL8H;
-typedef Geo<QLD_CFG_S128> GeoS;
-
 template <class OP, int MODE>
 static hipError_t set_lds_limits_mode()
 {
@@ -1525,6 +1660,38 @@ hipError_t launch_mfma_shared_units(int r2_mode, const WorkDesc &inline_desc,
     return r2_mode == QLD_R2_EXACT
                ? launch_one<FP4, QLD_R2_EXACT, GeoL8H, 3, 2>(a, (unsigned)grid, s)
                : launch_one<FP4, QLD_R2_REFERENCE, GeoL8H, 3, 2>(a, (unsigned)grid, s);
+}
+
+hipError_t launch_mfma_splitk(int r2_mode, const WorkDesc *descs, const WorkDesc &inline_desc,
+                              const WorkItem *items, int64_t nitems, int ksplit, float *kpart,
+                              uint32_t *kcnt, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    if (ksplit < 1 || ksplit > QLD_MAX_KSPLIT || !kpart || !kcnt ||
+        nitems * ksplit > 0x7FFFFFFF)
+        return hipErrorInvalidValue;
+    KArgs a = {};
+    a.inline_desc = inline_desc;
+    a.descs = descs;
+    a.items = items;
+    a.nitems = nitems;
+    a.ksplit = ksplit;
+    a.kpart = kpart;
+    a.kcnt = kcnt;
+    const unsigned grid = (unsigned)(nitems * ksplit);
+    constexpr size_t lds = lds_bytes<GeoS, 2>();
+    if (r2_mode == QLD_R2_EXACT)
+        hipLaunchKernelGGL(ld_splitk_kernel<QLD_R2_EXACT>, dim3(grid), dim3(GeoS::THREADS), lds,
+                           s, a);
+    else
+        hipLaunchKernelGGL(ld_splitk_kernel<QLD_R2_REFERENCE>, dim3(grid), dim3(GeoS::THREADS),
+                           lds, s, a);
+    return hipGetLastError();
+}
+
+size_t splitk_part_bytes(int64_t nitems, int ksplit)
+{
+    return (size_t)nitems * (size_t)ksplit * GeoS::WAVES * SPLITK_SLICE * sizeof(float);
 }
 
 void wave_block_shape(int tile, int *wtm, int *wtn)

@@ -37,11 +37,15 @@ struct qld_ctx {
     DevBuf ws;           // engine workspace (expanded operands)
     DevBuf tiles;        // device tile list of the last plain MFMA shape
     DevBuf queue;        // per-XCD queue heads of persistent plain launches
+    DevBuf kpart;        // split-K partial sums of plain launches
+    DevBuf kcnt;         // split-K arrival counters (zero between launches)
     std::vector<WorkItem> tiles_host;
     int tiles_key[4] = {-1, -1, -1, -1};         // (ni, nj, tri, tile edge)
     int tile_cutoff = 1024;   // windows of at least this many SNPs use 256-tiles
     int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape
     int fp4_packed = QLD_FP4_DIRECT;   // fp4 operands: QLD_FP4_DIRECT, _PACKED or _EXPANDED
+    int ksplit = 0;           // split-K chunks of plain 128-tile calls: 0 = by shape, 1 = none
+    int last_ksplit = 1;      // chunks per tile of the last plain MFMA call
     void *scratch = nullptr;  // device address of this device's static scratch array
     int plan_sched = QLD_SCHED_WAVES;  // persistent plans: per-wave units or whole tiles
     bool plan_graphs = true;  // capture each plan's launches in a HIP graph
@@ -67,6 +71,11 @@ int qld_hip_fail(qld_ctx *ctx, hipError_t e, const char *what);
 int qld_ctx_malloc(qld_ctx *ctx, size_t bytes, void **p);
 void qld_ctx_mfree(qld_ctx *ctx, void *p, size_t bytes);
 int qld_fill_async(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32);
+int qld_fill_on(qld_ctx *ctx, hipStream_t s, void *dptr, uint32_t value, size_t n_u32);
+// Split-K chunks per 128-tile for a direct fp4 launch of `ntiles` tiles whose
+// rows have `groups` word groups (1: none); the context's forced setting
+// (qld_set_ksplit) if any.
+int qld_ksplit_for(const qld_ctx *ctx, int64_t ntiles, int groups);
 int qld_run_job(qld_ctx *ctx, const LdJob &job);
 int qld_pick_engine(const qld_ctx *ctx, const LdJob &job);
 int qld_window_tile(const qld_ctx *ctx, int size);

@@ -8,6 +8,7 @@
 #include "qld_ctx.h"
 #include "ld_mfma.h"
 
+#include <algorithm>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -272,15 +273,20 @@ __global__ void qld_fill_u32(uint32_t *p, uint32_t v, size_t n)
     for (; i < n; i += stride) p[i] = v;
 }
 
-int qld_fill_async(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32)
+int qld_fill_on(qld_ctx *ctx, hipStream_t s, void *dptr, uint32_t value, size_t n_u32)
 {
     if (n_u32 == 0) return QLD_OK;
     size_t blocks = (n_u32 + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(qld_fill_u32, dim3((unsigned)blocks), dim3(256), 0,
-                       ctx->stream, (uint32_t *)dptr, value, n_u32);
+    hipLaunchKernelGGL(qld_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s,
+                       (uint32_t *)dptr, value, n_u32);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "fill kernel");
+}
+
+int qld_fill_async(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32)
+{
+    return qld_fill_on(ctx, ctx->stream, dptr, value, n_u32);
 }
 
 extern "C" int qld_dev_fill(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32)
@@ -397,6 +403,43 @@ static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job, int pmode)
     return QLD_TILE_SMALL;
 }
 
+// Split-K chunks per tile of a plain call on 128-tiles (1: none).  A tile's
+// k loop is a chain of dependent loads and MFMAs over every word group, so
+// a job with few tiles takes one tile's whole chain however few tiles it
+// has.  Splitting each tile's groups over S workgroups shortens the chain
+// S-fold and adds a fixed cost per unit (the partial sums written and, by
+// the last unit, read back: S - 1 slices of 16 KiB per wave).  Model, in
+// microseconds, fitted to 128-tile triangles at 16,622 and 200,000 samples:
+//   time(S) = rounds(S) * (0.6 * groups / S + 8 + 1.5 * (S - 1)),
+//   rounds(S) = ceil(tiles * S / (2 * CUs))   (two workgroups per CU)
+// (measured 2,000 SNPs: 44.8 us unsplit, 32.0 at S = 3; 500 SNPs: 62.5 / 22.2
+// at S = 4; 4,000 SNPs: 89.6 / 75.3 at S = 2; 1,000 SNPs at 200,000
+// samples: 495 / 70 at S = 14).  S is the model's minimum, with at least 4
+// groups per chunk.  `groups`: the word groups of the job's rows (the most
+// of any job in a mixed launch).
+int qld_ksplit_for(const qld_ctx *ctx, int64_t ntiles, int groups)
+{
+    const int64_t max_units = 16384;   // 1 GiB of partial sums
+    int best = 1;
+    if (ctx->ksplit) {
+        best = ctx->ksplit;
+    } else if (ntiles > 0) {
+        const int64_t slots = 2 * (int64_t)ctx->num_cu;
+        const int smax = std::min(QLD_MAX_KSPLIT, std::max(1, groups / 4));
+        double best_t = 0;
+        for (int s = 1; s <= smax; ++s) {
+            const double rounds = (double)((ntiles * s + slots - 1) / slots);
+            const double t = rounds * (0.6 * groups / s + 8.0 + 1.5 * (s - 1));
+            if (s == 1 || t < best_t) {
+                best_t = t;
+                best = s;
+            }
+        }
+    }
+    if (ntiles > 0 && ntiles * best > max_units) best = (int)(max_units / ntiles);
+    return best >= 2 ? best : 1;
+}
+
 static int upload_plain_tiles(qld_ctx *ctx, const LdJob &job, int T)
 {
     if (ctx->tiles_key[0] == job.n && ctx->tiles_key[1] == job.m &&
@@ -476,8 +519,29 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     // per tile share (256; the launcher doubles it for split tiles) drain
     // per-XCD queues over equal shares of the tile list (expanded operands,
     // 30,000-SNP triangle: 3.47 ms vs 4.11 ms for one workgroup per tile).
-    if ((rc = ctx->queue.reserve(ctx, 16 * sizeof(uint32_t)))) return rc;
     const int64_t ntiles = (int64_t)ctx->tiles_host.size();
+    const int ks = pmode == QLD_FP4_DIRECT && T == QLD_TILE_SMALL
+                       ? qld_ksplit_for(ctx, ntiles, (job.words + 3) / 4)
+                       : 1;
+    ctx->last_ksplit = ks;
+    if (ks > 1) {
+        // counters start zero (a fill kernel when the buffer grows) and every
+        // launch leaves them zero
+        const size_t had = ctx->kcnt.size;
+        if ((rc = ctx->kpart.reserve(ctx, splitk_part_bytes(ntiles, ks))) ||
+            (rc = ctx->kcnt.reserve(ctx, (size_t)ntiles * sizeof(uint32_t))))
+            return rc;
+        if (ctx->kcnt.size != had &&
+            (rc = qld_fill_async(ctx, ctx->kcnt.ptr, 0u, ctx->kcnt.size / sizeof(uint32_t))))
+            return rc;
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
+        e = launch_mfma_splitk(ctx->cfg.r2_mode, nullptr, d, (const WorkItem *)ctx->tiles.ptr,
+                               ntiles, ks, (float *)ctx->kpart.ptr, (uint32_t *)ctx->kcnt.ptr,
+                               ctx->stream);
+        if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
+        return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "split-K MFMA kernel launch");
+    }
+    if ((rc = ctx->queue.reserve(ctx, 16 * sizeof(uint32_t)))) return rc;
     int64_t grid = (int64_t)ctx->num_cu * (T == QLD_TILE_SMALL ? 2 : 1);
     if (grid > ntiles) grid = ntiles;
     if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
@@ -734,6 +798,18 @@ extern "C" int qld_set_fp4_operands(qld_ctx *ctx, int mode)
         return QLD_ERR_INVALID;
     ctx->fp4_packed = mode;
     return QLD_OK;
+}
+
+extern "C" int qld_set_ksplit(qld_ctx *ctx, int chunks)
+{
+    if (!ctx || chunks < 0 || chunks > QLD_MAX_KSPLIT) return QLD_ERR_INVALID;
+    ctx->ksplit = chunks;
+    return QLD_OK;
+}
+
+extern "C" int qld_last_ksplit(const qld_ctx *ctx)
+{
+    return ctx ? ctx->last_ksplit : QLD_ERR_INVALID;
 }
 
 extern "C" int qld_set_plain_tile(qld_ctx *ctx, int tile)

@@ -15,6 +15,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -44,6 +45,10 @@ struct Slot {
     void *h_stage = nullptr;      // pinned: descriptors + items
     void *d_stage = nullptr;
     size_t cap = 0;
+    uint32_t *d_kcnt = nullptr;   // split-K arrival counters (zero between launches)
+    size_t kcnt_cap = 0;
+    void *d_kpart = nullptr;      // split-K partial sums
+    size_t kpart_cap = 0;
     bool busy = false;
     std::vector<int64_t> reqs;
 };
@@ -133,6 +138,34 @@ static int slot_reserve(qld_server *s, Slot &sl, size_t bytes)
     return QLD_OK;
 }
 
+// Split-K scratch of slot `sl` for `items` tiles in `ks` chunks: counters
+// zeroed on the slot's stream whenever their buffer is (re)allocated, left
+// zero by every launch.  Only the dispatcher touches a slot, and only while
+// no batch of it is in flight.
+static int slot_split_reserve(qld_server *s, Slot &sl, int64_t items, int ks)
+{
+    const size_t cnt = (size_t)items * sizeof(uint32_t), part = splitk_part_bytes(items, ks);
+    int rc;
+    if (cnt > sl.kcnt_cap) {
+        if (sl.d_kcnt) qld_ctx_mfree(s->ctx, sl.d_kcnt, sl.kcnt_cap);
+        sl.d_kcnt = nullptr;
+        sl.kcnt_cap = 0;
+        const size_t cap = std::max(cnt, (size_t)4096);
+        if ((rc = qld_ctx_malloc(s->ctx, cap, (void **)&sl.d_kcnt))) return rc;
+        sl.kcnt_cap = cap;
+        if ((rc = qld_fill_on(s->ctx, sl.stream, sl.d_kcnt, 0u, cap / sizeof(uint32_t))))
+            return rc;
+    }
+    if (part > sl.kpart_cap) {
+        if (sl.d_kpart) qld_ctx_mfree(s->ctx, sl.d_kpart, sl.kpart_cap);
+        sl.d_kpart = nullptr;
+        sl.kpart_cap = 0;
+        if ((rc = qld_ctx_malloc(s->ctx, part, &sl.d_kpart))) return rc;
+        sl.kpart_cap = part;
+    }
+    return QLD_OK;
+}
+
 // Launch one batch on slot `sl`: `batch` holds copies of its requests, taken
 // under the server lock, so this runs without it (the HIP calls must not
 // stall submit, wait or release); only the dispatcher thread touches slots.
@@ -181,10 +214,25 @@ static int launch_batch(qld_server *s, Slot &sl, const std::vector<Request> &bat
     WorkDesc none;
     memset(&none, 0, sizeof(none));
     const WorkItem *d_items = (const WorkItem *)((char *)sl.d_stage + desc_bytes);
-    if (e == hipSuccess && split > 0)
+    // Direct-mode tiles of small batches on long rows split their k loops
+    // over several workgroups (qld_ksplit_for); every chunk count gives the
+    // same exact counts, so a result does not depend on its batch.
+    int ks = 1;
+    if (split > 0) {
+        int groups = 0;
+        for (const Request &r : batch)
+            if (r.t->direct) groups = std::max(groups, (r.t->words + 3) / 4);
+        ks = qld_ksplit_for(ctx, (int64_t)split, groups);
+    }
+    if (e == hipSuccess && split > 0 && ks > 1) {
+        if ((rc = slot_split_reserve(s, sl, (int64_t)split, ks))) return rc;
+        e = launch_mfma_splitk(ctx->cfg.r2_mode, (const WorkDesc *)sl.d_stage, none, d_items,
+                               (int64_t)split, ks, (float *)sl.d_kpart, sl.d_kcnt, sl.stream);
+    } else if (e == hipSuccess && split > 0) {
         e = launch_mfma_items(QLD_ENGINE_MFMA_FP4, ctx->cfg.r2_mode, T, QLD_FP4_DIRECT,
                               (const WorkDesc *)sl.d_stage, none, d_items, (int64_t)split, 0,
                               nullptr, nullptr, sl.stream);
+    }
     if (e == hipSuccess && cnt > split)
         e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, 0, (const WorkDesc *)sl.d_stage, none,
                               d_items + split, (int64_t)(cnt - split), 0, nullptr, nullptr,
@@ -352,6 +400,8 @@ extern "C" int qld_server_destroy(qld_server *s)
         if (sl.stream) hipStreamDestroy(sl.stream);
         if (sl.h_stage) hipHostFree(sl.h_stage);
         if (sl.d_stage) qld_ctx_mfree(s->ctx, sl.d_stage, sl.cap);
+        if (sl.d_kcnt) qld_ctx_mfree(s->ctx, sl.d_kcnt, sl.kcnt_cap);
+        if (sl.d_kpart) qld_ctx_mfree(s->ctx, sl.d_kpart, sl.kpart_cap);
     }
     qld_ctx_mfree(s->ctx, s->arena_mem, s->cfg.out_budget);
     delete s;
