@@ -250,10 +250,12 @@ def config1_bench(args, steps):
         eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
     eng.sync()
     reps = max(steps, 100)
+    eng.set_timing(False)     # timed calls without the per-call kernel events
     eng.timer_start()
     for _ in range(reps):
         eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
     ms = eng.timer_stop() / reps
+    eng.set_timing(True)
     engine = eng.last_engine
     for b in (dA, dbc, dres):
         b.free()
@@ -332,11 +334,14 @@ def windows_bench(args, steps):
         plan.use_graph(use_graph)
         for _ in range(10):
             plan.run()
+        eng.set_timing(False)     # timed runs without the per-call kernel events
         eng.sync()
         eng.timer_start()
         for _ in range(k):
             plan.run()
-        return eng.timer_stop() / k
+        ms = eng.timer_stop() / k
+        eng.set_timing(True)
+        return ms
 
     graphs = not args.no_graph
     shared_chosen = eng.plan_shared
@@ -443,11 +448,13 @@ def rect_bench(args, steps):
         eng.correlate_dev(dA, dbA, m, dB, dbB, n, W, N, False, dres, m)
     for _ in range(10):       # clocks ramp over the first launches
         step()
+    eng.set_timing(False)     # timed steps without the per-call kernel events
     eng.sync()
     eng.timer_start()
     for _ in range(steps):
         step()
     ms = eng.timer_stop() / steps
+    eng.set_timing(True)
     kms = []
     for _ in range(3):
         step()
@@ -795,6 +802,9 @@ def main():
     engine = eng.last_engine
 
     log(f"[bench] timing {args.steps} steps, engine {engine}")
+    # The timed steps run as a caller would run them, without the per-call
+    # kernel events; the kernel is timed afterwards with them.
+    eng.set_timing(False)
     eng.sync()
     t0 = time.perf_counter()
     eng.timer_start()
@@ -804,6 +814,7 @@ def main():
     eng.sync()
     wall_s = time.perf_counter() - t0
     ms_per_step = wall_s * 1e3 / args.steps
+    eng.set_timing(True)
 
     # Per-launch duration of the contraction kernel (events around it alone).
     kms = []
@@ -828,11 +839,13 @@ def main():
     dres2 = eng2.alloc(m * m * 4)
     for _ in range(args.warmup):
         eng2.correlate_dev(dA2, dbc2, m, dA2, dbc2, m, W, N, True, dres2, m)
+    eng2.set_timing(False)
     eng2.sync()
     eng2.timer_start()
     for _ in range(args.steps):
         eng2.correlate_dev(dA2, dbc2, m, dA2, dbc2, m, W, N, True, dres2, m)
     other_ms = eng2.timer_stop() / args.steps
+    eng2.set_timing(True)
     sample_rows = np.sort(np.random.default_rng(64).choice(m, min(64, m), replace=False))
     err_other = r2_error_vs_fp64(eng2, dA2, dbc2, m, W, N, sample_rows)
     for b in (dA2, dbc2, dres2):

@@ -154,6 +154,11 @@ int qld_last_kernel_ms(qld_ctx *ctx, float *ms);
  * its fused epilogue) of the last call; needs cfg.timing; waits for it. */
 int qld_last_contract_ms(qld_ctx *ctx, float *ms);
 
+/* Turn cfg.timing on or off for later calls.  Each timed call records four
+ * HIP events on the stream, about 15-30 us per call on the 30,000-SNP
+ * region; leave it off where only throughput matters. */
+int qld_set_timing(qld_ctx *ctx, int on);
+
 /* Device-time stopwatch over a region of the context's stream: start
  * records a HIP event on the stream, stop records a second one, waits for
  * it and returns the elapsed milliseconds. */

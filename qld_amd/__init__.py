@@ -81,6 +81,7 @@ GPU_SYMBOLS = [
     ("qld_last_engine", C.c_int, [_vp]),
     ("qld_set_plain_tile", C.c_int, [_vp, C.c_int]),
     ("qld_set_ksplit", C.c_int, [_vp, C.c_int]),
+    ("qld_set_timing", C.c_int, [_vp, C.c_int]),
     ("qld_last_ksplit", C.c_int, [_vp]),
     ("qld_set_fp4_operands", C.c_int, [_vp, C.c_int]),
     ("qld_table_create", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
@@ -670,6 +671,10 @@ class Engine:
 
     def set_plain_tile(self, tile: int):
         self._check(hip_lib().qld_set_plain_tile(self._ctx, int(tile)))
+
+    def set_timing(self, on: bool):
+        """Per-call kernel events on (last_kernel_ms / last_contract_ms) or off."""
+        self._check(hip_lib().qld_set_timing(self._ctx, 1 if on else 0))
 
     def set_ksplit(self, chunks: int):
         """Split-K chunks per 128-tile of plain calls: 0 by shape, 1 never, 2-64 forced."""

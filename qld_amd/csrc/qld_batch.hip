@@ -35,6 +35,7 @@ struct PlainLaunch {
     WorkDesc desc;
     int64_t item_off, item_cnt;
     int tile;
+    int ksplit = 1;                 // split-K chunks per tile (direct 128-tiles)
 };
 
 struct qld_plan {
@@ -45,6 +46,8 @@ struct qld_plan {
     std::vector<PlanClass> classes;       // persistent mode
     std::vector<PlainLaunch> plain;       // plain mode
     WorkItem *d_items_all = nullptr;      // plain mode item array
+    float *d_kpart = nullptr;             // plain mode split-K partial sums (launches in turn)
+    uint32_t *d_kcnt = nullptr;           // plain mode split-K counters, zero between launches
     std::vector<std::pair<void *, size_t>> owned;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -228,10 +231,15 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
         }
     } else {
         for (auto &l : p->plain) {
-            e = launch_mfma_items(p->engine, p->r2_mode, l.tile, p->direct ? QLD_FP4_DIRECT : 0,
-                                  nullptr, l.desc,
-                                  p->d_items_all + l.item_off, l.item_cnt, 0, nullptr, nullptr,
-                                  ctx->stream);
+            if (l.ksplit > 1)
+                e = launch_mfma_splitk(p->r2_mode, nullptr, l.desc, p->d_items_all + l.item_off,
+                                       l.item_cnt, l.ksplit, p->d_kpart, p->d_kcnt,
+                                       ctx->stream);
+            else
+                e = launch_mfma_items(p->engine, p->r2_mode, l.tile,
+                                      p->direct ? QLD_FP4_DIRECT : 0, nullptr, l.desc,
+                                      p->d_items_all + l.item_off, l.item_cnt, 0, nullptr,
+                                      nullptr, ctx->stream);
             if (e != hipSuccess) return e;
         }
     }
@@ -531,7 +539,13 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
             p->classes.push_back(c);
         }
     } else {
+        // One launch per window, each as a plain call on the same shape
+        // would run it: direct 128-tile windows split their k loops when
+        // the split model says so (qld_ksplit_for); the launches run in
+        // turn, so they share one split-K scratch.
         std::vector<WorkItem> items;
+        size_t part_max = 0;
+        int64_t cnt_max = 0;
         for (int k : order) {
             const PairWin &w = wins[k];
             if (pair_empty(w)) continue;
@@ -544,6 +558,12 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
             append_job_tiles(items.data(), &cnt, 0, w.b_size, w.a_size, w.tri, tile);
             l.item_cnt = (int64_t)cnt - l.item_off;
             l.desc = desc_of(w);
+            if (t->direct && tile == QLD_TILE_SMALL)
+                l.ksplit = qld_ksplit_for(ctx, l.item_cnt, (t->words + 3) / 4);
+            if (l.ksplit > 1) {
+                part_max = std::max(part_max, splitk_part_bytes(l.item_cnt, l.ksplit));
+                cnt_max = std::max(cnt_max, l.item_cnt);
+            }
             p->plain.push_back(l);
             p->pairs += pair_count(w);
         }
@@ -553,6 +573,16 @@ static int build_plan(qld_ctx *ctx, const qld_table *t, const std::vector<PairWi
                               items.size() * sizeof(WorkItem)))) {
             plan_release(ctx, p);
             return rc;
+        }
+        if (cnt_max > 0) {
+            // counters zeroed once here, before any capture; every launch
+            // leaves them zero
+            if ((rc = plan_alloc(ctx, p, (void **)&p->d_kpart, part_max)) ||
+                (rc = plan_alloc(ctx, p, (void **)&p->d_kcnt, (size_t)cnt_max * 4)) ||
+                (rc = qld_fill_async(ctx, p->d_kcnt, 0u, (size_t)cnt_max))) {
+                plan_release(ctx, p);
+                return rc;
+            }
         }
     }
 

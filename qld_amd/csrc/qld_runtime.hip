@@ -800,6 +800,13 @@ extern "C" int qld_set_fp4_operands(qld_ctx *ctx, int mode)
     return QLD_OK;
 }
 
+extern "C" int qld_set_timing(qld_ctx *ctx, int on)
+{
+    if (!ctx) return QLD_ERR_INVALID;
+    ctx->cfg.timing = on ? 1 : 0;
+    return QLD_OK;
+}
+
 extern "C" int qld_set_ksplit(qld_ctx *ctx, int chunks)
 {
     if (!ctx || chunks < 0 || chunks > QLD_MAX_KSPLIT) return QLD_ERR_INVALID;
