@@ -66,8 +66,11 @@ void set_tiled_operands(WorkDesc &d, const uint64_t *TI, int64_t rowI0, int rows
 // 32-row MFMA block is then 1 KiB of consecutive bytes instead of 32 pieces
 // 8*words bytes apart.
 int64_t tiled_bytes(int rows, int words);
+// zero_queue (optional): 16 device uint32 queue heads cleared by the same
+// kernel, for a persistent launch queued after it (launch_mfma_items with
+// queue_zeroed); rows and words must then be positive.
 hipError_t launch_tile_rows(const uint64_t *P, int rows, int words, uint64_t *T,
-                            hipStream_t s);
+                            hipStream_t s, uint32_t *zero_queue = nullptr);
 
 // Packed words per row after padding to whole 128-byte k stages.
 int mfma_padded_words(int engine, int words);
@@ -148,8 +151,10 @@ size_t splitk_part_bytes(int64_t nitems, int ksplit);
 //               x owns items [qbeg[x], qbeg[x+1]) (qbeg NULL: equal shares)
 //   packed      nonzero: fp4 engine reading PI/PJ packed rows (no expanded
 //               operand), 128-tiles only
+//   queue_zeroed  the queue heads were cleared on the stream already (by
+//               launch_tile_rows): no reset kernel
 hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              const WorkDesc *descs, const WorkDesc &inline_desc,
                              const WorkItem *items, int64_t nitems,
                              int persistent, uint32_t *queue, const int64_t *qbeg,
-                             hipStream_t s);
+                             hipStream_t s, bool queue_zeroed = false);

@@ -1501,11 +1501,14 @@ hipError_t launch_expand(int engine, const uint64_t *T, int rows, int words,
 
 // One thread per 32-byte piece (row r, word group g) of the tiled table:
 // consecutive threads write consecutive pieces; each reads its row's four
-// words (zero past the row or the table).
+// words (zero past the row or the table).  zq (optional): 16 queue heads the
+// first threads clear for the persistent launch that follows, which then
+// needs no reset kernel of its own.
 __global__ void tile_rows_kernel(const uint64_t *P, int rows, int W, int groups,
-                                 int64_t pieces, uint64_t *T)
+                                 int64_t pieces, uint64_t *T, uint32_t *zq)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (zq && t < 16) zq[t] = 0u;
     if (t >= pieces) return;
     const int r = (int)(t & 31);
     const int64_t bg = t >> 5;
@@ -1523,13 +1526,13 @@ __global__ void tile_rows_kernel(const uint64_t *P, int rows, int W, int groups,
 }
 
 hipError_t launch_tile_rows(const uint64_t *P, int rows, int words, uint64_t *T,
-                            hipStream_t s)
+                            hipStream_t s, uint32_t *zero_queue)
 {
-    if (rows <= 0 || words <= 0) return hipSuccess;
+    if (rows <= 0 || words <= 0) return zero_queue ? hipErrorInvalidValue : hipSuccess;
     const int groups = (words + 3) / 4;
     const int64_t pieces = (int64_t)((rows + 31) / 32) * groups * 32;
     hipLaunchKernelGGL(tile_rows_kernel, dim3((unsigned)((pieces + 255) / 256)), dim3(256), 0, s,
-                       P, rows, words, groups, pieces, T);
+                       P, rows, words, groups, pieces, T, zero_queue);
     return hipGetLastError();
 }
 
@@ -1704,7 +1707,7 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
                              const WorkDesc *descs, const WorkDesc &inline_desc,
                              const WorkItem *items, int64_t nitems,
                              int persistent, uint32_t *queue, const int64_t *qbeg,
-                             hipStream_t s)
+                             hipStream_t s, bool queue_zeroed)
 {
     if (nitems <= 0) return hipSuccess;
     KArgs a = {};
@@ -1717,9 +1720,11 @@ hipError_t launch_mfma_items(int engine, int r2_mode, int tile, int packed,
         a.qbeg[x] = qbeg ? qbeg[x] : nitems * x / QLD_NUM_XCD;
     if (persistent > 0) {
         if (!queue) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, queue);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if (!queue_zeroed) {
+            hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, s, queue);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
     }
     if (packed == 2) {      // packed rows straight into registers
         if (engine != QLD_ENGINE_MFMA_FP4) return hipErrorInvalidValue;

@@ -477,6 +477,15 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
     }
     int rc = upload_plain_tiles(ctx, job, T);
     if (rc) return rc;
+    const int64_t ntiles = (int64_t)ctx->tiles_host.size();
+    const int ks = pmode == QLD_FP4_DIRECT && T == QLD_TILE_SMALL
+                       ? qld_ksplit_for(ctx, ntiles, (job.words + 3) / 4)
+                       : 1;
+    ctx->last_ksplit = ks;
+    // Persistent launches take their queue heads from the context; the
+    // direct path clears them in its tiling kernel (no reset kernel).
+    if (ks == 1 && (rc = ctx->queue.reserve(ctx, 16 * sizeof(uint32_t)))) return rc;
+    bool queue_zeroed = false;
     hipError_t e = hipSuccess;
     WorkDesc d;
     if (pmode == QLD_FP4_DIRECT) {
@@ -489,7 +498,9 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
         if ((rc = ctx->ws.reserve(ctx, bytesA + bytesB))) return rc;
         uint64_t *TA = (uint64_t *)ctx->ws.ptr;
         uint64_t *TB = same ? TA : (uint64_t *)((char *)ctx->ws.ptr + bytesA);
-        e = launch_tile_rows(job.A, job.m, job.words, TA, ctx->stream);
+        queue_zeroed = ks == 1;
+        e = launch_tile_rows(job.A, job.m, job.words, TA, ctx->stream,
+                             queue_zeroed ? (uint32_t *)ctx->queue.ptr : nullptr);
         if (e == hipSuccess && !same) e = launch_tile_rows(job.B, job.n, job.words, TB, ctx->stream);
         if (e != hipSuccess) return qld_hip_fail(ctx, e, "row tiling kernel");
         fill_work_desc(d, nullptr, nullptr, job.bcB, job.bcA, job.n, job.m, 0, job.tri,
@@ -515,15 +526,6 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
         fill_work_desc(d, XB, XA, job.bcB, job.bcA, job.n, job.m, rb, job.tri, job.snp_size,
                        job.res, job.ldr, job.cnt, job.ldc);
     }
-    // Persistent launch: two workgroups per CU (128-tiles) or one per CU
-    // per tile share (256; the launcher doubles it for split tiles) drain
-    // per-XCD queues over equal shares of the tile list (expanded operands,
-    // 30,000-SNP triangle: 3.47 ms vs 4.11 ms for one workgroup per tile).
-    const int64_t ntiles = (int64_t)ctx->tiles_host.size();
-    const int ks = pmode == QLD_FP4_DIRECT && T == QLD_TILE_SMALL
-                       ? qld_ksplit_for(ctx, ntiles, (job.words + 3) / 4)
-                       : 1;
-    ctx->last_ksplit = ks;
     if (ks > 1) {
         // counters start zero (a fill kernel when the buffer grows) and every
         // launch leaves them zero
@@ -541,13 +543,16 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
         if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
         return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "split-K MFMA kernel launch");
     }
-    if ((rc = ctx->queue.reserve(ctx, 16 * sizeof(uint32_t)))) return rc;
+    // Persistent launch: two workgroups per CU (128-tiles) or one per CU
+    // per tile share (256; the launcher doubles it for split tiles) drain
+    // per-XCD queues over equal shares of the tile list (expanded operands,
+    // 30,000-SNP triangle: 3.47 ms vs 4.11 ms for one workgroup per tile).
     int64_t grid = (int64_t)ctx->num_cu * (T == QLD_TILE_SMALL ? 2 : 1);
     if (grid > ntiles) grid = ntiles;
     if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
     e = launch_mfma_items(engine, ctx->cfg.r2_mode, T, pmode, nullptr, d,
                           (const WorkItem *)ctx->tiles.ptr, ntiles, (int)grid,
-                          (uint32_t *)ctx->queue.ptr, nullptr, ctx->stream);
+                          (uint32_t *)ctx->queue.ptr, nullptr, ctx->stream, queue_zeroed);
     if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
     return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "MFMA kernel launch");
 }
