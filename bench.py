@@ -243,6 +243,8 @@ def config1_bench(args, steps):
     A, bc = qld_amd.synth_blocks(20261203 + 1, m, N)
     W = A.shape[1]
     eng = qld_amd.Engine(engine="auto", r2_mode=args.r2_mode, timing=True)
+    if args.ksplit is not None:
+        eng.set_ksplit(args.ksplit)
     dA = eng.alloc(A.nbytes).upload(A)
     dbc = eng.alloc(bc.nbytes).upload(bc)
     dres = eng.alloc(m * m * 4)
@@ -256,17 +258,27 @@ def config1_bench(args, steps):
         eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
     ms = eng.timer_stop() / reps
     eng.set_timing(True)
+    kms = []
+    for _ in range(10):
+        eng.correlate_dev(dA, dbc, m, dA, dbc, m, W, N, True, dres, m)
+        kms.append(eng.last_contract_ms())
     engine = eng.last_engine
+    ksplit = eng.last_ksplit
     for b in (dA, dbc, dres):
         b.free()
     eng.close()
     pairs = m * (m - 1) // 2
+    k_s = float(np.median(kms)) / 1e3
     cpu = host_cpu()
     threads = min(x for x in (cpu["physical_cores"], cpu["affinity_cpus"],
                               cpu["omp_num_threads"]) if x)
     out = {"workload": f"single region all-pairs r2, {m} SNPs x {N} samples, lower triangle",
            "pairs": pairs, "gpu": {"ms": round(ms, 4), "gpairs_s": round(pairs / ms / 1e6, 2),
-                                   "engine": engine}}
+                                   "engine": engine, "ksplit": ksplit,
+                                   "roofline": roofline(
+                                       engine, pairs, W, N, k_s,
+                                       measured_traffic(f"single{m}_splitk", args.r2_mode)
+                                       if ksplit == 3 else None, rows=m)}}
     for T in sorted({1, threads}):
         best = 1e30
         for _ in range(3):
@@ -738,6 +750,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0,
                     help="tile edge of the single-region MFMA launch (0: engine default)")
     ap.add_argument("--no-list", action="store_true")
+    ap.add_argument("--ksplit", type=int, default=None,
+                    help="split-K chunks for the configuration-1 leg (0 by shape, 1 none)")
     ap.add_argument("--list-snps", type=int, default=100000)
     ap.add_argument("--list-windows", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0,

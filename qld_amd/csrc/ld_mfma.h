@@ -132,14 +132,15 @@ void wave_block_shape(int tile, int *wtm, int *wtn);
 // Split-K launch (128-tiles, direct fp4 operands): each of the `nitems`
 // tiles is summed by `ksplit` workgroups over equal ranges of its word
 // groups; the last to finish adds the others' partial sums and writes the
-// tile's r^2 (and counts).  kpart: splitk_part_bytes(nitems, ksplit) bytes
-// of device scratch; kcnt: nitems device uint32, zero before the first
-// launch and left zero by every launch.  Launches that share kpart or kcnt
-// must not overlap.
+// tile's r^2 (and counts).  max_groups: the most word groups of any job's
+// rows (sets the partial sums' width).  kpart: splitk_part_bytes(nitems,
+// ksplit) bytes of device scratch; kcnt: nitems device uint32, zero before
+// the first launch and left zero by every launch.  Launches that share
+// kpart or kcnt must not overlap.
 #define QLD_MAX_KSPLIT 64
 hipError_t launch_mfma_splitk(int r2_mode, const WorkDesc *descs, const WorkDesc &inline_desc,
-                              const WorkItem *items, int64_t nitems, int ksplit, float *kpart,
-                              uint32_t *kcnt, hipStream_t s);
+                              const WorkItem *items, int64_t nitems, int ksplit, int max_groups,
+                              float *kpart, uint32_t *kcnt, hipStream_t s);
 size_t splitk_part_bytes(int64_t nitems, int ksplit);
 
 // Launch the MFMA kernel over `nitems` items.

@@ -1237,9 +1237,12 @@ __global__ __launch_bounds__(G::THREADS, G::SPLIT) void ld_mfma_kernel(KArgs p)
 // and invalidating the whole L2 of the XCD at every arrival, which evicts
 // the operand rows of every other workgroup there (the split launch ran 2-7x
 // slower than the unsplit one with them).
+// P16: every chunk's counts are below 2^16 (at most 255 word groups per
+// chunk), and the slices hold them as 16-bit integers: half the bytes of
+// fp32 slices, which are about half the kernel's HBM traffic.
 constexpr int SPLITK_SLICE = GeoS::NBM * GeoS::NBN * 16 * 64;   // floats per wave
 
-template <int MODE>
+template <int MODE, bool P16>
 __global__ __launch_bounds__(GeoS::THREADS) void ld_splitk_kernel(KArgs p)
 {
     typedef GeoS G;
@@ -1283,13 +1286,19 @@ __global__ __launch_bounds__(GeoS::THREADS) void ld_splitk_kernel(KArgs p)
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
     if (!skip) kloop_direct128<G>(d, i0, j0, wm, wn, acc, NoHook(), gb, ge);
 
-    // slice (item, chunk, wave), in 8-byte words: word 64 * (8 * block + r / 2)
-    // + lane holds registers r, r + 1 (r even) of MFMA block a * NBN + b
-    constexpr int WORDS = SPLITK_SLICE / 2;
+    // slice (item, chunk, wave), in 8-byte words: word 64 * (PW * block + k)
+    // + lane holds registers RW * k .. RW * k + RW - 1 of MFMA block
+    // a * NBN + b (RW = 2 fp32 values, or 4 16-bit integers with P16)
+    constexpr int WORDS = SPLITK_SLICE / 2;        // slice stride in 8-byte words
+    constexpr int RW = P16 ? 4 : 2, PW = 16 / RW;  // registers per word, words per block
     uint64_t *const slices = (uint64_t *)p.kpart + (item * S * G::WAVES + wave) * WORDS + lane;
     auto word = [&](int a, int b, int k) {
-        return (uint64_t)__float_as_uint(acc[a][b][2 * k]) |
-               ((uint64_t)__float_as_uint(acc[a][b][2 * k + 1]) << 32);
+        uint64_t w = 0;
+#pragma unroll
+        for (int e = 0; e < RW; ++e)
+            w |= P16 ? (uint64_t)(uint32_t)acc[a][b][RW * k + e] << (16 * e)
+                     : (uint64_t)__float_as_uint(acc[a][b][RW * k + e]) << (32 * e);
+        return w;
     };
     if (!skip) {
         uint64_t *const mine = slices + (int64_t)c * G::WAVES * WORDS;
@@ -1298,8 +1307,8 @@ __global__ __launch_bounds__(GeoS::THREADS) void ld_splitk_kernel(KArgs p)
 #pragma unroll
             for (int b = 0; b < NBN; ++b)
 #pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    __hip_atomic_store(mine + 64 * (8 * (a * NBN + b) + k), word(a, b, k),
+                for (int k = 0; k < PW; ++k)
+                    __hip_atomic_store(mine + 64 * (PW * (a * NBN + b) + k), word(a, b, k),
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's slice is written
@@ -1316,19 +1325,22 @@ __global__ __launch_bounds__(GeoS::THREADS) void ld_splitk_kernel(KArgs p)
     for (int s2 = 0; s2 < S; ++s2) {
         if (s2 == c) continue;
         uint64_t *const other = slices + (int64_t)s2 * G::WAVES * WORDS;
-        uint64_t v[NBM * NBN * 8];
+        uint64_t v[NBM * NBN * PW];
 #pragma unroll
-        for (int k = 0; k < NBM * NBN * 8; ++k)
+        for (int k = 0; k < NBM * NBN * PW; ++k)
             v[k] = __hip_atomic_load(other + 64 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int a = 0; a < NBM; ++a)
 #pragma unroll
             for (int b = 0; b < NBN; ++b)
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const uint64_t x = v[8 * (a * NBN + b) + k];
-                    acc[a][b][2 * k] += __uint_as_float((uint32_t)x);
-                    acc[a][b][2 * k + 1] += __uint_as_float((uint32_t)(x >> 32));
+                for (int k = 0; k < PW; ++k) {
+                    const uint64_t x = v[PW * (a * NBN + b) + k];
+#pragma unroll
+                    for (int e = 0; e < RW; ++e)
+                        acc[a][b][RW * k + e] +=
+                            P16 ? (float)(uint32_t)((x >> (16 * e)) & 0xFFFFu)
+                                : __uint_as_float((uint32_t)(x >> (32 * e)));
                 }
     }
     tile_epilogue<FP4, MODE, G>(d, i0, j0, wm, wn, wave, acc, bc_row, bc_col, smem);
@@ -1666,13 +1678,15 @@ hipError_t launch_mfma_shared_units(int r2_mode, const WorkDesc &inline_desc,
 }
 
 hipError_t launch_mfma_splitk(int r2_mode, const WorkDesc *descs, const WorkDesc &inline_desc,
-                              const WorkItem *items, int64_t nitems, int ksplit, float *kpart,
-                              uint32_t *kcnt, hipStream_t s)
+                              const WorkItem *items, int64_t nitems, int ksplit, int max_groups,
+                              float *kpart, uint32_t *kcnt, hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
-    if (ksplit < 1 || ksplit > QLD_MAX_KSPLIT || !kpart || !kcnt ||
+    if (ksplit < 1 || ksplit > QLD_MAX_KSPLIT || !kpart || !kcnt || max_groups < 1 ||
         nitems * ksplit > 0x7FFFFFFF)
         return hipErrorInvalidValue;
+    // a chunk of g word groups counts at most 256 g samples per pair
+    const bool p16 = (max_groups + ksplit - 1) / ksplit <= 255;
     KArgs a = {};
     a.inline_desc = inline_desc;
     a.descs = descs;
@@ -1683,12 +1697,14 @@ hipError_t launch_mfma_splitk(int r2_mode, const WorkDesc *descs, const WorkDesc
     a.kcnt = kcnt;
     const unsigned grid = (unsigned)(nitems * ksplit);
     constexpr size_t lds = lds_bytes<GeoS, 2>();
-    if (r2_mode == QLD_R2_EXACT)
-        hipLaunchKernelGGL(ld_splitk_kernel<QLD_R2_EXACT>, dim3(grid), dim3(GeoS::THREADS), lds,
-                           s, a);
-    else
-        hipLaunchKernelGGL(ld_splitk_kernel<QLD_R2_REFERENCE>, dim3(grid), dim3(GeoS::THREADS),
-                           lds, s, a);
+    const dim3 g(grid), b(GeoS::THREADS);
+    if (r2_mode == QLD_R2_EXACT) {
+        if (p16) hipLaunchKernelGGL((ld_splitk_kernel<QLD_R2_EXACT, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((ld_splitk_kernel<QLD_R2_EXACT, false>), g, b, lds, s, a);
+    } else {
+        if (p16) hipLaunchKernelGGL((ld_splitk_kernel<QLD_R2_REFERENCE, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((ld_splitk_kernel<QLD_R2_REFERENCE, false>), g, b, lds, s, a);
+    }
     return hipGetLastError();
 }
 

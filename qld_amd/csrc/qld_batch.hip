@@ -233,8 +233,8 @@ static hipError_t plan_issue(qld_ctx *ctx, qld_plan *p)
         for (auto &l : p->plain) {
             if (l.ksplit > 1)
                 e = launch_mfma_splitk(p->r2_mode, nullptr, l.desc, p->d_items_all + l.item_off,
-                                       l.item_cnt, l.ksplit, p->d_kpart, p->d_kcnt,
-                                       ctx->stream);
+                                       l.item_cnt, l.ksplit, (l.desc.words + 3) / 4,
+                                       p->d_kpart, p->d_kcnt, ctx->stream);
             else
                 e = launch_mfma_items(p->engine, p->r2_mode, l.tile,
                                       p->direct ? QLD_FP4_DIRECT : 0, nullptr, l.desc,

@@ -538,8 +538,8 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
             return rc;
         if (ctx->cfg.timing) hipEventRecord(ctx->kev0, ctx->stream);
         e = launch_mfma_splitk(ctx->cfg.r2_mode, nullptr, d, (const WorkItem *)ctx->tiles.ptr,
-                               ntiles, ks, (float *)ctx->kpart.ptr, (uint32_t *)ctx->kcnt.ptr,
-                               ctx->stream);
+                               ntiles, ks, (job.words + 3) / 4, (float *)ctx->kpart.ptr,
+                               (uint32_t *)ctx->kcnt.ptr, ctx->stream);
         if (ctx->cfg.timing) hipEventRecord(ctx->kev1, ctx->stream);
         return e == hipSuccess ? QLD_OK : qld_hip_fail(ctx, e, "split-K MFMA kernel launch");
     }

@@ -217,9 +217,8 @@ static int launch_batch(qld_server *s, Slot &sl, const std::vector<Request> &bat
     // Direct-mode tiles of small batches on long rows split their k loops
     // over several workgroups (qld_ksplit_for); every chunk count gives the
     // same exact counts, so a result does not depend on its batch.
-    int ks = 1;
+    int ks = 1, groups = 0;
     if (split > 0) {
-        int groups = 0;
         for (const Request &r : batch)
             if (r.t->direct) groups = std::max(groups, (r.t->words + 3) / 4);
         ks = qld_ksplit_for(ctx, (int64_t)split, groups);
@@ -227,7 +226,8 @@ static int launch_batch(qld_server *s, Slot &sl, const std::vector<Request> &bat
     if (e == hipSuccess && split > 0 && ks > 1) {
         if ((rc = slot_split_reserve(s, sl, (int64_t)split, ks))) return rc;
         e = launch_mfma_splitk(ctx->cfg.r2_mode, (const WorkDesc *)sl.d_stage, none, d_items,
-                               (int64_t)split, ks, (float *)sl.d_kpart, sl.d_kcnt, sl.stream);
+                               (int64_t)split, ks, groups, (float *)sl.d_kpart, sl.d_kcnt,
+                               sl.stream);
     } else if (e == hipSuccess && split > 0) {
         e = launch_mfma_items(QLD_ENGINE_MFMA_FP4, ctx->cfg.r2_mode, T, QLD_FP4_DIRECT,
                               (const WorkDesc *)sl.d_stage, none, d_items, (int64_t)split, 0,

@@ -33,12 +33,14 @@ def _check(res, cnt, A, bcA, B, bcB, N, tri, mode):
         assert (res[~sel] == 0).all() and (cnt[~sel] == 0).all()
 
 
-# (kind, N, shape): 65, 4 and 391 word groups; tiles on the diagonal, ragged
-# edges, a single row; chunk counts above the group count leave empty chunks
+# (kind, N, shape): 65, 4, 391 and 782 word groups; tiles on the diagonal,
+# ragged edges, a single row; chunk counts above the group count leave empty
+# chunks; at 782 groups 2 and 3 chunks keep fp32 partial sums (more than 255
+# groups per chunk), more chunks 16-bit ones
 CASES = [
     ("tri", 16622, 300), ("tri", 16622, 129), ("tri", 16622, 1),
     ("rect", 16622, (129, 127)), ("rect", 1000, (300, 257)), ("rect", 100000, (65, 200)),
-    ("tri", 100000, 257),
+    ("tri", 100000, 257), ("rect", 200000, (40, 150)),
 ]
 
 
@@ -142,3 +144,27 @@ def test_device_rows_sampled(engine_factory):
     assert (res.view(np.uint32)[sel] == want.view(np.uint32)[sel]).all()
     for b in (dA, dbc, dres):
         b.free()
+
+
+@pytest.mark.parametrize("groups, chunks", [(510, 2), (512, 2), (765, 3), (768, 3)])
+def test_partial_sum_width_bounds(groups, chunks, engine_factory):
+    """Partial sums are 16-bit while a chunk holds at most 255 word groups
+    (counts up to 65,280) and fp32 beyond: rows of all ones and their
+    complements at chunk sizes 255 and 256 groups, counts against the
+    oracle (a pair of all-one rows counts every sample)."""
+    N = groups * 256
+    W = groups * 4
+    rng = np.random.default_rng(groups)
+    bits = np.zeros((70, W * 64), dtype=np.uint8)
+    bits[:10] = 1                                       # all ones
+    bits[10:40] = rng.random((30, W * 64)) < 0.97       # dense
+    bits[40:] = 1 - bits[10:40]                         # complements
+    A = np.packbits(bits, axis=1, bitorder="big").view(">u8").astype(np.uint64)
+    bc = bits.sum(axis=1).astype(np.uint32)
+    e = engine_factory(engine="mfma_fp4", r2_mode="exact")
+    e.set_ksplit(chunks)
+    say("width", groups, chunks)
+    res, cnt = e.correlate(A, bc, snp_size=N, triangular=True, counts=True)
+    assert e.last_ksplit == chunks
+    assert cnt[0, 1] == N
+    _check(res, cnt, A, bc, A, bc, N, True, "exact")
