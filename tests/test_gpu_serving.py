@@ -132,3 +132,28 @@ def test_copy_mode_pinned(tables):
     srv.release(rid)
     srv.close()
     qld_amd.hip_lib().qld_host_free(e._ctx, p)
+
+
+@pytest.mark.parametrize("chunks", [0, 1, 5, 64])
+def test_split_batches_match_oracle(chunks, engine_factory):
+    """Batches on a direct table of long rows (100,000 samples, 391 word
+    groups) split their tiles' k loops by shape (0), never (1) or into a
+    forced chunk count (5, 64: more chunks than some tiles' batches need):
+    every result against the oracle, alone and batched."""
+    e = engine_factory(engine="auto", r2_mode="reference")
+    e.set_ksplit(chunks)
+    N = 100000
+    A, bc = qld_amd.synth_blocks(1000 + chunks, 900, N)
+    t = e.table(A, bc, N)
+    srv = qld_amd.Server(e, out_budget=256 << 20, max_batch=6, streams=2)
+    reqs = [(s, n) for (_, s, n) in _requests(np.random.default_rng(chunks), [(t, A)], 16, 2, 400)]
+    say("split batches", chunks, len(reqs))
+    ids = [srv.submit(t, s, n) for s, n in reqs]
+    for rid, (s, n) in zip(ids, reqs):
+        srv.wait(rid, WAIT_MS)
+        got = srv.fetch(rid, n)
+        assert (_tri(got).view(np.uint32) == _tri(_want(A, bc, N, s, n)).view(np.uint32)).all(), (s, n)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["status"] == 0
+    srv.close()
+    t.free()
