@@ -408,7 +408,7 @@ static int plain_tile_edge(const qld_ctx *ctx, const LdJob &job, int pmode)
 // a job with few tiles takes one tile's whole chain however few tiles it
 // has.  Splitting each tile's groups over S workgroups shortens the chain
 // S-fold and adds a fixed cost per unit (the partial sums written and, by
-// the last unit, read back: S - 1 slices of 16 KiB per wave).  Model, in
+// the last unit, read back: S - 1 slices of 8 or 16 KiB per wave).  Model, in
 // microseconds, fitted to 128-tile triangles at 16,622 and 200,000 samples:
 //   time(S) = rounds(S) * (0.6 * groups / S + 8 + 1.5 * (S - 1)),
 //   rounds(S) = ceil(tiles * S / (2 * CUs))   (two workgroups per CU)
