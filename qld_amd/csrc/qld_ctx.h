@@ -39,7 +39,6 @@ struct qld_ctx {
     DevBuf queue;        // per-XCD queue heads of persistent plain launches
     DevBuf kpart;        // split-K partial sums of plain launches
     DevBuf kcnt;         // split-K arrival counters (zero between launches)
-    std::vector<WorkItem> tiles_host;
     int tiles_key[4] = {-1, -1, -1, -1};         // (ni, nj, tri, tile edge)
     int tile_cutoff = 1024;   // windows of at least this many SNPs use 256-tiles
     int plain_tile = 0;       // tile edge of plain MFMA calls; 0 = by shape

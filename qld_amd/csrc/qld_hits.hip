@@ -192,6 +192,8 @@ struct Slot {
     float *h_r2 = nullptr;
     uint64_t h_cap = 0;
     hipEvent_t done = nullptr;      // hits of this slot are in host memory
+    uint64_t *h_total = nullptr;    // pinned: the block's hit count
+    hipEvent_t ev_total = nullptr;  // h_total has arrived
     int row0 = 0, rows = 0;
     bool pending = false;
 };
@@ -199,11 +201,10 @@ struct Slot {
 struct HitsRun {
     qld_ctx *ctx;
     hipStream_t copy = nullptr;
-    hipEvent_t ev_total = nullptr, ev_written = nullptr;
+    hipEvent_t ev_written = nullptr;
     std::vector<std::pair<void *, size_t>> dev;
     std::vector<void *> pinned;
-    Slot slot[2];
-    uint64_t *h_total = nullptr;
+    Slot slot[3];
 
     int dalloc(void **p, size_t bytes)
     {
@@ -241,9 +242,10 @@ struct HitsRun {
         hipStreamSynchronize(ctx->stream);
         for (auto &d : dev) qld_ctx_mfree(ctx, d.first, d.second);
         for (void *p : pinned) hipHostFree(p);
-        for (auto &s : slot)
+        for (auto &s : slot) {
             if (s.done) hipEventDestroy(s.done);
-        if (ev_total) hipEventDestroy(ev_total);
+            if (s.ev_total) hipEventDestroy(s.ev_total);
+        }
         if (ev_written) hipEventDestroy(ev_written);
         if (copy) hipStreamDestroy(copy);
     }
@@ -327,25 +329,111 @@ extern "C" int qld_correlate_hits(qld_ctx *ctx,
     int64_t R = block_pairs / std::max(m, 1);
     R = std::max<int64_t>(256, (R + 255) / 256 * 256);
     R = std::min<int64_t>(R, n);
-    float *dres;
+    // Pipeline over blocks, so that the GPU always has the next block queued
+    // while the host waits on it: iteration b queues block b's r^2, hit
+    // counts and total; then waits for block b-1's total (block b keeps the
+    // GPU busy meanwhile), sizes its buffers and queues its hit writes and
+    // copies; then hands block b-2's hits to the callback.  Two r^2 blocks
+    // (block b is computed while block b-1's hits are written from the
+    // other) and three hit slots (b, b-1, b-2 in flight).
+    float *dres[2];
     uint32_t *dcounts;
-    if ((rc = run.dalloc((void **)&dres, (size_t)R * m * 4)) ||
+    if ((rc = run.dalloc((void **)&dres[0], (size_t)R * m * 4)) ||
+        (rc = run.dalloc((void **)&dres[1], (size_t)R * m * 4)) ||
         (rc = run.dalloc((void **)&dcounts, (size_t)R * 4)))
         return rc;
     for (auto &sl : run.slot) {
         if ((rc = run.dalloc((void **)&sl.d_off, (size_t)(R + 1) * 8)) ||
-            (rc = run.halloc((void **)&sl.h_off, (size_t)(R + 1) * 8)))
+            (rc = run.halloc((void **)&sl.h_off, (size_t)(R + 1) * 8)) ||
+            (rc = run.halloc((void **)&sl.h_total, 8)))
             return rc;
-        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
+        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&sl.ev_total, hipEventDisableTiming)) != hipSuccess)
             return qld_hip_fail(ctx, e, "event");
     }
-    if ((rc = run.halloc((void **)&run.h_total, 8))) return rc;
     if ((e = hipStreamCreateWithFlags(&run.copy, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&run.ev_total, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&run.ev_written, hipEventDisableTiming)) != hipSuccess)
         return qld_hip_fail(ctx, e, "stream/event");
 
-    auto deliver = [&](Slot &sl) -> int {
+    struct Block {
+        int64_t i0;
+        int rows, col0, cols;
+        int64_t ldr;
+    };
+    auto block = [&](int64_t b) {
+        Block k;
+        k.i0 = b * R;
+        k.rows = (int)std::min<int64_t>(R, n - k.i0);
+        k.col0 = upper ? (int)std::min<int64_t>(k.i0, m) : 0;
+        k.cols = m - k.col0;
+        k.ldr = std::max(k.cols, 1);
+        return k;
+    };
+    // r^2 of block rows [i0, i0 + rows) x columns [col0, m), its hit counts
+    // and their total (to the slot's pinned word).
+    auto compute = [&](int64_t b) -> int {
+        const Block k = block(b);
+        Slot &sl = run.slot[b % 3];
+        float *res = dres[b & 1];
+        if (k.cols > 0) {
+            if (upper && same) {
+                LdJob diag{dA + (size_t)k.i0 * words, dbcA + k.i0, k.rows,
+                           dA + (size_t)k.i0 * words, dbcA + k.i0, k.rows, words, snp_size, 1,
+                           res, k.ldr, nullptr, k.rows};
+                if ((rc = qld_run_job(ctx, diag))) return rc;
+                if (k.i0 + k.rows < m) {
+                    const int rest = (int)(m - k.i0 - k.rows);
+                    LdJob rect{dA + (size_t)(k.i0 + k.rows) * words, dbcA + k.i0 + k.rows, rest,
+                               dA + (size_t)k.i0 * words, dbcA + k.i0, k.rows, words, snp_size,
+                               0, res + k.rows, k.ldr, nullptr, rest};
+                    if ((rc = qld_run_job(ctx, rect))) return rc;
+                }
+            } else {
+                LdJob job{dA + (size_t)k.col0 * words, dbcA + k.col0, k.cols,
+                          dB + (size_t)k.i0 * words, dbcB + k.i0, k.rows, words, snp_size, 0,
+                          res, k.ldr, nullptr, k.cols};
+                if ((rc = qld_run_job(ctx, job))) return rc;
+            }
+        }
+        if ((e = enqueue_count_scan(res, k.ldr, k.rows, k.cols, (int)k.i0, k.col0, upper,
+                                    r2limit, dcounts, sl.d_off, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(sl.h_total, sl.d_off + k.rows, 8, hipMemcpyDeviceToHost, s)) !=
+                hipSuccess ||
+            (e = hipEventRecord(sl.ev_total, s)) != hipSuccess)
+            return qld_hip_fail(ctx, e, "hits count");
+        return QLD_OK;
+    };
+    // Block b's hits into its slot's buffers, then copied to the host.
+    auto write = [&](int64_t b) -> int {
+        const Block k = block(b);
+        Slot &sl = run.slot[b % 3];
+        if ((e = hipEventSynchronize(sl.ev_total)) != hipSuccess)
+            return qld_hip_fail(ctx, e, "hits total");
+        const uint64_t hits = *sl.h_total;
+        if ((rc = run.reserve(sl, hits))) return rc;
+        if (hits) {
+            hipLaunchKernelGGL(hits_write_kernel, dim3(k.rows), dim3(HIT_WG), 0, s, dres[b & 1],
+                               k.ldr, k.cols, (int)k.i0, k.col0, upper, r2limit, sl.d_off,
+                               sl.d_cols, sl.d_r2);
+            if ((e = hipGetLastError()) != hipSuccess) return qld_hip_fail(ctx, e, "hits write");
+        }
+        if ((e = hipEventRecord(run.ev_written, s)) != hipSuccess ||
+            (e = hipStreamWaitEvent(run.copy, run.ev_written, 0)) != hipSuccess ||
+            (e = hipMemcpyAsync(sl.h_off, sl.d_off, (size_t)(k.rows + 1) * 8,
+                                hipMemcpyDeviceToHost, run.copy)) != hipSuccess ||
+            (hits && ((e = hipMemcpyAsync(sl.h_cols, sl.d_cols, hits * 4, hipMemcpyDeviceToHost,
+                                          run.copy)) != hipSuccess ||
+                      (e = hipMemcpyAsync(sl.h_r2, sl.d_r2, hits * 4, hipMemcpyDeviceToHost,
+                                          run.copy)) != hipSuccess)) ||
+            (e = hipEventRecord(sl.done, run.copy)) != hipSuccess)
+            return qld_hip_fail(ctx, e, "hits D2H");
+        sl.row0 = (int)k.i0;
+        sl.rows = k.rows;
+        sl.pending = true;
+        return QLD_OK;
+    };
+    auto deliver = [&](int64_t b) -> int {
+        Slot &sl = run.slot[b % 3];
         if ((e = hipEventSynchronize(sl.done)) != hipSuccess) return qld_hip_fail(ctx, e, "D2H");
         sl.pending = false;
         int u = fn(user, sl.row0, sl.rows, sl.h_off, sl.h_cols, sl.h_r2);
@@ -356,64 +444,13 @@ extern "C" int qld_correlate_hits(qld_ctx *ctx,
         return QLD_OK;
     };
 
-    int prev = -1;
-    for (int64_t i0 = 0, b = 0; i0 < n; i0 += R, b++) {
-        const int rows = (int)std::min<int64_t>(R, n - i0);
-        const int col0 = upper ? (int)std::min<int64_t>(i0, m) : 0;
-        const int cols = m - col0;
-        const int64_t ldr = std::max(cols, 1);
-        Slot &sl = run.slot[b & 1];
-        // r^2 of block rows [i0, i0 + rows) x columns [col0, m).
-        if (cols > 0) {
-            if (upper && same) {
-                LdJob diag{dA + (size_t)i0 * words, dbcA + i0, rows, dA + (size_t)i0 * words,
-                           dbcA + i0, rows, words, snp_size, 1, dres, ldr, nullptr, rows};
-                if ((rc = qld_run_job(ctx, diag))) return rc;
-                if (i0 + rows < m) {
-                    const int rest = (int)(m - i0 - rows);
-                    LdJob rect{dA + (size_t)(i0 + rows) * words, dbcA + i0 + rows, rest,
-                               dA + (size_t)i0 * words, dbcA + i0, rows, words, snp_size, 0,
-                               dres + rows, ldr, nullptr, rest};
-                    if ((rc = qld_run_job(ctx, rect))) return rc;
-                }
-            } else {
-                LdJob job{dA + (size_t)col0 * words, dbcA + col0, cols, dB + (size_t)i0 * words,
-                          dbcB + i0, rows, words, snp_size, 0, dres, ldr, nullptr, cols};
-                if ((rc = qld_run_job(ctx, job))) return rc;
-            }
-        }
-        if ((e = enqueue_count_scan(dres, ldr, rows, cols, (int)i0, col0, upper, r2limit,
-                                    dcounts, sl.d_off, s)) != hipSuccess ||
-            (e = hipMemcpyAsync(run.h_total, sl.d_off + rows, 8, hipMemcpyDeviceToHost, s)) !=
-                hipSuccess ||
-            (e = hipEventRecord(run.ev_total, s)) != hipSuccess)
-            return qld_hip_fail(ctx, e, "hits count");
-        // The previous block is formatted while this one computes.
-        if (prev >= 0 && (rc = deliver(run.slot[prev]))) return rc;
-        if ((e = hipEventSynchronize(run.ev_total)) != hipSuccess)
-            return qld_hip_fail(ctx, e, "hits total");
-        const uint64_t hits = *run.h_total;
-        if ((rc = run.reserve(sl, hits))) return rc;
-        if (hits) {
-            hipLaunchKernelGGL(hits_write_kernel, dim3(rows), dim3(HIT_WG), 0, s, dres, ldr, cols,
-                               (int)i0, col0, upper, r2limit, sl.d_off, sl.d_cols, sl.d_r2);
-            if ((e = hipGetLastError()) != hipSuccess) return qld_hip_fail(ctx, e, "hits write");
-        }
-        if ((e = hipEventRecord(run.ev_written, s)) != hipSuccess ||
-            (e = hipStreamWaitEvent(run.copy, run.ev_written, 0)) != hipSuccess ||
-            (e = hipMemcpyAsync(sl.h_off, sl.d_off, (size_t)(rows + 1) * 8,
-                                hipMemcpyDeviceToHost, run.copy)) != hipSuccess ||
-            (hits && ((e = hipMemcpyAsync(sl.h_cols, sl.d_cols, hits * 4, hipMemcpyDeviceToHost,
-                                          run.copy)) != hipSuccess ||
-                      (e = hipMemcpyAsync(sl.h_r2, sl.d_r2, hits * 4, hipMemcpyDeviceToHost,
-                                          run.copy)) != hipSuccess)) ||
-            (e = hipEventRecord(sl.done, run.copy)) != hipSuccess)
-            return qld_hip_fail(ctx, e, "hits D2H");
-        sl.row0 = (int)i0;
-        sl.rows = rows;
-        sl.pending = true;
-        prev = (int)(b & 1);
+    const int64_t nb = (n + R - 1) / R;
+    for (int64_t b = 0; b < nb; ++b) {
+        if ((rc = compute(b))) return rc;
+        if (b >= 1 && (rc = write(b - 1))) return rc;
+        if (b >= 2 && (rc = deliver(b - 2))) return rc;
     }
-    if (prev >= 0 && (rc = deliver(run.slot[prev]))) return rc;
-    return QLD_OK;
+    if ((rc = write(nb - 1))) return rc;
+    if (nb >= 2 && (rc = deliver(nb - 2))) return rc;
+    return deliver(nb - 1);
 }

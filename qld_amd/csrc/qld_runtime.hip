@@ -440,21 +440,25 @@ int qld_ksplit_for(const qld_ctx *ctx, int64_t ntiles, int groups)
     return best >= 2 ? best : 1;
 }
 
-static int upload_plain_tiles(qld_ctx *ctx, const LdJob &job, int T)
+// The device tile list of a plain job (desc 0, append_job_tiles' order),
+// written by a kernel on the stream when the shape differs from the last
+// call's: no host copy, so a call never waits for earlier work on the stream
+// (an upload from host memory did, whenever the shape changed -- every
+// block of a streamed report alternates two shapes).  Returns the count.
+static int plain_tiles(qld_ctx *ctx, const LdJob &job, int T, int64_t *ntiles)
 {
+    const size_t cnt = job_tile_count(job.n, job.m, job.tri, T);
+    *ntiles = (int64_t)cnt;
     if (ctx->tiles_key[0] == job.n && ctx->tiles_key[1] == job.m &&
         ctx->tiles_key[2] == job.tri && ctx->tiles_key[3] == T)
         return QLD_OK;
-    size_t cnt = 0;
-    ctx->tiles_host.resize(job_tile_count(job.n, job.m, job.tri, T));
-    append_job_tiles(ctx->tiles_host.data(), &cnt, 0, job.n, job.m, job.tri, T);
-    size_t tb = cnt * sizeof(WorkItem);
     ctx->tiles_key[0] = -1;
+    const size_t tb = cnt * sizeof(WorkItem);
     int rc = ctx->tiles.reserve(ctx, tb ? tb : 8);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(ctx->tiles.ptr, ctx->tiles_host.data(), tb,
-                                  hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return qld_hip_fail(ctx, e, "tile list upload");
+    hipError_t e = launch_job_tiles((WorkItem *)ctx->tiles.ptr, job.n, job.m, job.tri, T,
+                                    ctx->stream);
+    if (e != hipSuccess) return qld_hip_fail(ctx, e, "tile list kernel");
     ctx->tiles_key[0] = job.n;
     ctx->tiles_key[1] = job.m;
     ctx->tiles_key[2] = job.tri;
@@ -475,9 +479,9 @@ static int run_mfma(qld_ctx *ctx, int engine, const LdJob &job)
         qld_set_error(ctx, "too many rows for one launch (m=%d n=%d)", job.m, job.n);
         return QLD_ERR_INVALID;
     }
-    int rc = upload_plain_tiles(ctx, job, T);
+    int64_t ntiles = 0;
+    int rc = plain_tiles(ctx, job, T, &ntiles);
     if (rc) return rc;
-    const int64_t ntiles = (int64_t)ctx->tiles_host.size();
     const int ks = pmode == QLD_FP4_DIRECT && T == QLD_TILE_SMALL
                        ? qld_ksplit_for(ctx, ntiles, (job.words + 3) / 4)
                        : 1;
