@@ -10,6 +10,7 @@
 #include <vector>
 
 struct qld_ctx;
+struct qld_hits_cache;    // scratch of the streamed selection (qld_hits.hip)
 
 // Grow-only device buffer owned by a context.
 struct DevBuf {
@@ -50,6 +51,7 @@ struct qld_ctx {
     bool plan_graphs = true;  // capture each plan's launches in a HIP graph
     bool plan_shared = false; // persistent plans of single-region windows count
                               // each pair once (QLD_BATCH_SHARED form)
+    qld_hits_cache *hits = nullptr;   // kept between streamed-selection calls
     char err[512] = {0};
 };
 
@@ -71,6 +73,7 @@ int qld_ctx_malloc(qld_ctx *ctx, size_t bytes, void **p);
 void qld_ctx_mfree(qld_ctx *ctx, void *p, size_t bytes);
 int qld_fill_async(qld_ctx *ctx, void *dptr, uint32_t value, size_t n_u32);
 int qld_fill_on(qld_ctx *ctx, hipStream_t s, void *dptr, uint32_t value, size_t n_u32);
+void qld_hits_release(qld_ctx *ctx);
 // Split-K chunks per 128-tile for a direct fp4 launch of `ntiles` tiles whose
 // rows have `groups` word groups (1: none); the context's forced setting
 // (qld_set_ksplit) if any.

@@ -182,105 +182,116 @@ extern "C" int qld_compact_dev(qld_ctx *ctx, const float *dres, int64_t ldr, int
 
 namespace {
 
+// Pinned host buffer, grow-only.
+struct PinBuf {
+    void *p = nullptr;
+    size_t size = 0;
+    int reserve(qld_ctx *ctx, size_t bytes)
+    {
+        if (bytes <= size) return QLD_OK;
+        release();
+        hipError_t e = hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return qld_hip_fail(ctx, e, "pinned host buffer");
+        }
+        size = bytes ? bytes : 8;
+        return QLD_OK;
+    }
+    void release()
+    {
+        if (p) hipHostFree(p);
+        p = nullptr;
+        size = 0;
+    }
+};
+
 struct Slot {
-    uint64_t *d_off = nullptr;      // rows + 1 offsets
-    uint32_t *d_cols = nullptr;
-    float *d_r2 = nullptr;
-    uint64_t cap = 0;
-    uint64_t *h_off = nullptr;      // pinned
-    uint32_t *h_cols = nullptr;
-    float *h_r2 = nullptr;
-    uint64_t h_cap = 0;
+    DevBuf d_off, d_cols, d_r2;     // rows + 1 offsets; the block's hits
+    PinBuf h_off, h_cols, h_r2;     // their host copies
+    PinBuf h_total;                 // the block's hit count
     hipEvent_t done = nullptr;      // hits of this slot are in host memory
-    uint64_t *h_total = nullptr;    // pinned: the block's hit count
     hipEvent_t ev_total = nullptr;  // h_total has arrived
     int row0 = 0, rows = 0;
-    bool pending = false;
 };
 
-struct HitsRun {
-    qld_ctx *ctx;
+}  // namespace
+
+// Scratch of the streamed selection, kept by the context between calls (the
+// buffers only grow): allocating and page-locking it anew cost every call a
+// few milliseconds.  Calls on one context do not overlap.
+struct qld_hits_cache {
+    DevBuf tabA, bcA, tabB, bcB;    // the tables on the device
+    DevBuf res[2], counts;          // two r^2 blocks, row hit counts
+    Slot slot[3];                   // blocks b, b-1, b-2 in flight
     hipStream_t copy = nullptr;
     hipEvent_t ev_written = nullptr;
-    std::vector<std::pair<void *, size_t>> dev;
-    std::vector<void *> pinned;
-    Slot slot[3];
-
-    int dalloc(void **p, size_t bytes)
-    {
-        int rc = qld_ctx_malloc(ctx, bytes ? bytes : 8, p);
-        if (rc == QLD_OK) dev.push_back({*p, bytes ? bytes : 8});
-        return rc;
-    }
-    void dfree(void *p)
-    {
-        for (size_t k = 0; k < dev.size(); k++)
-            if (dev[k].first == p) {
-                qld_ctx_mfree(ctx, p, dev[k].second);
-                dev.erase(dev.begin() + (long)k);
-                return;
-            }
-    }
-    int halloc(void **p, size_t bytes)
-    {
-        hipError_t e = hipHostMalloc(p, bytes ? bytes : 8, hipHostMallocDefault);
-        if (e != hipSuccess) return qld_hip_fail(ctx, e, "pinned host buffer");
-        pinned.push_back(*p);
-        return QLD_OK;
-    }
-    void hfree(void *p)
-    {
-        auto it = std::find(pinned.begin(), pinned.end(), p);
-        if (it != pinned.end()) {
-            hipHostFree(p);
-            pinned.erase(it);
-        }
-    }
-    ~HitsRun()
-    {
-        if (copy) hipStreamSynchronize(copy);
-        hipStreamSynchronize(ctx->stream);
-        for (auto &d : dev) qld_ctx_mfree(ctx, d.first, d.second);
-        for (void *p : pinned) hipHostFree(p);
-        for (auto &s : slot) {
-            if (s.done) hipEventDestroy(s.done);
-            if (s.ev_total) hipEventDestroy(s.ev_total);
-        }
-        if (ev_written) hipEventDestroy(ev_written);
-        if (copy) hipStreamDestroy(copy);
-    }
-    // Device and pinned hit capacity of a slot for `hits` entries (the slot's
-    // previous copy has completed: its callback already ran).
-    int reserve(Slot &s, uint64_t hits)
-    {
-        int rc;
-        if (hits > s.cap) {
-            uint64_t cap = std::max<uint64_t>(hits + hits / 4, 1u << 20);
-            dfree(s.d_cols);
-            dfree(s.d_r2);
-            s.d_cols = nullptr;
-            s.d_r2 = nullptr;
-            s.cap = 0;
-            if ((rc = dalloc((void **)&s.d_cols, cap * 4)) ||
-                (rc = dalloc((void **)&s.d_r2, cap * 4)))
-                return rc;
-            s.cap = cap;
-        }
-        if (hits > s.h_cap) {
-            uint64_t cap = std::max<uint64_t>(hits + hits / 4, 1u << 20);
-            hfree(s.h_cols);
-            hfree(s.h_r2);
-            s.h_cols = nullptr;
-            s.h_r2 = nullptr;
-            s.h_cap = 0;
-            if ((rc = halloc((void **)&s.h_cols, cap * 4)) ||
-                (rc = halloc((void **)&s.h_r2, cap * 4)))
-                return rc;
-            s.h_cap = cap;
-        }
-        return QLD_OK;
-    }
 };
+
+void qld_hits_release(qld_ctx *ctx)
+{
+    qld_hits_cache *c = ctx->hits;
+    if (!c) return;
+    if (c->copy) hipStreamSynchronize(c->copy);
+    hipStreamSynchronize(ctx->stream);
+    for (DevBuf *d : {&c->tabA, &c->bcA, &c->tabB, &c->bcB, &c->res[0], &c->res[1], &c->counts})
+        d->release();
+    for (Slot &sl : c->slot) {
+        sl.d_off.release();
+        sl.d_cols.release();
+        sl.d_r2.release();
+        sl.h_off.release();
+        sl.h_cols.release();
+        sl.h_r2.release();
+        sl.h_total.release();
+        if (sl.done) hipEventDestroy(sl.done);
+        if (sl.ev_total) hipEventDestroy(sl.ev_total);
+    }
+    if (c->ev_written) hipEventDestroy(c->ev_written);
+    if (c->copy) hipStreamDestroy(c->copy);
+    delete c;
+    ctx->hits = nullptr;
+}
+
+namespace {
+
+int hits_cache(qld_ctx *ctx, qld_hits_cache **out)
+{
+    if (!ctx->hits) {
+        qld_hits_cache *c = new (std::nothrow) qld_hits_cache();
+        if (!c) return QLD_ERR_NOMEM;
+        ctx->hits = c;
+        hipError_t e;
+        for (Slot &sl : c->slot)
+            if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&sl.ev_total, hipEventDisableTiming)) != hipSuccess) {
+                qld_hits_release(ctx);
+                return qld_hip_fail(ctx, e, "event");
+            }
+        if ((e = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&c->ev_written, hipEventDisableTiming)) != hipSuccess) {
+            qld_hits_release(ctx);
+            return qld_hip_fail(ctx, e, "stream/event");
+        }
+    }
+    *out = ctx->hits;
+    return QLD_OK;
+}
+
+// Device and pinned hit capacity of a slot for `hits` entries (the slot's
+// previous copy has completed: its callback already ran).
+int slot_reserve(qld_ctx *ctx, Slot &sl, uint64_t hits)
+{
+    const uint64_t want = std::max<uint64_t>(hits + hits / 4, 1u << 20) * 4;
+    int rc;
+    if (hits * 4 > sl.d_cols.size &&
+        ((rc = sl.d_cols.reserve(ctx, want)) || (rc = sl.d_r2.reserve(ctx, want))))
+        return rc;
+    if (hits * 4 > sl.h_cols.size &&
+        ((rc = sl.h_cols.reserve(ctx, want)) || (rc = sl.h_r2.reserve(ctx, want))))
+        return rc;
+    return QLD_OK;
+}
 
 }  // namespace
 
@@ -299,23 +310,21 @@ extern "C" int qld_correlate_hits(qld_ctx *ctx,
     if (rc) return rc;
     if (m == 0 || n == 0) return QLD_OK;
     const bool same = A == B && bcA == bcB && m == n;
-    HitsRun run{ctx};
+    qld_hits_cache *run;
+    if ((rc = hits_cache(ctx, &run))) return rc;
     hipError_t e;
     hipStream_t s = ctx->stream;
 
     // Tables on the device.
-    uint64_t *dA, *dB;
-    uint32_t *dbcA, *dbcB;
     const size_t ta = (size_t)m * words * 8, tb = (size_t)n * words * 8;
-    if ((rc = run.dalloc((void **)&dA, ta)) || (rc = run.dalloc((void **)&dbcA, (size_t)m * 4)))
+    if ((rc = run->tabA.reserve(ctx, ta)) || (rc = run->bcA.reserve(ctx, (size_t)m * 4)))
         return rc;
-    if (same) {
-        dB = dA;
-        dbcB = dbcA;
-    } else if ((rc = run.dalloc((void **)&dB, tb)) ||
-               (rc = run.dalloc((void **)&dbcB, (size_t)n * 4))) {
+    if (!same && ((rc = run->tabB.reserve(ctx, tb)) || (rc = run->bcB.reserve(ctx, (size_t)n * 4))))
         return rc;
-    }
+    uint64_t *dA = (uint64_t *)run->tabA.ptr;
+    uint32_t *dbcA = (uint32_t *)run->bcA.ptr;
+    uint64_t *dB = same ? dA : (uint64_t *)run->tabB.ptr;
+    uint32_t *dbcB = same ? dbcA : (uint32_t *)run->bcB.ptr;
     if ((e = hipMemcpyAsync(dA, A, ta, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(dbcA, bcA, (size_t)m * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
         return qld_hip_fail(ctx, e, "H2D");
@@ -336,24 +345,17 @@ extern "C" int qld_correlate_hits(qld_ctx *ctx,
     // copies; then hands block b-2's hits to the callback.  Two r^2 blocks
     // (block b is computed while block b-1's hits are written from the
     // other) and three hit slots (b, b-1, b-2 in flight).
-    float *dres[2];
-    uint32_t *dcounts;
-    if ((rc = run.dalloc((void **)&dres[0], (size_t)R * m * 4)) ||
-        (rc = run.dalloc((void **)&dres[1], (size_t)R * m * 4)) ||
-        (rc = run.dalloc((void **)&dcounts, (size_t)R * 4)))
+    if ((rc = run->res[0].reserve(ctx, (size_t)R * m * 4)) ||
+        (rc = run->res[1].reserve(ctx, (size_t)R * m * 4)) ||
+        (rc = run->counts.reserve(ctx, (size_t)R * 4)))
         return rc;
-    for (auto &sl : run.slot) {
-        if ((rc = run.dalloc((void **)&sl.d_off, (size_t)(R + 1) * 8)) ||
-            (rc = run.halloc((void **)&sl.h_off, (size_t)(R + 1) * 8)) ||
-            (rc = run.halloc((void **)&sl.h_total, 8)))
+    for (Slot &sl : run->slot)
+        if ((rc = sl.d_off.reserve(ctx, (size_t)(R + 1) * 8)) ||
+            (rc = sl.h_off.reserve(ctx, (size_t)(R + 1) * 8)) ||
+            (rc = sl.h_total.reserve(ctx, 8)))
             return rc;
-        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&sl.ev_total, hipEventDisableTiming)) != hipSuccess)
-            return qld_hip_fail(ctx, e, "event");
-    }
-    if ((e = hipStreamCreateWithFlags(&run.copy, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&run.ev_written, hipEventDisableTiming)) != hipSuccess)
-        return qld_hip_fail(ctx, e, "stream/event");
+    float *dres[2] = {(float *)run->res[0].ptr, (float *)run->res[1].ptr};
+    uint32_t *dcounts = (uint32_t *)run->counts.ptr;
 
     struct Block {
         int64_t i0;
@@ -373,7 +375,7 @@ extern "C" int qld_correlate_hits(qld_ctx *ctx,
     // and their total (to the slot's pinned word).
     auto compute = [&](int64_t b) -> int {
         const Block k = block(b);
-        Slot &sl = run.slot[b % 3];
+        Slot &sl = run->slot[b % 3];
         float *res = dres[b & 1];
         if (k.cols > 0) {
             if (upper && same) {
@@ -396,9 +398,9 @@ extern "C" int qld_correlate_hits(qld_ctx *ctx,
             }
         }
         if ((e = enqueue_count_scan(res, k.ldr, k.rows, k.cols, (int)k.i0, k.col0, upper,
-                                    r2limit, dcounts, sl.d_off, s)) != hipSuccess ||
-            (e = hipMemcpyAsync(sl.h_total, sl.d_off + k.rows, 8, hipMemcpyDeviceToHost, s)) !=
-                hipSuccess ||
+                                    r2limit, dcounts, (uint64_t *)sl.d_off.ptr, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(sl.h_total.p, (uint64_t *)sl.d_off.ptr + k.rows, 8,
+                                hipMemcpyDeviceToHost, s)) != hipSuccess ||
             (e = hipEventRecord(sl.ev_total, s)) != hipSuccess)
             return qld_hip_fail(ctx, e, "hits count");
         return QLD_OK;
@@ -406,37 +408,39 @@ extern "C" int qld_correlate_hits(qld_ctx *ctx,
     // Block b's hits into its slot's buffers, then copied to the host.
     auto write = [&](int64_t b) -> int {
         const Block k = block(b);
-        Slot &sl = run.slot[b % 3];
+        Slot &sl = run->slot[b % 3];
         if ((e = hipEventSynchronize(sl.ev_total)) != hipSuccess)
             return qld_hip_fail(ctx, e, "hits total");
-        const uint64_t hits = *sl.h_total;
-        if ((rc = run.reserve(sl, hits))) return rc;
+        const uint64_t hits = *(const uint64_t *)sl.h_total.p;
+        if ((rc = slot_reserve(ctx, sl, hits))) return rc;
+        uint64_t *d_off = (uint64_t *)sl.d_off.ptr;
+        uint32_t *d_cols = (uint32_t *)sl.d_cols.ptr;
+        float *d_r2 = (float *)sl.d_r2.ptr;
         if (hits) {
             hipLaunchKernelGGL(hits_write_kernel, dim3(k.rows), dim3(HIT_WG), 0, s, dres[b & 1],
-                               k.ldr, k.cols, (int)k.i0, k.col0, upper, r2limit, sl.d_off,
-                               sl.d_cols, sl.d_r2);
+                               k.ldr, k.cols, (int)k.i0, k.col0, upper, r2limit, d_off, d_cols,
+                               d_r2);
             if ((e = hipGetLastError()) != hipSuccess) return qld_hip_fail(ctx, e, "hits write");
         }
-        if ((e = hipEventRecord(run.ev_written, s)) != hipSuccess ||
-            (e = hipStreamWaitEvent(run.copy, run.ev_written, 0)) != hipSuccess ||
-            (e = hipMemcpyAsync(sl.h_off, sl.d_off, (size_t)(k.rows + 1) * 8,
-                                hipMemcpyDeviceToHost, run.copy)) != hipSuccess ||
-            (hits && ((e = hipMemcpyAsync(sl.h_cols, sl.d_cols, hits * 4, hipMemcpyDeviceToHost,
-                                          run.copy)) != hipSuccess ||
-                      (e = hipMemcpyAsync(sl.h_r2, sl.d_r2, hits * 4, hipMemcpyDeviceToHost,
-                                          run.copy)) != hipSuccess)) ||
-            (e = hipEventRecord(sl.done, run.copy)) != hipSuccess)
+        if ((e = hipEventRecord(run->ev_written, s)) != hipSuccess ||
+            (e = hipStreamWaitEvent(run->copy, run->ev_written, 0)) != hipSuccess ||
+            (e = hipMemcpyAsync(sl.h_off.p, d_off, (size_t)(k.rows + 1) * 8,
+                                hipMemcpyDeviceToHost, run->copy)) != hipSuccess ||
+            (hits && ((e = hipMemcpyAsync(sl.h_cols.p, d_cols, hits * 4, hipMemcpyDeviceToHost,
+                                          run->copy)) != hipSuccess ||
+                      (e = hipMemcpyAsync(sl.h_r2.p, d_r2, hits * 4, hipMemcpyDeviceToHost,
+                                          run->copy)) != hipSuccess)) ||
+            (e = hipEventRecord(sl.done, run->copy)) != hipSuccess)
             return qld_hip_fail(ctx, e, "hits D2H");
         sl.row0 = (int)k.i0;
         sl.rows = k.rows;
-        sl.pending = true;
         return QLD_OK;
     };
     auto deliver = [&](int64_t b) -> int {
-        Slot &sl = run.slot[b % 3];
+        Slot &sl = run->slot[b % 3];
         if ((e = hipEventSynchronize(sl.done)) != hipSuccess) return qld_hip_fail(ctx, e, "D2H");
-        sl.pending = false;
-        int u = fn(user, sl.row0, sl.rows, sl.h_off, sl.h_cols, sl.h_r2);
+        int u = fn(user, sl.row0, sl.rows, (const uint64_t *)sl.h_off.p,
+                   (const uint32_t *)sl.h_cols.p, (const float *)sl.h_r2.p);
         if (u) {
             qld_set_error(ctx, "hits callback returned %d", u);
             return QLD_ERR_INVALID;

@@ -152,11 +152,14 @@ extern "C" void qld_gpu_release(qld_ctx *ctx)
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    qld_hits_release(ctx);
     for (auto &kv : ctx->allocs) hipFree(kv.first);
     ctx->allocs.clear();
     ctx->ws.release();
     ctx->tiles.release();
     ctx->queue.release();
+    ctx->kpart.release();
+    ctx->kcnt.release();
     ctx->host_stage.release();
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);

@@ -150,9 +150,12 @@ long qld_write_results(FILE *fp,
 /* ---- the drop-in correlate call ------------------------------------------ */
 
 /* Formats the hits of row blocks delivered by qld_correlate_hits: rows are cut
- * into segments of about SEG_HITS lines, a group of segments is formatted in
- * parallel into per-thread buffers and the buffers are written in order. */
-enum { SEG_HITS = 1 << 18 };
+ * into segments of about an equal share of the block's lines per thread (at
+ * least SEG_MIN, at most SEG_MAX lines), a group of segments is formatted in
+ * parallel into per-thread buffers and the buffers are written in order.  (A
+ * fixed 2^18-line segment left a streamed report's blocks of about 14,000
+ * lines to one thread.) */
+enum { SEG_MIN = 1 << 11, SEG_MAX = 1 << 18 };
 
 typedef struct {
     FILE *fp;
@@ -211,13 +214,16 @@ static int write_hits(void *user, int row0, int rows, const uint64_t *off,
 {
     hits_writer *w = (hits_writer *)user;
     if (w->err || off[rows] == off[0]) return w->err;
+    uint64_t seg = (off[rows] - off[0] + (uint64_t)w->nthreads - 1) / (uint64_t)w->nthreads;
+    if (seg < SEG_MIN) seg = SEG_MIN;
+    if (seg > SEG_MAX) seg = SEG_MAX;
     int r = 0;
     while (r < rows) {
-        /* the next group: up to nthreads segments of about SEG_HITS hits */
+        /* the next group: up to nthreads segments of about `seg` hits */
         int seg_lo[64], seg_hi[64], nseg = 0;
         while (r < rows && nseg < w->nthreads) {
             int r1 = r;
-            while (r1 < rows && (r1 == r || off[r1 + 1] - off[r] <= SEG_HITS)) r1++;
+            while (r1 < rows && (r1 == r || off[r1 + 1] - off[r] <= seg)) r1++;
             seg_lo[nseg] = r;
             seg_hi[nseg] = r1;
             nseg++;
