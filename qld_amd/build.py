@@ -61,7 +61,7 @@ def build_cli() -> str:
     """qLD-compute: the drop-in command line over libqld_host + libqld_hip."""
     out = os.path.join(LIBDIR, "qLD-compute")
     tmp = out + ".tmp"
-    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter",
+    _run(["gcc", "-O2", "-std=gnu11", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter",
           f"-I{os.path.join(ROOT, 'include')}", "-o", tmp,
           os.path.join(PKG, "cli", "qld_compute.c"), f"-L{LIBDIR}", "-lqld_host",
           "-lqld_hip", "-Wl,-rpath,$ORIGIN", "-lz", "-lm"])

@@ -607,25 +607,33 @@ static void run_batch(qld_ctx *ctx, const options *o, task *const *bt, loaded *L
         uint32_t *bc = (uint32_t *)xmalloc((size_t)rows * 4);
         int64_t r = 0, off = 0;
         for (int k = 0; k < nb; k++) {
-            const qld_snp_table *A = &L[k].A, *B = table_b(&L[k]);
+            const qld_snp_table *B = table_b(&L[k]);
             memset(&w[k], 0, sizeof w[k]);
-            memcpy(snp + r * words, A->snp, (size_t)A->rows * words * 8);
-            memcpy(bc + r, A->bc, (size_t)A->rows * 4);
             w[k].a_start = (int32_t)r;
-            w[k].a_size = A->rows;
-            r += A->rows;
+            w[k].a_size = L[k].A.rows;
+            r += L[k].A.rows;
             if (L[k].single) {
                 w[k].b_start = w[k].a_start;
                 w[k].tri = 1;
             } else {
-                memcpy(snp + r * words, B->snp, (size_t)B->rows * words * 8);
-                memcpy(bc + r, B->bc, (size_t)B->rows * 4);
                 w[k].b_start = (int32_t)r;
                 r += B->rows;
             }
             w[k].b_size = B->rows;
             w[k].out_offset = off;
             off += task_cells(&L[k]);
+        }
+        /* the tasks' rows side by side (overlapping windows repeat theirs:
+         * about a gigabyte for the benchmark's list), copied in parallel */
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int k = 0; k < nb; k++) {
+            const qld_snp_table *A = &L[k].A, *B = table_b(&L[k]);
+            memcpy(snp + (int64_t)w[k].a_start * words, A->snp, (size_t)A->rows * words * 8);
+            memcpy(bc + w[k].a_start, A->bc, (size_t)A->rows * 4);
+            if (!L[k].single) {
+                memcpy(snp + (int64_t)w[k].b_start * words, B->snp, (size_t)B->rows * words * 8);
+                memcpy(bc + w[k].b_start, B->bc, (size_t)B->rows * 4);
+            }
         }
         gpu_check(ctx, qld_table_create(ctx, snp, bc, (int)rows, words, snp_size, &tab),
                   "table upload");
@@ -637,48 +645,94 @@ static void run_batch(qld_ctx *ctx, const options *o, task *const *bt, loaded *L
         gpu_check(ctx, qld_plan_run(ctx, plan), "batch run");
         gpu_check(ctx, qld_sync(ctx), "batch run");
         gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_rows_b + 1) * 4, &dcnt), "allocation");
-        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_rows_b + 1) * 8, &doff), "allocation");
-        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_cells ? max_cells : 1) * 4, &dcols),
-                  "allocation");
-        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)(max_cells ? max_cells : 1) * 4, &dr2),
-                  "allocation");
     }
-    uint64_t *hoff = (uint64_t *)xmalloc((size_t)(max_rows_b + 1) * 8);
-    uint64_t hcap = 0;
+    /* Selection: every task's hits go into one device buffer (offsets per
+     * task), flushed when the next task's would not fit; a flush copies the
+     * buffer to the host in one transfer and formats and writes the pending
+     * tasks' reports in parallel, one task per thread (tasks are a few
+     * thousand lines each: too few to split one over the threads). */
+    int64_t off_rows = 0;
+    int64_t *obase = (int64_t *)xmalloc((size_t)(nb ? nb : 1) * sizeof(int64_t));
+    for (int k = 0; k < nb; k++) {
+        obase[k] = off_rows;
+        off_rows += table_b(&L[k])->rows + 1;
+    }
+    const uint64_t cap = (uint64_t)(max_cells > (1 << 26) ? max_cells : (1 << 26));
+    if (rows > 0) {
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)off_rows * 8, &doff), "allocation");
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)cap * 4, &dcols), "allocation");
+        gpu_check(ctx, qld_dev_alloc(ctx, (size_t)cap * 4, &dr2), "allocation");
+    }
+    uint64_t *hoff = (uint64_t *)xmalloc((size_t)(off_rows ? off_rows : 1) * 8);
+    uint64_t *hbase = (uint64_t *)xmalloc((size_t)(nb ? nb : 1) * sizeof(uint64_t));
+    uint64_t hcap = 0, used = 0;
     uint32_t *hcols = NULL;
     float *hr2 = NULL;
-    for (int k = 0; k < nb; k++) {
-        const task *t = bt[k];
-        const qld_snp_table *A = &L[k].A, *B = table_b(&L[k]);
-        FILE *fp = fopen(t->output, "wb");
-        if (!fp) die("\n ERROR: cannot create output file %s\n\n", t->output);
-        qld_write_results_header(fp);
-        if (task_cells(&L[k]) > 0) {
-            uint64_t total = 0;
-            gpu_check(ctx, qld_compact_dev(ctx, (float *)dout + w[k].out_offset, A->rows, B->rows,
-                                           A->rows, 0, 0, L[k].upper, o->r2limit,
-                                           (uint32_t *)dcnt, (uint64_t *)doff, (uint32_t *)dcols,
-                                           (float *)dr2, (uint64_t)max_cells, &total),
-                      "report selection");
-            if (total > hcap) {
-                free(hcols);
-                free(hr2);
-                hcap = total + total / 4;
-                hcols = (uint32_t *)xmalloc(hcap * 4);
-                hr2 = (float *)xmalloc(hcap * 4);
-            }
-            gpu_check(ctx, qld_d2h(ctx, hoff, doff, (size_t)(B->rows + 1) * 8), "D2H");
-            if (total) {
-                gpu_check(ctx, qld_d2h(ctx, hcols, dcols, total * 4), "D2H");
-                gpu_check(ctx, qld_d2h(ctx, hr2, dr2, total * 4), "D2H");
-                if (qld_write_hits(fp, A->pos, A->id, A->bc, A->rows, B->pos, B->id, B->bc,
-                                   B->rows, snp_size, 0, B->rows, hoff, hcols, hr2) < 0)
-                    die("\n ERROR: write error on %s\n\n", t->output);
-            }
+    int first = 0;      /* the pending tasks are first .. k-1 */
+    for (int k = 0; k <= nb; k++) {
+        uint64_t total = 0;
+        int rc = QLD_OK;
+        if (k < nb) {
+            const qld_snp_table *A = &L[k].A, *B = table_b(&L[k]);
+            hbase[k] = used;
+            if (task_cells(&L[k]) > 0)
+                rc = qld_compact_dev(ctx, (float *)dout + w[k].out_offset, A->rows, B->rows,
+                                     A->rows, 0, 0, L[k].upper, o->r2limit, (uint32_t *)dcnt,
+                                     (uint64_t *)doff + obase[k], (uint32_t *)dcols + used,
+                                     (float *)dr2 + used, cap - used, &total);
+            if (rc != QLD_OK && rc != QLD_ERR_BUDGET) gpu_check(ctx, rc, "report selection");
         }
-        if (fclose(fp) != 0) die("\n ERROR: write error on %s\n\n", t->output);
+        if (k == nb || rc == QLD_ERR_BUDGET) {
+            /* flush tasks first .. k-1 */
+            if (k > first) {
+                const int64_t r0 = obase[first];
+                const int64_t r1 = k < nb ? obase[k] : off_rows;
+                if (doff)
+                    gpu_check(ctx, qld_d2h(ctx, hoff + r0, (uint64_t *)doff + r0,
+                                           (size_t)(r1 - r0) * 8), "D2H");
+                if (used > hcap) {
+                    free(hcols);
+                    free(hr2);
+                    hcap = used + used / 4;
+                    hcols = (uint32_t *)xmalloc(hcap * 4);
+                    hr2 = (float *)xmalloc(hcap * 4);
+                }
+                if (used) {
+                    gpu_check(ctx, qld_d2h(ctx, hcols, dcols, used * 4), "D2H");
+                    gpu_check(ctx, qld_d2h(ctx, hr2, dr2, used * 4), "D2H");
+                }
+                int failed = -1;
+#pragma omp parallel for schedule(dynamic, 1)
+                for (int t = first; t < k; t++) {
+                    const qld_snp_table *A = &L[t].A, *B = table_b(&L[t]);
+                    FILE *fp = fopen(bt[t]->output, "wb");
+                    int ok = fp != NULL;
+                    if (ok) qld_write_results_header(fp);
+                    if (ok && task_cells(&L[t]) > 0 && hoff[obase[t] + B->rows] > 0)
+                        ok = qld_write_hits(fp, A->pos, A->id, A->bc, A->rows, B->pos, B->id,
+                                            B->bc, B->rows, snp_size, 0, B->rows,
+                                            hoff + obase[t], hcols + hbase[t],
+                                            hr2 + hbase[t]) >= 0;
+                    if (fp && fclose(fp) != 0) ok = 0;
+                    if (!ok) {
+#pragma omp critical
+                        if (failed < 0 || t < failed) failed = t;
+                    }
+                }
+                if (failed >= 0)
+                    die("\n ERROR: cannot write output file %s\n\n", bt[failed]->output);
+            }
+            if (k == nb) break;
+            first = k;
+            used = 0;
+            k--;        /* the task that did not fit, again at the buffer's start */
+            continue;
+        }
+        used += total;
     }
     free(hoff);
+    free(hbase);
+    free(obase);
     free(hcols);
     free(hr2);
     free(w);
