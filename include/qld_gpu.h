@@ -183,7 +183,8 @@ int qld_set_fp4_operands(qld_ctx *ctx, int mode);
  * else 128; expanded operands: 128). */
 int qld_set_plain_tile(qld_ctx *ctx, int tile);
 
-/* Split-K for plain calls on 128-tiles of the direct fp4 engine: each tile's
+/* Split-K for plain calls on 128-tiles of the direct fp4 engine (and for the
+ * serving batches and one-launch-per-window plans of the same context): each tile's
  * word groups are summed by `chunks` workgroups and the last to finish adds
  * the partial sums (exact integers) and writes the tile.  0 (default): by
  * shape -- the chunk count that a cost model of the tiles' k-loop chains and
