@@ -168,3 +168,19 @@ def test_partial_sum_width_bounds(groups, chunks, engine_factory):
     assert e.last_ksplit == chunks
     assert cnt[0, 1] == N
     _check(res, cnt, A, bc, A, bc, N, True, "exact")
+
+
+def test_timing_switch(engine_factory):
+    """qld_set_timing: with the per-call events off a call records no kernel
+    time (asking for it is an error); on again, the next call does."""
+    e = engine_factory(engine="mfma_fp4", r2_mode="exact", timing=True)
+    A, bc = qld_amd.synth_blocks(3, 500, 16622)
+    e.correlate(A, bc, snp_size=16622, triangular=True)
+    assert e.last_contract_ms() > 0
+    e.set_timing(False)
+    e.correlate(A, bc, snp_size=16622, triangular=True)
+    with pytest.raises(qld_amd.QldError):
+        e.last_contract_ms()
+    e.set_timing(True)
+    e.correlate(A, bc, snp_size=16622, triangular=True)
+    assert e.last_contract_ms() > 0
